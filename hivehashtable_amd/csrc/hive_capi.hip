@@ -1,0 +1,556 @@
+// hive_capi.hip -- host runtime of libhive.so: table lifecycle, launches of the
+// bulk / mixed / maintenance kernels, and the drop-in dispatch pair with the
+// reference's host semantics (src/hive_dispatch.cu:29-315,317-549,551-872).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hive.h"
+#include "hive_kernels.hpp"
+
+using namespace hive;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+hive_status_t fail(hive_status_t s, const std::string& what) {
+  g_last_error = what;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(HIVE_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define HIP_CHECK_LAUNCH() HIP_TRY(hipGetLastError())
+
+constexpr uint32_t kDefaultBlock = 256;
+
+}  // namespace
+
+struct hive_table {
+  int device = 0;
+  int policy = 0;
+  TableView v{};
+  size_t bytes = 0;
+  hive_launch_t launch[3] = {{kDefaultBlock, 1}, {kDefaultBlock, 1}, {kDefaultBlock, 1}};
+  unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
+};
+
+namespace {
+
+size_t flag_words(uint32_t nb) { return (static_cast<size_t>(nb) + 31) / 32; }
+size_t counter_words() { return static_cast<size_t>(2) * kCounterLanes * kCounterStride; }
+
+void free_view(TableView& v) {
+  if (v.slots) (void)hipFree(v.slots);
+  if (v.stash) (void)hipFree(v.stash);
+  if (v.stash_count) (void)hipFree(v.stash_count);
+  if (v.stash_flag) (void)hipFree(v.stash_flag);
+  if (v.locks) (void)hipFree(v.locks);
+  if (v.counters) (void)hipFree(v.counters);
+  v = TableView{};
+}
+
+size_t view_bytes(uint32_t nb) {
+  return static_cast<size_t>(nb) * kSlots * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
+         static_cast<size_t>(nb) * 4 * 2 + flag_words(nb) * 4 + counter_words() * 8;
+}
+
+hive_status_t alloc_view(TableView& v, uint32_t nb) {
+  v = TableView{};
+  v.nb = nb;
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kSlots * 8)) != hipSuccess ||
+      (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
+      (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.stash_flag, flag_words(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.counters, counter_words() * 8)) != hipSuccess) {
+    free_view(v);
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return HIVE_OK;
+}
+
+unsigned fill_grid(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  return static_cast<unsigned>(g ? g : 1);
+}
+
+// zero buckets, stash pointers and flags, locks and counters (stash slots at or
+// above a bucket's count are never read, so the stash body is left as is).
+hive_status_t clear_view(const TableView& v, hipStream_t s) {
+  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
+  k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
+  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
+  k_fill_u32<<<fill_grid(flag_words(v.nb)), 256, 0, s>>>(v.stash_flag, flag_words(v.nb), 0u);
+  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.locks, v.nb, 0u);
+  k_fill_u64<<<fill_grid(counter_words()), 256, 0, s>>>(v.counters, counter_words(), 0ull);
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+#define HIVE_POLICY_SWITCH(policy, BODY)                   \
+  switch (policy) {                                        \
+    case HIVE_POLICY_DEFAULT2HASH: { using P = PolicyDefault2; BODY; } break;       \
+    case HIVE_POLICY_MURMURCITY: { using P = PolicyMurmurCity; BODY; } break;       \
+    case HIVE_POLICY_TRIPLE: { using P = PolicyTriple; BODY; } break;               \
+    case HIVE_POLICY_MURMURCITYBIT: { using P = PolicyMurmurCityBit; BODY; } break; \
+    case HIVE_POLICY_LOOKUP2: { using P = PolicyLookup2; BODY; } break;             \
+    case HIVE_POLICY_LOOKUP3: { using P = PolicyLookup3; BODY; } break;             \
+    default: return fail(HIVE_EINVAL, "unknown policy");   \
+  }
+
+unsigned grid_for(size_t n, uint32_t block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+bool valid_block(uint32_t b) { return b >= 64 && b <= static_cast<uint32_t>(kMaxBlock) && b % 64 == 0; }
+
+template <class Src, class Out>
+hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_INSERT].block_threads;
+  HIVE_POLICY_SWITCH(t->policy, (k_insert<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+template <class Src, class Out>
+hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
+  HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+template <class Src, class Out>
+hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_DELETE].block_threads;
+  HIVE_POLICY_SWITCH(t->policy, (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+// a mixed batch = three stream-ordered phases: lookups, deletes, inserts
+template <class Src, class Out>
+hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_t s) {
+  hive_status_t r;
+  if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;
+  if ((r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
+  return launch_insert(t, src, n, out, OP_INSERT, s);
+}
+
+hive_status_t check_table(const hive_table* t) {
+  if (!t) return fail(HIVE_EINVAL, "null table");
+  return HIVE_OK;
+}
+
+hive_status_t set_device(const hive_table* t) {
+  HIP_TRY(hipSetDevice(t->device));
+  return HIVE_OK;
+}
+
+hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_t* st) {
+  std::vector<unsigned long long> h(counter_words());
+  HIP_TRY(hipMemcpyAsync(h.data(), t->v.counters, counter_words() * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  unsigned long long a = 0, b = 0;
+  for (int l = 0; l < kCounterLanes; ++l) {
+    a += h[(0 * kCounterLanes + l) * kCounterStride];
+    b += h[(1 * kCounterLanes + l) * kCounterStride];
+  }
+  *tab = a;
+  *st = b;
+  return HIVE_OK;
+}
+
+hive_status_t count_exact_view(hive_table* t, const TableView& v, hipStream_t s, uint64_t* tab, uint64_t* st) {
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
+  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
+  k_count_exact<<<fill_grid(ns), 256, 0, s>>>(v, t->scratch);
+  HIP_CHECK_LAUNCH();
+  unsigned long long h[2];
+  HIP_TRY(hipMemcpyAsync(h, t->scratch, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *tab = h[0];
+  *st = h[1];
+  return HIVE_OK;
+}
+
+// set the spread counters to exact totals (lane 0 holds everything)
+hive_status_t reset_counters(hive_table* t, const TableView& v, hipStream_t s, uint64_t tab, uint64_t st) {
+  std::vector<unsigned long long> h(counter_words(), 0ull);
+  h[0] = tab;
+  h[static_cast<size_t>(kCounterLanes) * kCounterStride] = st;
+  HIP_TRY(hipMemcpyAsync(v.counters, h.data(), counter_words() * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  (void)t;
+  return HIVE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hive_status_string(hive_status_t s) {
+  switch (s) {
+    case HIVE_OK: return "ok";
+    case HIVE_EINVAL: return "invalid argument";
+    case HIVE_ENOMEM: return "out of memory";
+    case HIVE_EDEVICE: return "device error";
+    case HIVE_ECAPACITY: return "capacity exceeded";
+    case HIVE_ERANGE: return "output buffer too small";
+  }
+  return "unknown status";
+}
+
+const char* hive_last_error(void) { return g_last_error.c_str(); }
+
+hive_status_t hive_policy_from_name(const char* name, int* policy) {
+  static const char* names[] = {"Default2Hash", "MurmurCityHash", "TripleHash",
+                                "MurmurCityBitHash", "Lookup2Hash", "Lookup3Hash"};
+  if (!name || !policy) return fail(HIVE_EINVAL, "null argument");
+  for (int i = 0; i < 6; ++i)
+    if (std::strcmp(name, names[i]) == 0) {
+      *policy = i;
+      return HIVE_OK;
+    }
+  return fail(HIVE_EINVAL, std::string("unknown hash policy: ") + name);
+}
+
+hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
+  if (!out) return fail(HIVE_EINVAL, "null out");
+  *out = nullptr;
+  if (policy < 0 || policy > HIVE_POLICY_LOOKUP3) return fail(HIVE_EINVAL, "unknown policy");
+  const uint64_t nb = num_slots / kSlots;
+  if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "num_slots must give 1 .. 2^32-1 buckets");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  auto* t = new (std::nothrow) hive_table();
+  if (!t) return fail(HIVE_ENOMEM, "host allocation");
+  t->device = dev;
+  t->policy = policy;
+  hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb));
+  if (r != HIVE_OK) {
+    delete t;
+    return r;
+  }
+  if (hipMalloc(&t->scratch, 64) != hipSuccess) {
+    free_view(t->v);
+    delete t;
+    return fail(HIVE_ENOMEM, "hipMalloc scratch");
+  }
+  t->bytes = view_bytes(static_cast<uint32_t>(nb)) + 16;
+  if ((r = clear_view(t->v, nullptr)) != HIVE_OK || hipStreamSynchronize(nullptr) != hipSuccess) {
+    free_view(t->v);
+    (void)hipFree(t->scratch);
+    delete t;
+    return r != HIVE_OK ? r : fail(HIVE_EDEVICE, "clear at create");
+  }
+  *out = t;
+  return HIVE_OK;
+}
+
+hive_status_t hive_destroy(hive_table_t* t) {
+  if (!t) return HIVE_OK;
+  (void)hipSetDevice(t->device);
+  (void)hipDeviceSynchronize();
+  free_view(t->v);
+  if (t->scratch) (void)hipFree(t->scratch);
+  delete t;
+  return HIVE_OK;
+}
+
+hive_status_t hive_clear(hive_table_t* t, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  return clear_view(t->v, static_cast<hipStream_t>(stream));
+}
+
+uint64_t hive_num_buckets(const hive_table_t* t) { return t ? t->v.nb : 0; }
+
+hive_status_t hive_insert(hive_table_t* t, const uint32_t* keys, const uint32_t* vals, size_t n, uint8_t* status,
+                          void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !keys) return fail(HIVE_EINVAL, "null keys");
+  return launch_insert(t, SrcNative{keys, vals, nullptr}, n, OutNative{status, nullptr}, -1,
+                       static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_lookup(hive_table_t* t, const uint32_t* keys, size_t n, uint32_t* vals_out, uint8_t* status,
+                          void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !keys) return fail(HIVE_EINVAL, "null keys");
+  return launch_lookup(t, SrcNative{keys, nullptr, nullptr}, n, OutNative{status, vals_out}, -1,
+                       static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_erase(hive_table_t* t, const uint32_t* keys, size_t n, uint8_t* status, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !keys) return fail(HIVE_EINVAL, "null keys");
+  return launch_erase(t, SrcNative{keys, nullptr, nullptr}, n, OutNative{status, nullptr}, -1,
+                      static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_mixed(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals, size_t n,
+                         uint32_t* out_vals, uint8_t* status, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && (!keys || !types)) return fail(HIVE_EINVAL, "null keys or types");
+  return launch_mixed(t, SrcNative{keys, vals, types}, n, OutNative{status, out_vals},
+                      static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_insert_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results,
+                              void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !ops) return fail(HIVE_EINVAL, "null ops");
+  return launch_insert(t, SrcOps{reinterpret_cast<const Op16*>(ops)}, n,
+                       OutU64{reinterpret_cast<unsigned long long*>(results)}, -1, static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_lookup_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results,
+                              void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !ops) return fail(HIVE_EINVAL, "null ops");
+  return launch_lookup(t, SrcOps{reinterpret_cast<const Op16*>(ops)}, n,
+                       OutU64{reinterpret_cast<unsigned long long*>(results)}, -1, static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results,
+                             void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && !ops) return fail(HIVE_EINVAL, "null ops");
+  return launch_mixed(t, SrcOps{reinterpret_cast<const Op16*>(ops)}, n,
+                      OutU64{reinterpret_cast<unsigned long long*>(results)}, static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  const uint64_t nb = new_num_slots / kSlots;
+  if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "new size must give 1 .. 2^32-1 buckets");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TableView nv;
+  if ((r = alloc_view(nv, static_cast<uint32_t>(nb))) != HIVE_OK) return r;
+  if ((r = clear_view(nv, s)) != HIVE_OK) {
+    free_view(nv);
+    return r;
+  }
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
+  const size_t nsrc = 2 * static_cast<size_t>(t->v.nb) * kSlots;
+  const uint32_t blk = kDefaultBlock;
+  HIVE_POLICY_SWITCH(t->policy, (k_rehash<P><<<grid_for(nsrc, blk), blk, 0, s>>>(nv, t->v, t->scratch)));
+  if (hipGetLastError() != hipSuccess) {
+    free_view(nv);
+    return fail(HIVE_EDEVICE, "rehash launch");
+  }
+  unsigned long long failed = 0;
+  if (hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    free_view(nv);
+    return fail(HIVE_EDEVICE, "rehash sync");
+  }
+  if (failed) {
+    free_view(nv);
+    return fail(HIVE_ECAPACITY, "resize target cannot hold every live entry");
+  }
+  free_view(t->v);
+  t->v = nv;
+  t->bytes = view_bytes(static_cast<uint32_t>(nb)) + 16;
+  return HIVE_OK;
+}
+
+hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
+  k_stash_count_live<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, t->scratch);
+  HIP_CHECK_LAUNCH();
+  unsigned long long total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, t->scratch, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n_out) *n_out = static_cast<size_t>(total);
+  if (total > cap) return fail(HIVE_ERANGE, "stash drain: output buffer too small");
+  if (total) {
+    if (!out_kv) return fail(HIVE_EINVAL, "null out_kv");
+    HIP_TRY(hipMemsetAsync(t->scratch, 0, 8, s));
+    k_stash_compact<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, reinterpret_cast<unsigned long long*>(out_kv),
+                                                            t->scratch);
+    HIP_CHECK_LAUNCH();
+  }
+  k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.stash_count, t->v.nb, 0u);
+  k_fill_u32<<<fill_grid(flag_words(t->v.nb)), 256, 0, s>>>(t->v.stash_flag, flag_words(t->v.nb), 0u);
+  HIP_CHECK_LAUNCH();
+  uint64_t tab = 0, st = 0;
+  if ((r = read_counters(t, s, &tab, &st)) != HIVE_OK) return r;
+  return reset_counters(t, t->v, s, tab, 0);
+}
+
+hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t cap = static_cast<size_t>(t->v.nb) * kStashSlots;
+  unsigned long long* buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, cap * 8));
+  size_t n = 0;
+  r = hive_stash_drain(t, reinterpret_cast<uint64_t*>(buf), cap, &n, stream);
+  if (r == HIVE_OK && n) {
+    unsigned long long* failed = t->scratch + 1;
+    if (hipMemsetAsync(failed, 0, 8, s) != hipSuccess) r = fail(HIVE_EDEVICE, "memset");
+    if (r == HIVE_OK) r = launch_insert(t, SrcKV{buf}, n, OutCount{failed}, -1, s);
+    unsigned long long nf = 0;
+    if (r == HIVE_OK && (hipMemcpyAsync(&nf, failed, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                         hipStreamSynchronize(s) != hipSuccess))
+      r = fail(HIVE_EDEVICE, "reinsert sync");
+    if (r == HIVE_OK && nf) r = fail(HIVE_ECAPACITY, "stash re-insert: entries could not be placed");
+  }
+  (void)hipFree(buf);
+  if (n_reinserted) *n_reinserted = n;
+  return r;
+}
+
+hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (!out) return fail(HIVE_EINVAL, "null out");
+  uint64_t tab = 0, st = 0;
+  if ((r = read_counters(t, static_cast<hipStream_t>(stream), &tab, &st)) != HIVE_OK) return r;
+  out->num_buckets = t->v.nb;
+  out->slots_per_bucket = kSlots;
+  out->capacity = static_cast<uint64_t>(t->v.nb) * kSlots;
+  out->table_live = tab;
+  out->stash_live = st;
+  out->load_factor = static_cast<double>(tab) / static_cast<double>(out->capacity);
+  out->device_bytes = t->bytes;
+  out->policy = t->policy;
+  return HIVE_OK;
+}
+
+hive_status_t hive_count_exact(hive_table_t* t, uint64_t* table_live, uint64_t* stash_live, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  uint64_t a = 0, b = 0;
+  if ((r = count_exact_view(t, t->v, static_cast<hipStream_t>(stream), &a, &b)) != HIVE_OK) return r;
+  if (table_live) *table_live = a;
+  if (stash_live) *stash_live = b;
+  return HIVE_OK;
+}
+
+hive_status_t hive_probe_stats(hive_table_t* t, const uint32_t* keys, size_t n, uint64_t* candidate_buckets,
+                               uint64_t* lookup_bucket_reads, uint64_t* lookup_stash_reads, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (n && !keys) return fail(HIVE_EINVAL, "null keys");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 24, s));
+  if (n) {
+    HIVE_POLICY_SWITCH(t->policy, (k_probe_stats<P><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, t->scratch)));
+    HIP_CHECK_LAUNCH();
+  }
+  unsigned long long h[3];
+  HIP_TRY(hipMemcpyAsync(h, t->scratch, 24, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (candidate_buckets) *candidate_buckets = h[0];
+  if (lookup_bucket_reads) *lookup_bucket_reads = h[1];
+  if (lookup_stash_reads) *lookup_stash_reads = h[2];
+  return HIVE_OK;
+}
+
+hive_status_t hive_export_raw(hive_table_t* t, uint64_t* h_slots, uint64_t* h_stash, uint32_t* h_stash_count,
+                              void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t nb = t->v.nb;
+  if (h_slots) HIP_TRY(hipMemcpyAsync(h_slots, t->v.slots, nb * kSlots * 8, hipMemcpyDeviceToHost, s));
+  if (h_stash) HIP_TRY(hipMemcpyAsync(h_stash, t->v.stash, nb * kStashSlots * 8, hipMemcpyDeviceToHost, s));
+  if (h_stash_count) HIP_TRY(hipMemcpyAsync(h_stash_count, t->v.stash_count, nb * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return HIVE_OK;
+}
+
+hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const uint64_t* h_stash,
+                              const uint32_t* h_stash_count, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (!h_slots || !h_stash || !h_stash_count) return fail(HIVE_EINVAL, "null raw array");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t nb = t->v.nb;
+  for (size_t b = 0; b < nb; ++b)
+    if (h_stash_count[b] > static_cast<uint32_t>(kStashSlots)) return fail(HIVE_EINVAL, "stash count above 8");
+  HIP_TRY(hipMemcpyAsync(t->v.slots, h_slots, nb * kSlots * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(t->v.stash, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(t->v.stash_count, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));
+  k_rebuild_flags<<<grid_for(flag_words(t->v.nb), 256), 256, 0, s>>>(t->v);
+  k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.locks, t->v.nb, 0u);
+  HIP_CHECK_LAUNCH();
+  uint64_t a = 0, b = 0;
+  if ((r = count_exact_view(t, t->v, s, &a, &b)) != HIVE_OK) return r;
+  return reset_counters(t, t->v, s, a, b);
+}
+
+hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t* cfg) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (!cfg || op_type < 0 || op_type > 2) return fail(HIVE_EINVAL, "bad op type or null config");
+  hive_launch_t c = *cfg;
+  if (c.block_threads == 0) c.block_threads = kDefaultBlock;
+  if (c.keys_per_lane == 0) c.keys_per_lane = 1;
+  if (!valid_block(c.block_threads)) return fail(HIVE_EINVAL, "block_threads must be a multiple of 64 in 64..1024");
+  if (c.keys_per_lane != 1) return fail(HIVE_EINVAL, "keys_per_lane must be 1");
+  t->launch[op_type] = c;
+  return HIVE_OK;
+}
+
+hive_status_t hive_get_launch(const hive_table_t* t, int op_type, hive_launch_t* cfg) {
+  if (!t || !cfg || op_type < 0 || op_type > 2) return fail(HIVE_EINVAL, "bad argument");
+  *cfg = t->launch[op_type];
+  return HIVE_OK;
+}
+
+hive_status_t hive_ceiling_gather(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed,
+                                  uint32_t* sink, void* stream) {
+  const uint64_t nbk = footprint_bytes / 64;
+  if (!buf || nbk == 0 || nbk > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "bad ceiling buffer");
+  if (n == 0) return HIVE_OK;
+  k_ceiling_gather<<<grid_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      reinterpret_cast<const unsigned long long*>(buf), static_cast<uint32_t>(nbk), n, seed, sink);
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed, void* stream) {
+  const uint64_t nw = footprint_bytes / 8;
+  if (!buf || nw == 0) return fail(HIVE_EINVAL, "bad ceiling buffer");
+  if (n == 0) return HIVE_OK;
+  k_ceiling_cas<<<grid_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      reinterpret_cast<unsigned long long*>(buf), nw, n, seed);
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+}  // extern "C"

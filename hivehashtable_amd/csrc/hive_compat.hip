@@ -1,0 +1,223 @@
+// hive_compat.hip -- the reference's host dispatch pair over libhive.so.
+//
+// hive_hash_table_kernel_dispatch restates hash_table_kernel_dispatch_t +
+// run_hash_table_benchmark (src/hive_dispatch.cu:551-813, :29-315):
+//   table per call (num_buckets = table_size / 8); all-LOOKUP or all-DELETE
+//   batches are prefilled with INSERTs of the same keys through the mixed
+//   path; 5 warm-up launches (none for all-DELETE, which runs once); the table
+//   is cleared after warm-up unless the batch is all-LOOKUP/all-DELETE; then
+//   num_iterations timed launches, after each of which the stash is drained
+//   and its keys appended to the next launch as INSERT ops.  Results are the
+//   last launch's 0/1 values for the first num_ops ops; elapsed_ms is the mean
+//   kernel time per timed launch.
+// Deliberate differences: the stash is cleared together with the table
+// (hiveHashTableClear leaves it, src/hive_memory.cu:203-218), unknown policy
+// names are an error, and the 30%-of-free-HBM pre-reservation
+// (include/hive_hash_table_struct.cuh:38-48) is not made.
+//
+// hive_hash_table_kernel_dispatch_ycsb restates hash_table_kernel_dispatch_YCSB
+// (src/hive_dispatch.cu:317-549): prefill with the bulk insert kernel, then 5
+// warm-up and num_iterations timed mixed launches on the same table, no clear
+// and no stash drain; policy Default2Hash.
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hive.h"
+#include "hive_device.hpp"
+
+namespace {
+
+__global__ void k_kv_to_ops(const unsigned long long* kv, size_t n, hive::Op16* ops) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ops[i].type = HIVE_OP_INSERT;
+  ops[i].pad = 0;
+  ops[i].key = static_cast<unsigned long long>(hive::kv_key(kv[i]));
+}
+
+__global__ void k_fill_ops(hive::Op16* ops, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ops[i].type = HIVE_OP_NONE;
+  ops[i].pad = 0;
+  ops[i].key = 0;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Table {
+  hive_table_t* t = nullptr;
+  ~Table() {
+    if (t) hive_destroy(t);
+  }
+};
+
+struct Events {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~Events() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+enum class Kind { Lookup, Insert, Mixed };
+
+hive_status_t run_kind(hive_table_t* t, Kind k, const hive_operation_t* ops, size_t n, uint64_t* res, hipStream_t s) {
+  switch (k) {
+    case Kind::Lookup: return hive_lookup_ops(t, ops, n, res, s);
+    case Kind::Insert: return hive_insert_ops(t, ops, n, res, s);
+    default: return hive_mixed_ops(t, ops, n, res, s);
+  }
+}
+
+#define CK(expr)                                  \
+  do {                                            \
+    hive_status_t r_ = (expr);                    \
+    if (r_ != HIVE_OK) return r_;                 \
+  } while (0)
+#define HK(expr)                                  \
+  do {                                            \
+    if ((expr) != hipSuccess) return HIVE_EDEVICE; \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+hive_status_t hive_hash_table_kernel_dispatch(const hive_operation_t* h_ops, size_t num_ops, size_t table_size,
+                                              size_t threads_per_block, size_t num_blocks, size_t num_iterations,
+                                              double* elapsed_ms, uint64_t* h_results, int verify, int layout,
+                                              const char* hash_policy) {
+  (void)num_blocks;
+  (void)verify;
+  int policy = 0;
+  CK(hive_policy_from_name(hash_policy ? hash_policy : "Default2Hash", &policy));
+  if (layout < 0 || layout > 2) return HIVE_EINVAL;  // the WarpSpeed competitor layout is not provided
+  if (num_ops && (!h_ops || !h_results)) return HIVE_EINVAL;
+  if (table_size / 8 == 0) return HIVE_EINVAL;
+  bool all_lookup = num_ops > 0, all_delete = num_ops > 0, all_insert = num_ops > 0;
+  for (size_t i = 0; i < num_ops; ++i) {
+    all_lookup &= h_ops[i].type == HIVE_OP_LOOKUP;
+    all_delete &= h_ops[i].type == HIVE_OP_DELETE;
+    all_insert &= h_ops[i].type == HIVE_OP_INSERT;
+  }
+  Table tb;
+  CK(hive_create(table_size, policy, &tb.t));
+  if (threads_per_block >= 64 && threads_per_block <= 1024 && threads_per_block % 64 == 0) {
+    hive_launch_t cfg{static_cast<uint32_t>(threads_per_block), 1};
+    for (int op = 0; op < 3; ++op) CK(hive_set_launch(tb.t, op, &cfg));
+  }
+  const size_t stash_cap = static_cast<size_t>(hive_num_buckets(tb.t)) * 8;
+  const size_t max_ops = num_ops + stash_cap;
+  DevBuf d_ops, d_res, d_flat, d_pre;
+  HK(hipMalloc(&d_ops.p, max_ops * sizeof(hive_operation_t)));
+  HK(hipMalloc(&d_res.p, max_ops * sizeof(uint64_t)));
+  HK(hipMalloc(&d_flat.p, stash_cap * sizeof(uint64_t)));
+  hipStream_t s = nullptr;
+  auto* ops = static_cast<hive_operation_t*>(d_ops.p);
+  auto* res = static_cast<uint64_t*>(d_res.p);
+  k_fill_ops<<<(max_ops + 255) / 256, 256, 0, s>>>(reinterpret_cast<hive::Op16*>(ops), max_ops);
+  HK(hipGetLastError());
+  HK(hipMemsetAsync(res, 0, max_ops * sizeof(uint64_t), s));
+  if (num_ops) HK(hipMemcpyAsync(ops, h_ops, num_ops * sizeof(hive_operation_t), hipMemcpyHostToDevice, s));
+  if (all_lookup || all_delete) {
+    std::vector<hive_operation_t> pre(h_ops, h_ops + num_ops);
+    for (auto& o : pre) o.type = HIVE_OP_INSERT;
+    HK(hipMalloc(&d_pre.p, num_ops * sizeof(hive_operation_t)));
+    HK(hipMemcpyAsync(d_pre.p, pre.data(), num_ops * sizeof(hive_operation_t), hipMemcpyHostToDevice, s));
+    CK(hive_mixed_ops(tb.t, static_cast<hive_operation_t*>(d_pre.p), num_ops, nullptr, s));
+  }
+  const Kind kind = all_lookup ? Kind::Lookup : (all_insert ? Kind::Insert : Kind::Mixed);
+  const size_t warmup = all_delete ? 0 : 5;
+  const size_t iters = all_delete ? 1 : num_iterations;
+  for (size_t w = 0; w < warmup; ++w) CK(run_kind(tb.t, kind, ops, num_ops, res, s));
+  HK(hipStreamSynchronize(s));
+  if (!all_lookup && !all_delete) CK(hive_clear(tb.t, s));
+  Events ev;
+  HK(hipEventCreate(&ev.a));
+  HK(hipEventCreate(&ev.b));
+  double total_ms = 0.0;
+  size_t n_cur = num_ops;
+  for (size_t it = 0; it < iters; ++it) {
+    HK(hipEventRecord(ev.a, s));
+    CK(run_kind(tb.t, kind, ops, n_cur, res, s));
+    HK(hipEventRecord(ev.b, s));
+    HK(hipEventSynchronize(ev.b));
+    float ms = 0.f;
+    HK(hipEventElapsedTime(&ms, ev.a, ev.b));
+    total_ms += ms;
+    if (it + 1 < iters) {  // stash entries become INSERT ops of the next launch
+      size_t drained = 0;
+      CK(hive_stash_drain(tb.t, static_cast<uint64_t*>(d_flat.p), stash_cap, &drained, s));
+      if (drained)
+        k_kv_to_ops<<<(drained + 255) / 256, 256, 0, s>>>(static_cast<unsigned long long*>(d_flat.p), drained,
+                                                         reinterpret_cast<hive::Op16*>(ops + num_ops));
+      HK(hipGetLastError());
+      n_cur = num_ops + drained;
+    }
+  }
+  if (num_ops) HK(hipMemcpyAsync(h_results, res, num_ops * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HK(hipStreamSynchronize(s));
+  if (elapsed_ms) *elapsed_ms = iters ? total_ms / static_cast<double>(iters) : 0.0;
+  return HIVE_OK;
+}
+
+hive_status_t hive_hash_table_kernel_dispatch_ycsb(const hive_operation_t* h_prefill_ops,
+                                                   const hive_operation_t* h_workload_ops, size_t num_prefill_ops,
+                                                   size_t num_workload_ops, size_t table_size,
+                                                   size_t threads_per_block, size_t num_iterations,
+                                                   double* elapsed_ms, uint64_t* h_results, int verify, int layout) {
+  (void)verify;
+  if (layout < 0 || layout > 2) return HIVE_EINVAL;
+  if ((num_prefill_ops && !h_prefill_ops) || (num_workload_ops && (!h_workload_ops || !h_results)))
+    return HIVE_EINVAL;
+  if (table_size / 8 == 0) return HIVE_EINVAL;
+  Table tb;
+  CK(hive_create(table_size, HIVE_POLICY_DEFAULT2HASH, &tb.t));
+  if (threads_per_block >= 64 && threads_per_block <= 1024 && threads_per_block % 64 == 0) {
+    hive_launch_t cfg{static_cast<uint32_t>(threads_per_block), 1};
+    for (int op = 0; op < 3; ++op) CK(hive_set_launch(tb.t, op, &cfg));
+  }
+  hipStream_t s = nullptr;
+  DevBuf d_pre, d_work, d_res;
+  if (num_prefill_ops) {
+    HK(hipMalloc(&d_pre.p, num_prefill_ops * sizeof(hive_operation_t)));
+    HK(hipMemcpyAsync(d_pre.p, h_prefill_ops, num_prefill_ops * sizeof(hive_operation_t), hipMemcpyHostToDevice, s));
+    CK(hive_insert_ops(tb.t, static_cast<hive_operation_t*>(d_pre.p), num_prefill_ops, nullptr, s));
+  }
+  const size_t nw = num_workload_ops;
+  HK(hipMalloc(&d_work.p, (nw ? nw : 1) * sizeof(hive_operation_t)));
+  HK(hipMalloc(&d_res.p, (nw ? nw : 1) * sizeof(uint64_t)));
+  HK(hipMemsetAsync(d_res.p, 0, (nw ? nw : 1) * sizeof(uint64_t), s));
+  if (nw) HK(hipMemcpyAsync(d_work.p, h_workload_ops, nw * sizeof(hive_operation_t), hipMemcpyHostToDevice, s));
+  auto* ops = static_cast<hive_operation_t*>(d_work.p);
+  auto* res = static_cast<uint64_t*>(d_res.p);
+  for (int w = 0; w < 5; ++w) CK(hive_mixed_ops(tb.t, ops, nw, res, s));
+  HK(hipStreamSynchronize(s));
+  Events ev;
+  HK(hipEventCreate(&ev.a));
+  HK(hipEventCreate(&ev.b));
+  double total_ms = 0.0;
+  for (size_t it = 0; it < num_iterations; ++it) {
+    HK(hipEventRecord(ev.a, s));
+    CK(hive_mixed_ops(tb.t, ops, nw, res, s));
+    HK(hipEventRecord(ev.b, s));
+    HK(hipEventSynchronize(ev.b));
+    float ms = 0.f;
+    HK(hipEventElapsedTime(&ms, ev.a, ev.b));
+    total_ms += ms;
+  }
+  if (nw) HK(hipMemcpyAsync(h_results, res, nw * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HK(hipStreamSynchronize(s));
+  if (elapsed_ms) *elapsed_ms = num_iterations ? total_ms / static_cast<double>(num_iterations) : 0.0;
+  return HIVE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,491 @@
+// hive_device.hpp -- device-side primitives of the MI355X Hive hash table:
+// hash functions and policies, the table view, coherent loads and CAS, the
+// per-bucket lock, and the insert / lookup / erase protocols.
+//
+// Memory-model notes for gfx950 (8 XCDs, one L2 per XCD):
+//  * plain loads may return lines cached in this XCD's L2 before another XCD
+//    wrote them during the same kernel; kernel boundaries make all writes
+//    visible.  Every fast path therefore reads with plain loads and confirms
+//    with CAS (performed at the coherence point, returns the true old value).
+//  * slow paths read with agent-scope relaxed atomic loads (global_load sc1),
+//    which bypass stale L2 lines, and write only with atomics.
+//  * no acquire/release fences at agent scope: they emit buffer_inv/buffer_wbl2
+//    over the whole L2.  The lock release waits for the critical section's
+//    memory operations with s_waitcnt vmcnt(0) and releases with an atomic swap.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hive {
+
+constexpr int kSlots = 8;        // reference bucket: 8 x 8 B (include/hive_hash_table_struct.cuh:20-22)
+constexpr int kStashSlots = 8;   // include/hive_stash_table.cuh:20
+constexpr int kMaxKicks = 8;     // include/hive_hash_table.cuh:21
+constexpr int kCounterLanes = 64;
+constexpr int kCounterStride = 16;  // u64 words between counter lanes (128 B)
+
+// result codes (include/hive.h)
+constexpr uint8_t R_FAILED = 0, R_INSERTED = 1, R_UPDATED = 2, R_STASHED = 3, R_FOUND = 4,
+                  R_NOT_FOUND = 5, R_ERASED = 6, R_INVALID_KEY = 7, R_NOOP = 8, R_PENDING = 0xFF;
+constexpr int OP_INSERT = 0, OP_LOOKUP = 1, OP_DELETE = 2;
+
+struct Op16 {  // the reference's Operation record (include/utils.h:18-21)
+  int32_t type;
+  uint32_t pad;
+  unsigned long long key;
+};
+
+struct TableView {
+  unsigned long long* slots;        // nb * 8
+  unsigned long long* stash;        // nb * 8
+  uint32_t* stash_count;            // nb
+  uint32_t* stash_flag;             // (nb + 31) / 32
+  uint32_t* locks;                  // nb
+  unsigned long long* counters;     // [2][kCounterLanes][kCounterStride]: table, stash
+  uint32_t nb;
+};
+
+// ------------------------------------------------------------------ hashes
+// include/hash.hpp:39-47
+__host__ __device__ __forceinline__ uint32_t mix_hash1(uint32_t key) {
+  key = ~key + (key << 15);
+  key = key ^ (key >> 12);
+  key = key + (key << 2);
+  key = key ^ (key >> 4);
+  key = key * 2057u;
+  key = key ^ (key >> 16);
+  return key;
+}
+
+// include/hash.hpp:64-78 (the lop3 0x96 steps are three-input XORs)
+__host__ __device__ __forceinline__ uint32_t mix_hash2(uint32_t key) {
+  key = (key + 0x7ed55d16u) + (key << 12);
+  key = (key ^ 0xc761c23cu) ^ (key >> 19);
+  key = (key + 0x165667b1u) + (key << 5);
+  key = (key + 0xd3a2646cu) ^ (key << 9);
+  key = (key + 0xfd7046c5u) + (key << 3);
+  key = (key ^ 0xb55a4f09u) ^ (key >> 16);
+  return key;
+}
+
+__host__ __device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// include/hash.hpp:110-138: MurmurHash3_x86_32, 4-byte key, seed 0
+__host__ __device__ __forceinline__ uint32_t mix_murmur(uint32_t key) {
+  uint32_t k = rotl32(key * 0xcc9e2d51u, 15) * 0x1b873593u;
+  uint32_t h = rotl32(k, 13) * 5u + 0xe6546b64u;
+  return fmix32(h ^ 4u);
+}
+
+// include/hash.hpp:80-108 (32-bit arithmetic throughout)
+__host__ __device__ __forceinline__ uint32_t mix_city(uint32_t key) {
+  uint32_t k = ((key ^ 0x9ae16a3bu) * 0xc2b2ae35u ^ 0x1b873593u) * 0x85ebca6bu;
+  uint32_t h = ((k * 0xc2b2ae35u) ^ 0x1b873593u) * 0x85ebca6bu;
+  return fmix32(h ^ 4u);
+}
+
+// include/hash.hpp:143-207: reflected CRC tables with the reference's polynomials
+struct CrcTables {
+  unsigned long long t64[256];
+  uint32_t t32[256];
+};
+
+constexpr CrcTables make_crc_tables() {
+  CrcTables c{};
+  for (uint32_t i = 0; i < 256; ++i) {
+    unsigned long long a = i;
+    uint32_t b = i;
+    for (int j = 0; j < 8; ++j) {
+      a = (a & 1ull) ? (a >> 1) ^ 0x42F0E1EBA9EA3693ull : (a >> 1);
+      b = (b & 1u) ? (b >> 1) ^ 0x04C11D87u : (b >> 1);
+    }
+    c.t64[i] = a;
+    c.t32[i] = b;
+  }
+  return c;
+}
+
+__constant__ constexpr CrcTables kCrc = make_crc_tables();
+
+__device__ __forceinline__ uint32_t mix_crc64(uint32_t key) {
+  unsigned long long crc = ~0ull;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) crc = (crc >> 8) ^ kCrc.t64[(crc ^ (key >> (8 * i))) & 0xFF];
+  return static_cast<uint32_t>(crc ^ ~0ull);
+}
+
+__device__ __forceinline__ uint32_t mix_crc32(uint32_t key) {
+  uint32_t crc = 0xFFFFFFFFu;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) crc = (crc >> 8) ^ kCrc.t32[(crc ^ (key >> (8 * i))) & 0xFF];
+  return crc ^ 0xFFFFFFFFu;
+}
+
+// ---------------------------------------------------------------- policies
+// include/HashPolicies.cuh:10-65
+struct PolicyDefault2 {
+  static constexpr int NH = 2;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_hash1(k) : mix_hash2(k); }
+};
+struct PolicyMurmurCity {
+  static constexpr int NH = 2;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_murmur(k) : mix_city(k); }
+};
+struct PolicyTriple {
+  static constexpr int NH = 3;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
+    return i == 0 ? mix_hash1(k) : (i == 1 ? mix_hash2(k) : mix_city(k));
+  }
+};
+struct PolicyMurmurCityBit {
+  static constexpr int NH = 3;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
+    return i == 0 ? mix_murmur(k) : (i == 1 ? mix_city(k) : mix_hash1(k));
+  }
+};
+struct PolicyLookup2 {
+  static constexpr int NH = 2;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_crc64(k) : mix_crc32(k); }
+};
+struct PolicyLookup3 {
+  static constexpr int NH = 3;
+  __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
+    return i == 0 ? mix_crc64(k) : (i == 1 ? mix_crc32(k) : mix_hash1(k));
+  }
+};
+
+template <class P>
+__device__ __forceinline__ uint32_t bucket_of(int i, uint32_t key, uint32_t nb) {
+  return P::raw(i, key) % nb;
+}
+
+// include/hive_hash_insert.cuh:58-71
+template <class P>
+__device__ __forceinline__ uint32_t alt_bucket(uint32_t key, uint32_t cur, uint32_t nb) {
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i)
+    if (bucket_of<P>(i, key, nb) == cur) return bucket_of<P>((i + 1) % P::NH, key, nb);
+  return bucket_of<P>(0, key, nb);
+}
+
+// ------------------------------------------------------------------ memory
+__device__ __forceinline__ uint32_t kv_key(unsigned long long kv) { return static_cast<uint32_t>(kv); }
+__device__ __forceinline__ uint32_t kv_val(unsigned long long kv) { return static_cast<uint32_t>(kv >> 32); }
+__device__ __forceinline__ unsigned long long kv_pack(uint32_t k, uint32_t v) {
+  return (static_cast<unsigned long long>(v) << 32) | k;
+}
+
+__device__ __forceinline__ unsigned long long ld_sync(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_sync32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// returns the value held before the operation (== expect on success)
+__device__ __forceinline__ unsigned long long cas64(unsigned long long* p, unsigned long long expect,
+                                                    unsigned long long desired) {
+  __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expect;
+}
+
+// all of this lane's outstanding memory operations have completed
+__device__ __forceinline__ void wait_vmem() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // s_waitcnt vmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ bool try_lock(uint32_t* l) {
+  uint32_t z = 0;
+  return __hip_atomic_compare_exchange_strong(l, &z, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void unlock(uint32_t* l) {
+  wait_vmem();
+  __hip_atomic_exchange(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// plain 64-byte bucket read (four 16-byte loads)
+__device__ __forceinline__ void load_bucket(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int j = 0; j < kSlots / 2; ++j) {
+    uint4 v = q[j];
+    s[2 * j] = (static_cast<unsigned long long>(v.y) << 32) | v.x;
+    s[2 * j + 1] = (static_cast<unsigned long long>(v.w) << 32) | v.z;
+  }
+}
+
+__device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) s[j] = ld_sync(p + j);
+}
+
+__device__ __forceinline__ uint32_t match_mask(const unsigned long long (&s)[kSlots], uint32_t key) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) m |= static_cast<uint32_t>(kv_key(s[j]) == key) << j;
+  return m;  // key != 0, so an empty slot never matches
+}
+
+__device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[kSlots]) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) m |= static_cast<uint32_t>(s[j] == 0ull) << j;
+  return m;
+}
+
+__device__ __forceinline__ bool stash_flag_set(const TableView& t, uint32_t b) {
+  return (t.stash_flag[b >> 5] >> (b & 31)) & 1u;
+}
+
+__device__ __forceinline__ unsigned long long* bucket_ptr(const TableView& t, uint32_t b) {
+  return t.slots + static_cast<size_t>(b) * kSlots;
+}
+__device__ __forceinline__ unsigned long long* stash_ptr(const TableView& t, uint32_t b) {
+  return t.stash + static_cast<size_t>(b) * kStashSlots;
+}
+
+// ------------------------------------------------------------------ insert
+// Fast path of one insert, run by every lane with plain loads:
+//   Stage A  (include/hive_hash_insert.cuh:911-930)  update in any candidate
+//   Stage B  (include/hive_hash_insert.cuh:271-310)  claim lowest empty H0 slot
+// Returns R_PENDING when the stash must be consulted, the canonical bucket is
+// full, or a CAS revealed a concurrent change; the op then takes slow_insert.
+template <class P>
+__device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key, uint32_t val) {
+  if (key == 0) return R_INVALID_KEY;
+  const unsigned long long kv = kv_pack(key, val);
+  uint32_t b[P::NH];
+  unsigned long long s[P::NH][kSlots];
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) load_bucket(bucket_ptr(t, b[i]), s[i]);
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) {
+    uint32_t m = match_mask(s[i], key);
+    if (m) {
+      const int j = __builtin_ctz(m);
+      return cas64(bucket_ptr(t, b[i]) + j, s[i][j], kv) == s[i][j] ? R_UPDATED : R_PENDING;
+    }
+  }
+  if (stash_flag_set(t, b[0])) return R_PENDING;  // Stage A' runs under the lock
+  uint32_t e = empty_mask(s[0]);
+  unsigned long long* cb = bucket_ptr(t, b[0]);
+  while (e) {
+    const int j = __builtin_ctz(e);
+    const unsigned long long seen = cas64(cb + j, 0ull, kv);
+    if (seen == 0ull) return R_INSERTED;
+    if (kv_key(seen) == key) return R_PENDING;  // same key landed first: update under the lock
+    e &= e - 1;
+  }
+  return R_PENDING;
+}
+
+// Stage A on one bucket with coherent reads: 1 updated, 0 absent, -1 raced.
+__device__ __forceinline__ int locked_update(unsigned long long* bk, int nslots, uint32_t key,
+                                             unsigned long long kv) {
+  for (int j = 0; j < nslots; ++j) {
+    unsigned long long cur = ld_sync(bk + j);
+    if (cur == 0ull || kv_key(cur) != key) continue;
+    for (int spin = 0; spin < 64; ++spin) {
+      const unsigned long long seen = cas64(bk + j, cur, kv);
+      if (seen == cur) return 1;
+      if (seen == 0ull || kv_key(seen) != key) return -1;
+      cur = seen;
+    }
+    return -1;
+  }
+  return 0;
+}
+
+// The whole protocol for one key while holding lock(H0(key)):
+//   Stage A / A' re-check   include/hive_hash_insert.cuh:522-541, :166-218
+//   Stage B re-try          include/hive_hash_insert.cuh:586-614
+//   Stage C cuckoo kicks    include/hive_hash_insert.cuh:574-692
+//   Stage D stash push      include/hive_hash_insert.cuh:489-517
+// The victim is copied into its alternate bucket (claimed by CAS 0 -> victim)
+// before its source slot is CAS-replaced, so it is never absent from both.
+template <class P>
+__device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val, uint32_t c) {
+  const unsigned long long kv = kv_pack(key, val);
+  unsigned long long* cb = bucket_ptr(t, c);
+  unsigned long long* sb = stash_ptr(t, c);
+  for (int attempt = 0; attempt < 256; ++attempt) {
+    bool retry = false;
+#pragma unroll
+    for (int i = 0; i < P::NH; ++i) {
+      if (retry) break;
+      const int r = locked_update(bucket_ptr(t, bucket_of<P>(i, key, t.nb)), kSlots, key, kv);
+      if (r > 0) return R_UPDATED;
+      if (r < 0) retry = true;
+    }
+    if (retry) continue;
+    uint32_t cnt = ld_sync32(t.stash_count + c);
+    if (cnt > kStashSlots) cnt = kStashSlots;
+    {
+      const int r = locked_update(sb, static_cast<int>(cnt), key, kv);
+      if (r > 0) return R_UPDATED;
+      if (r < 0) continue;
+    }
+    unsigned long long cs[kSlots];
+    load_bucket_sync(cb, cs);
+    uint32_t e = empty_mask(cs);
+    while (e) {
+      const int j = __builtin_ctz(e);
+      const unsigned long long seen = cas64(cb + j, 0ull, kv);
+      if (seen == 0ull) return R_INSERTED;
+      cs[j] = seen;
+      if (kv_key(seen) == key) { retry = true; break; }
+      e &= e - 1;
+    }
+    if (retry) continue;
+    for (int kick = 0; kick < kMaxKicks; ++kick) {
+      const int vs = kick % kSlots;
+      const unsigned long long victim = cs[vs];
+      if (victim == 0ull) continue;
+      const uint32_t a = alt_bucket<P>(kv_key(victim), c, t.nb);
+      if (a == c) continue;
+      unsigned long long* ab = bucket_ptr(t, a);
+      unsigned long long as[kSlots];
+      load_bucket_sync(ab, as);
+      uint32_t ae = empty_mask(as);
+      int d = -1;
+      while (ae) {
+        const int j = __builtin_ctz(ae);
+        if (cas64(ab + j, 0ull, victim) == 0ull) { d = j; break; }
+        ae &= ae - 1;
+      }
+      if (d < 0) continue;
+      if (cas64(cb + vs, victim, kv) == victim) return R_INSERTED;
+      // the victim's slot changed (a concurrent update of the victim key):
+      // withdraw the copy and start over
+      unsigned long long cur = victim;
+      for (int spin = 0; spin < 64; ++spin) {
+        const unsigned long long seen = cas64(ab + d, cur, 0ull);
+        if (seen == cur || seen == 0ull || kv_key(seen) != kv_key(victim)) break;
+        cur = seen;
+      }
+      retry = true;
+      break;
+    }
+    if (retry) continue;
+    if (cnt < kStashSlots) {
+      __hip_atomic_store(sb + cnt, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(t.stash_count + c, cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_or(t.stash_flag + (c >> 5), 1u << (c & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return R_STASHED;
+    }
+    return R_FAILED;
+  }
+  return R_FAILED;
+}
+
+// Acquire lock(H0) with a try-lock whose critical section completes inside the
+// same loop iteration, so lanes of one wavefront that contend for the same
+// lock cannot wait on each other (no independent thread scheduling on CDNA).
+template <class P>
+__device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
+  const uint32_t c = bucket_of<P>(0, key, t.nb);
+  uint8_t result = R_PENDING;
+  uint32_t backoff = 1;
+  while (result == R_PENDING) {
+    if (try_lock(t.locks + c)) {
+      result = locked_insert<P>(t, key, val, c);
+      unlock(t.locks + c);
+    } else {
+      for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
+      backoff = backoff < 64 ? backoff * 2 : 64;
+    }
+  }
+  return result;
+}
+
+// ------------------------------------------------------------------ lookup
+// include/hive_hash_lookup.cuh:126-155; returns R_FOUND / R_NOT_FOUND.
+template <class P>
+__device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, uint32_t& val) {
+  val = 0;
+  if (key == 0) return R_NOT_FOUND;
+  uint32_t b[P::NH];
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) {
+    bool dup = false;
+#pragma unroll
+    for (int j = 0; j < i; ++j) dup |= b[j] == b[i];
+    if (dup) continue;
+    unsigned long long s[kSlots];
+    load_bucket(bucket_ptr(t, b[i]), s);
+    const uint32_t m = match_mask(s, key);
+    if (m) {
+      val = kv_val(s[__builtin_ctz(m)]);
+      return R_FOUND;
+    }
+  }
+  if (stash_flag_set(t, b[0])) {
+    uint32_t cnt = t.stash_count[b[0]];
+    if (cnt > kStashSlots) cnt = kStashSlots;
+    const unsigned long long* sb = stash_ptr(t, b[0]);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const unsigned long long cur = sb[j];
+      if (cur != 0ull && kv_key(cur) == key) {
+        val = kv_val(cur);
+        return R_FOUND;
+      }
+    }
+  }
+  return R_NOT_FOUND;
+}
+
+// ------------------------------------------------------------------- erase
+// include/hive_hash_delete.cuh:168-194: lowest match in every candidate bucket,
+// then the canonical stash (count unchanged).  removed_tab / removed_stash
+// receive the number of copies removed.
+__device__ __forceinline__ int erase_in(unsigned long long* bk, int nslots, uint32_t key) {
+  for (int j = 0; j < nslots; ++j) {
+    unsigned long long cur = bk[j];
+    if (cur == 0ull || kv_key(cur) != key) continue;
+    for (int spin = 0; spin < 64; ++spin) {
+      const unsigned long long seen = cas64(bk + j, cur, 0ull);
+      if (seen == cur) return 1;
+      if (seen == 0ull || kv_key(seen) != key) return 0;
+      cur = seen;
+    }
+    return 0;
+  }
+  return 0;
+}
+
+template <class P>
+__device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, int& removed_tab, int& removed_stash) {
+  removed_tab = 0;
+  removed_stash = 0;
+  if (key == 0) return R_INVALID_KEY;
+  uint32_t b[P::NH];
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+#pragma unroll
+  for (int i = 0; i < P::NH; ++i) removed_tab += erase_in(bucket_ptr(t, b[i]), kSlots, key);
+  if (stash_flag_set(t, b[0])) {
+    uint32_t cnt = t.stash_count[b[0]];
+    if (cnt > kStashSlots) cnt = kStashSlots;
+    removed_stash = erase_in(stash_ptr(t, b[0]), static_cast<int>(cnt), key);
+  }
+  return (removed_tab + removed_stash) ? R_ERASED : R_NOT_FOUND;
+}
+
+}  // namespace hive

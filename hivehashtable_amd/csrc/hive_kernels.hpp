@@ -1,0 +1,341 @@
+// hive_kernels.hpp -- __global__ kernels of the MI355X Hive hash table.
+//
+// Work mapping: one operation per lane (lane-per-key).  A lane reads each
+// 64-byte candidate bucket with four 16-byte loads, so a wavefront keeps 64
+// independent random bucket reads in flight.  Divergent insert slow paths
+// (cuckoo eviction, stash push) are staged through LDS: lanes whose fast path
+// cannot finish append their op to a per-block LDS queue, and after a block
+// barrier the queue is processed by consecutive lanes, so the slow path runs
+// on densely packed wavefronts instead of one lane per wave.
+#pragma once
+
+#include "hive_device.hpp"
+
+namespace hive {
+
+// ---------------------------------------------------------------- sources
+struct SrcNative {
+  const uint32_t* keys;
+  const uint32_t* vals;   // NULL: value = key + 1
+  const uint8_t* types;   // NULL: every op has the kernel's own type
+  __device__ __forceinline__ uint32_t key(size_t i) const { return keys[i]; }
+  __device__ __forceinline__ uint32_t val(size_t i, uint32_t k) const { return vals ? vals[i] : k + 1u; }
+  __device__ __forceinline__ int type(size_t i, int dflt) const { return types ? static_cast<int>(types[i]) : dflt; }
+};
+
+struct SrcOps {  // value = key + 1 (include/hive_kernels.cuh:156,238)
+  const Op16* ops;
+  __device__ __forceinline__ uint32_t key(size_t i) const { return static_cast<uint32_t>(ops[i].key); }
+  __device__ __forceinline__ uint32_t val(size_t, uint32_t k) const { return k + 1u; }
+  __device__ __forceinline__ int type(size_t i, int) const { return ops[i].type; }
+};
+
+struct SrcKV {  // packed kv records (stash re-insert)
+  const unsigned long long* kv;
+  __device__ __forceinline__ uint32_t key(size_t i) const { return kv_key(kv[i]); }
+  __device__ __forceinline__ uint32_t val(size_t i, uint32_t) const { return kv_val(kv[i]); }
+  __device__ __forceinline__ int type(size_t, int dflt) const { return dflt; }
+};
+
+// ---------------------------------------------------------------- outputs
+struct OutNative {
+  uint8_t* status;
+  uint32_t* vals;
+  __device__ __forceinline__ void put(size_t i, uint8_t st, uint32_t v) const {
+    if (status) status[i] = st;
+    if (vals) vals[i] = v;
+  }
+};
+
+struct OutU64 {  // 0/1 per op (include/hive_kernels.cuh:110-113,187-190)
+  unsigned long long* res;
+  __device__ __forceinline__ void put(size_t i, uint8_t st, uint32_t) const {
+    if (res)
+      res[i] = (st == R_INSERTED || st == R_UPDATED || st == R_STASHED || st == R_FOUND || st == R_ERASED) ? 1ull : 0ull;
+  }
+};
+
+struct OutCount {  // discards per-op results, counts failures
+  unsigned long long* failed;
+  __device__ __forceinline__ void put(size_t, uint8_t st, uint32_t) const {
+    if (st == R_FAILED) atomicAdd(failed, 1ull);
+  }
+};
+
+// ---------------------------------------------------------------- counters
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Called by every thread of the block (contains __syncthreads).
+__device__ __forceinline__ void block_count(const TableView& t, int dtab, int dstash, int* blk) {
+  const int wt = wave_sum(dtab);
+  const int ws = wave_sum(dstash);
+  if ((threadIdx.x & 63) == 0) {
+    if (wt) atomicAdd(&blk[0], wt);
+    if (ws) atomicAdd(&blk[1], ws);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int lane = blockIdx.x & (kCounterLanes - 1);
+    if (blk[0]) atomicAdd(t.counters + (0 * kCounterLanes + lane) * kCounterStride,
+                          static_cast<unsigned long long>(static_cast<long long>(blk[0])));
+    if (blk[1]) atomicAdd(t.counters + (1 * kCounterLanes + lane) * kCounterStride,
+                          static_cast<unsigned long long>(static_cast<long long>(blk[1])));
+  }
+}
+
+constexpr int kMaxBlock = 1024;
+
+// ---------------------------------------------------------------- insert
+// phase < 0: every op is an insert.  phase == OP_INSERT: only ops whose type
+// is INSERT (the insert phase of a mixed batch).
+template <class P, class Src, class Out>
+__global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase) {
+  __shared__ uint32_t queue[kMaxBlock];
+  __shared__ uint32_t qn;
+  __shared__ int blk[2];
+  if (threadIdx.x == 0) {
+    qn = 0;
+    blk[0] = blk[1] = 0;
+  }
+  __syncthreads();
+  const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
+  const size_t i = base + threadIdx.x;
+  int dtab = 0, dstash = 0;
+  if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
+    const uint32_t key = src.key(i);
+    const uint8_t r = fast_insert<P>(t, key, src.val(i, key));
+    if (r == R_PENDING) {
+      queue[atomicAdd(&qn, 1u)] = threadIdx.x;
+    } else {
+      out.put(i, r, 0);
+      dtab += (r == R_INSERTED);
+    }
+  }
+  __syncthreads();
+  const uint32_t pending = qn;
+  if (threadIdx.x < pending) {  // LDS-staged slow path on packed lanes
+    const size_t idx = base + queue[threadIdx.x];
+    const uint32_t key = src.key(idx);
+    const uint8_t r = slow_insert<P>(t, key, src.val(idx, key));
+    out.put(idx, r, 0);
+    dtab += (r == R_INSERTED);
+    dstash += (r == R_STASHED);
+  }
+  block_count(t, dtab, dstash, blk);
+}
+
+// ---------------------------------------------------------------- lookup
+template <class P, class Src, class Out>
+__global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size_t n, Out out, int phase) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (phase >= 0) {
+    const int ty = src.type(i, OP_LOOKUP);
+    if (ty != OP_LOOKUP) {
+      if (ty != OP_INSERT && ty != OP_DELETE) out.put(i, R_NOOP, 0);
+      return;
+    }
+  }
+  uint32_t v;
+  const uint8_t r = lookup_one<P>(t, src.key(i), v);
+  out.put(i, r, v);
+}
+
+// ---------------------------------------------------------------- erase
+template <class P, class Src, class Out>
+__global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_t n, Out out, int phase) {
+  __shared__ int blk[2];
+  if (threadIdx.x == 0) blk[0] = blk[1] = 0;
+  __syncthreads();
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int rt = 0, rs = 0;
+  if (i < n && (phase < 0 || src.type(i, OP_DELETE) == OP_DELETE)) {
+    const uint8_t r = erase_one<P>(t, src.key(i), rt, rs);
+    out.put(i, r, 0);
+  }
+  block_count(t, -rt, -rs, blk);
+}
+
+// ---------------------------------------------------------------- rehash
+// One thread per source entry: slots [0, nb*8) then stash slots.  Entries are
+// inserted into dst with the same fast/slow protocol.
+template <class P>
+__global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView src, unsigned long long* failed) {
+  __shared__ unsigned long long queue[kMaxBlock];
+  __shared__ uint32_t qn;
+  __shared__ int blk[2];
+  if (threadIdx.x == 0) {
+    qn = 0;
+    blk[0] = blk[1] = 0;
+  }
+  __syncthreads();
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t ntab = static_cast<size_t>(src.nb) * kSlots;
+  unsigned long long kv = 0;
+  if (i < ntab) {
+    kv = src.slots[i];
+  } else if (i < 2 * ntab) {
+    const size_t e = i - ntab;
+    const uint32_t b = static_cast<uint32_t>(e / kStashSlots);
+    const uint32_t j = static_cast<uint32_t>(e % kStashSlots);
+    if (j < src.stash_count[b]) kv = src.stash[e];
+  }
+  int dtab = 0, dstash = 0;
+  if (kv != 0ull) {
+    const uint8_t r = fast_insert<P>(dst, kv_key(kv), kv_val(kv));
+    if (r == R_PENDING) queue[atomicAdd(&qn, 1u)] = kv;
+    else dtab += (r == R_INSERTED);
+  }
+  __syncthreads();
+  if (threadIdx.x < qn) {
+    const unsigned long long q = queue[threadIdx.x];
+    const uint8_t r = slow_insert<P>(dst, kv_key(q), kv_val(q));
+    dtab += (r == R_INSERTED);
+    dstash += (r == R_STASHED);
+    if (r == R_FAILED) atomicAdd(failed, 1ull);
+  }
+  block_count(dst, dtab, dstash, blk);
+}
+
+// ---------------------------------------------------------------- utility
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// exact occupancy (include/hive_kernels.cuh:303-325, counting kv != 0)
+__global__ void k_count_exact(TableView t, unsigned long long* out) {
+  const size_t ntab = static_cast<size_t>(t.nb) * kSlots;
+  unsigned long long ct = 0, cs = 0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < ntab;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    ct += (t.slots[i] != 0ull);
+    const uint32_t b = static_cast<uint32_t>(i / kStashSlots);
+    const uint32_t j = static_cast<uint32_t>(i % kStashSlots);
+    uint32_t cnt = t.stash_count[b];
+    cs += (j < cnt && t.stash[i] != 0ull);
+  }
+  int a = static_cast<int>(ct), c = static_cast<int>(cs);
+  a = wave_sum(a);
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicAdd(out, static_cast<unsigned long long>(a));
+    if (c) atomicAdd(out + 1, static_cast<unsigned long long>(c));
+  }
+}
+
+__global__ void k_rebuild_flags(TableView t) {
+  const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nw = (static_cast<size_t>(t.nb) + 31) / 32;
+  if (w >= nw) return;
+  uint32_t bits = 0;
+  for (int j = 0; j < 32; ++j) {
+    const size_t b = w * 32 + j;
+    if (b < t.nb && t.stash_count[b] > 0) bits |= 1u << j;
+  }
+  t.stash_flag[w] = bits;
+}
+
+// stash drain (include/hive_stash_table.cuh:33-64; holes are skipped)
+__global__ void k_stash_count_live(TableView t, unsigned long long* total) {
+  const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int c = 0;
+  if (b < t.nb) {
+    uint32_t cnt = t.stash_count[b];
+    if (cnt > kStashSlots) cnt = kStashSlots;
+    for (uint32_t j = 0; j < cnt; ++j) c += (t.stash[b * kStashSlots + j] != 0ull);
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, static_cast<unsigned long long>(c));
+}
+
+__global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned long long* pos) {
+  const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= t.nb) return;
+  uint32_t cnt = t.stash_count[b];
+  if (cnt > kStashSlots) cnt = kStashSlots;
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const unsigned long long kv = t.stash[b * kStashSlots + j];
+    if (kv != 0ull) out[atomicAdd(pos, 1ull)] = kv;
+  }
+}
+
+// ---------------------------------------------------------------- probe stats
+// Per key: the distinct candidate buckets an insert's fast path reads, the
+// buckets a lookup reads before its early exit, and whether the lookup also
+// reads the stash.  out[0..2] accumulate the three totals.
+template <class P>
+__global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t* keys, size_t n,
+                                                     unsigned long long* out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int cand = 0, reads = 0, stash = 0;
+  if (i < n && keys[i] != 0) {
+    const uint32_t key = keys[i];
+    uint32_t b[P::NH];
+#pragma unroll
+    for (int j = 0; j < P::NH; ++j) b[j] = bucket_of<P>(j, key, t.nb);
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < P::NH; ++j) {
+      bool dup = false;
+#pragma unroll
+      for (int q = 0; q < j; ++q) dup |= b[q] == b[j];
+      if (dup) continue;
+      ++cand;
+      if (found) continue;
+      ++reads;
+      unsigned long long s[kSlots];
+      load_bucket(bucket_ptr(t, b[j]), s);
+      found = match_mask(s, key) != 0;
+    }
+    if (!found && stash_flag_set(t, b[0])) stash = 1;
+  }
+  cand = wave_sum(cand);
+  reads = wave_sum(reads);
+  stash = wave_sum(stash);
+  if ((threadIdx.x & 63) == 0) {
+    if (cand) atomicAdd(out, static_cast<unsigned long long>(cand));
+    if (reads) atomicAdd(out + 1, static_cast<unsigned long long>(reads));
+    if (stash) atomicAdd(out + 2, static_cast<unsigned long long>(stash));
+  }
+}
+
+// ---------------------------------------------------------------- ceilings
+__device__ __forceinline__ uint32_t ceiling_index(size_t i, uint32_t seed) {
+  return mix_murmur(static_cast<uint32_t>(i) ^ seed) ^ mix_hash2(static_cast<uint32_t>(i >> 32) + seed);
+}
+
+// n random 64-byte reads (one per lane), same load shape as a bucket probe
+__global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned long long* buf, uint32_t nbuckets, size_t n,
+                                                        uint32_t seed, uint32_t* sink) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = ceiling_index(i, seed) % nbuckets;
+  unsigned long long s[kSlots];
+  load_bucket(buf + static_cast<size_t>(b) * kSlots, s);
+  unsigned long long acc = 0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) acc ^= s[j];
+  if (static_cast<uint32_t>(acc) == 0x9e3779b9u && sink) sink[blockIdx.x] = static_cast<uint32_t>(acc >> 32);
+}
+
+// n random 64-bit CAS (one per lane)
+__global__ __launch_bounds__(256) void k_ceiling_cas(unsigned long long* buf, size_t nwords, size_t n, uint32_t seed) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t w = (static_cast<size_t>(ceiling_index(i, seed)) * 8u + (i & 7)) % nwords;
+  cas64(buf + w, 0ull, static_cast<unsigned long long>(i) + 1ull);
+}
+
+}  // namespace hive

@@ -1,0 +1,249 @@
+"""HiveTable: Python handle over the libhive.so C-ABI.
+
+Mirrors the reference's table operations (bulk insert / lookup / delete,
+mixed batches, resize, stash drain).  Inputs are device tensors (torch is
+only the carrier of device memory and streams); numpy arrays are copied to
+the current device first.  Every call is stream-ordered on the given torch
+stream (default: torch's current stream).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (Launch, Operation, Stats, POLICIES, check, lib, OP_INSERT, OP_LOOKUP, OP_DELETE)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("HiveTable needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev(x, dtype):
+    """Contiguous device tensor of the given torch dtype."""
+    torch = _torch()
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x)
+        if x.dtype == np.uint32:  # keys and values are u32 bit patterns carried in int32 tensors
+            x = x.view(np.int32)
+        x = torch.from_numpy(x)
+    if x.dtype != dtype:
+        x = x.to(dtype)
+    if not x.is_cuda:
+        x = x.cuda()
+    return x.contiguous()
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class HiveTable:
+    """A device-resident Hive hash table (8-slot buckets, 8-slot stash per bucket).
+
+    ``num_slots`` is the reference's ``table_size``: num_buckets = num_slots // 8.
+    """
+
+    def __init__(self, num_slots, policy="Default2Hash"):
+        self.L = lib()
+        self.policy = POLICIES[policy] if isinstance(policy, str) else int(policy)
+        h = C.c_void_p()
+        check(self.L.hive_create(int(num_slots), self.policy, C.byref(h)), "hive_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hive_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_buckets(self):
+        return int(self.L.hive_num_buckets(self.h))
+
+    # ---------------------------------------------------------- bulk ops
+    def clear(self, stream=None):
+        check(self.L.hive_clear(self.h, _stream_ptr(stream)), "hive_clear")
+
+    def insert(self, keys, vals=None, status=None, stream=None):
+        torch = _torch()
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        check(self.L.hive_insert(self.h, _ptr(k), _ptr(v), k.numel(), _ptr(status), _stream_ptr(stream)),
+              "hive_insert")
+        return status
+
+    def lookup(self, keys, vals=None, status=None, stream=None):
+        torch = _torch()
+        k = _dev(keys, torch.int32)
+        if vals is None:
+            vals = torch.empty(k.numel(), dtype=torch.int32, device=k.device)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        check(self.L.hive_lookup(self.h, _ptr(k), k.numel(), _ptr(vals), _ptr(status), _stream_ptr(stream)),
+              "hive_lookup")
+        return vals, status
+
+    def erase(self, keys, status=None, stream=None):
+        torch = _torch()
+        k = _dev(keys, torch.int32)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        check(self.L.hive_erase(self.h, _ptr(k), k.numel(), _ptr(status), _stream_ptr(stream)), "hive_erase")
+        return status
+
+    def mixed(self, types, keys, vals=None, out_vals=None, status=None, stream=None):
+        torch = _torch()
+        ty = _dev(types, torch.uint8)
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        if out_vals is None:
+            out_vals = torch.empty(k.numel(), dtype=torch.int32, device=k.device)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        check(self.L.hive_mixed(self.h, _ptr(ty), _ptr(k), _ptr(v), k.numel(), _ptr(out_vals), _ptr(status),
+                                _stream_ptr(stream)), "hive_mixed")
+        return out_vals, status
+
+    # --------------------------------------------- reference record ops
+    def _ops(self, fn, ops, results, stream):
+        torch = _torch()
+        if isinstance(ops, np.ndarray):
+            ops = torch.from_numpy(ops.view(np.uint8)).cuda()
+        n = ops.numel() * ops.element_size() // 16
+        if results is None:
+            results = torch.empty(n, dtype=torch.int64, device=ops.device)
+        check(fn(self.h, _ptr(ops), n, _ptr(results), _stream_ptr(stream)), fn.__name__)
+        return results
+
+    def insert_ops(self, ops, results=None, stream=None):
+        return self._ops(self.L.hive_insert_ops, ops, results, stream)
+
+    def lookup_ops(self, ops, results=None, stream=None):
+        return self._ops(self.L.hive_lookup_ops, ops, results, stream)
+
+    def mixed_ops(self, ops, results=None, stream=None):
+        return self._ops(self.L.hive_mixed_ops, ops, results, stream)
+
+    # -------------------------------------------------------- maintenance
+    def resize(self, new_num_slots, stream=None):
+        check(self.L.hive_resize(self.h, int(new_num_slots), _stream_ptr(stream)), "hive_resize")
+
+    def stash_drain(self, stream=None):
+        torch = _torch()
+        cap = self.num_buckets * 8
+        out = torch.empty(max(cap, 1), dtype=torch.int64, device="cuda")
+        n = C.c_size_t(0)
+        check(self.L.hive_stash_drain(self.h, _ptr(out), cap, C.byref(n), _stream_ptr(stream)), "hive_stash_drain")
+        return out[: n.value]
+
+    def stash_reinsert(self, stream=None):
+        n = C.c_size_t(0)
+        check(self.L.hive_stash_reinsert(self.h, C.byref(n), _stream_ptr(stream)), "hive_stash_reinsert")
+        return n.value
+
+    def stats(self, stream=None):
+        s = Stats()
+        check(self.L.hive_stats(self.h, C.byref(s), _stream_ptr(stream)), "hive_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def count_exact(self, stream=None):
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        check(self.L.hive_count_exact(self.h, C.byref(a), C.byref(b), _stream_ptr(stream)), "hive_count_exact")
+        return a.value, b.value
+
+    def probe_stats(self, keys, stream=None):
+        """(insert candidate buckets, lookup bucket reads, lookup stash reads) for a key set."""
+        torch = _torch()
+        k = _dev(keys, torch.int32)
+        a, b, c = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        check(self.L.hive_probe_stats(self.h, _ptr(k), k.numel(), C.byref(a), C.byref(b), C.byref(c),
+                                      _stream_ptr(stream)), "hive_probe_stats")
+        return a.value, b.value, c.value
+
+    def export_raw(self, stream=None):
+        nb = self.num_buckets
+        slots = np.empty(nb * 8, np.uint64)
+        stash = np.empty(nb * 8, np.uint64)
+        cnt = np.empty(nb, np.uint32)
+        check(self.L.hive_export_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
+                                     cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_export_raw")
+        return slots, stash, cnt
+
+    def import_raw(self, slots, stash, stash_count, stream=None):
+        slots = np.ascontiguousarray(slots, np.uint64)
+        stash = np.ascontiguousarray(stash, np.uint64)
+        cnt = np.ascontiguousarray(stash_count, np.uint32)
+        nb = self.num_buckets
+        if slots.size != nb * 8 or stash.size != nb * 8 or cnt.size != nb:
+            raise ValueError("raw arrays do not match the table size")
+        check(self.L.hive_import_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
+                                     cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_import_raw")
+
+    def set_launch(self, op_type, block_threads, keys_per_lane=1):
+        cfg = Launch(block_threads, keys_per_lane)
+        check(self.L.hive_set_launch(self.h, op_type, C.byref(cfg)), "hive_set_launch")
+
+    def get_launch(self, op_type):
+        cfg = Launch()
+        check(self.L.hive_get_launch(self.h, op_type, C.byref(cfg)), "hive_get_launch")
+        return cfg.block_threads, cfg.keys_per_lane
+
+
+def make_ops(types, keys):
+    """numpy array of the reference's 16-byte Operation records."""
+    n = len(keys)
+    a = np.zeros(n, dtype=np.dtype([("type", "<i4"), ("pad", "<u4"), ("key", "<u8")]))
+    a["type"] = types
+    a["key"] = np.asarray(keys, dtype=np.uint64)
+    return a
+
+
+def dispatch(ops, table_size, threads_per_block=256, num_iterations=1, policy="Default2Hash", layout=0):
+    """The reference's hash_table_kernel_dispatch over host buffers.
+
+    Returns (results uint64[n], elapsed_ms per iteration).
+    """
+    L = lib()
+    ops = np.ascontiguousarray(ops)
+    n = len(ops)
+    res = np.zeros(n, np.uint64)
+    ms = C.c_double(0.0)
+    nblocks = (n + threads_per_block - 1) // threads_per_block
+    check(L.hive_hash_table_kernel_dispatch(ops.ctypes.data_as(C.c_void_p), n, table_size, threads_per_block,
+                                            nblocks, num_iterations, C.byref(ms), res.ctypes.data_as(C.c_void_p),
+                                            1, layout, policy.encode()), "hive_hash_table_kernel_dispatch")
+    return res, ms.value
+
+
+def dispatch_ycsb(prefill_ops, workload_ops, table_size, threads_per_block=256, num_iterations=1, layout=0):
+    """The reference's hash_table_kernel_dispatch_YCSB over host buffers."""
+    L = lib()
+    pre = np.ascontiguousarray(prefill_ops)
+    work = np.ascontiguousarray(workload_ops)
+    res = np.zeros(len(work), np.uint64)
+    ms = C.c_double(0.0)
+    check(L.hive_hash_table_kernel_dispatch_ycsb(pre.ctypes.data_as(C.c_void_p), work.ctypes.data_as(C.c_void_p),
+                                                 len(pre), len(work), table_size, threads_per_block, num_iterations,
+                                                 C.byref(ms), res.ctypes.data_as(C.c_void_p), 1, layout),
+          "hive_hash_table_kernel_dispatch_ycsb")
+    return res, ms.value
+
+
+__all__ = ["HiveTable", "make_ops", "dispatch", "dispatch_ycsb", "Operation", "OP_INSERT", "OP_LOOKUP", "OP_DELETE"]

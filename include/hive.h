@@ -1,0 +1,205 @@
+/*
+ * hive.h -- C-ABI of libhive.so, the MI355X (gfx950) Hive hash table.
+ *
+ * Plain C types only: device pointers are passed as typed pointers, HIP
+ * streams as void* (a hipStream_t; NULL = the default stream), sizes as
+ * size_t.  Every entry point returns hive_status_t and never exits the
+ * process.  Citations are to the reference (SabbirPulok/HiveHashTable)
+ * interfaces each entry point replaces.
+ *
+ * Table layout (device memory owned by the handle):
+ *   slots[nb * 8]   u64 packed kv = (uint64)value << 32 | key, 0 = empty
+ *                   (8 slots x 8 B = one 64-byte bucket, densely packed)
+ *   stash[nb * 8]   u64, one 8-slot overflow bucket per canonical bucket
+ *   stash_count[nb] u32 monotonic push pointer of each stash bucket
+ *   stash_flag[nb/32] u32 bitmap, bit set <=> stash_count > 0
+ *   locks[nb]       u32 per-canonical-bucket lock of the insert slow path
+ * Key 0 is the empty sentinel and is rejected by every operation
+ * (include/utils.h:40-47 of the reference).
+ */
+#ifndef HIVE_H
+#define HIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HIVE_OK = 0,
+  HIVE_EINVAL = 1,     /* bad argument, unknown policy, bad size */
+  HIVE_ENOMEM = 2,     /* device or host allocation failed */
+  HIVE_EDEVICE = 3,    /* HIP runtime error, or no usable device */
+  HIVE_ECAPACITY = 4,  /* resize target cannot hold the live entries */
+  HIVE_ERANGE = 5      /* output buffer too small */
+} hive_status_t;
+
+/* Hash policies: include/HashPolicies.cuh:10-65.  The string names accepted
+ * by hive_policy_from_name are the reference's (src/hive_dispatch.cu:829-864). */
+typedef enum {
+  HIVE_POLICY_DEFAULT2HASH = 0,   /* hash1, hash2 */
+  HIVE_POLICY_MURMURCITY = 1,     /* murmur, city */
+  HIVE_POLICY_TRIPLE = 2,         /* hash1, hash2, city */
+  HIVE_POLICY_MURMURCITYBIT = 3,  /* murmur, city, hash1 */
+  HIVE_POLICY_LOOKUP2 = 4,        /* crc64, crc32 */
+  HIVE_POLICY_LOOKUP3 = 5         /* crc64, crc32, hash1 */
+} hive_policy_t;
+
+/* Operation types, numbered as the reference's OperationType (include/utils.h:10-16). */
+enum { HIVE_OP_INSERT = 0, HIVE_OP_LOOKUP = 1, HIVE_OP_DELETE = 2, HIVE_OP_ATOMIC_INC = 3, HIVE_OP_NONE = 4 };
+
+/* Per-operation result codes (uint8_t per op). */
+enum {
+  HIVE_R_FAILED = 0,      /* insert: canonical bucket, cuckoo and stash all full */
+  HIVE_R_INSERTED = 1,    /* insert: new key placed in a bucket */
+  HIVE_R_UPDATED = 2,     /* insert: key existed, value replaced */
+  HIVE_R_STASHED = 3,     /* insert: new key placed in the overflow stash */
+  HIVE_R_FOUND = 4,       /* lookup hit */
+  HIVE_R_NOT_FOUND = 5,   /* lookup miss, or delete of an absent key */
+  HIVE_R_ERASED = 6,      /* delete removed the key */
+  HIVE_R_INVALID_KEY = 7, /* key 0 */
+  HIVE_R_NOOP = 8         /* NONE / ATOMIC_INC: no effect, result 0 (include/hive_kernels.cuh:181-184) */
+};
+
+/* The reference's 16-byte Operation (include/utils.h:18-21): int type, 4 pad bytes, uint64 key. */
+typedef struct {
+  int32_t type;
+  uint32_t pad;
+  uint64_t key;
+} hive_operation_t;
+
+typedef struct hive_table hive_table_t;
+
+typedef struct {
+  uint64_t num_buckets;
+  uint64_t slots_per_bucket; /* 8 */
+  uint64_t capacity;         /* num_buckets * 8 */
+  uint64_t table_live;       /* entries stored in buckets */
+  uint64_t stash_live;       /* entries stored in the stash */
+  double load_factor;        /* table_live / capacity, as count_occupancy_kernel (include/hive_kernels.cuh:303-325) */
+  uint64_t device_bytes;     /* device memory held by the handle */
+  int policy;
+} hive_stats_t;
+
+/* Launch shape of the plain bulk kernels.  0 = library default / autotuned. */
+typedef struct {
+  uint32_t block_threads; /* threads per block: 64..1024, multiple of 64 */
+  uint32_t keys_per_lane; /* 1, 2 or 4 keys processed per lane */
+} hive_launch_t;
+
+/* ---- lifecycle: replaces hiveHashTableCreate / Destroy / Clear
+ *      (include/hive_hash_table.cuh:92-144, src/hive_memory.cu:29-90,174-218).
+ * num_slots is the reference's table_size: num_buckets = num_slots / 8
+ * (src/hive_dispatch.cu:630-631).  The handle keeps the table resident on the
+ * current device until hive_destroy. */
+hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out);
+hive_status_t hive_destroy(hive_table_t* t);
+hive_status_t hive_clear(hive_table_t* t, void* stream);
+hive_status_t hive_policy_from_name(const char* name, int* policy);
+
+/* ---- bulk operations on device buffers, stream-ordered and asynchronous.
+ * vals may be NULL for hive_insert (value = key + 1, the reference's
+ * convention, include/hive_kernels.cuh:156,238).  Output pointers may be NULL.
+ * hive_insert  replaces hive_insert_kernel (include/hive_kernels.cuh:205-266)
+ * hive_lookup  replaces hive_lookup_kernel (include/hive_kernels.cuh:76-117),
+ *              returning the value as well as the found flag
+ * hive_erase   replaces the DELETE arm of hive_mixed_kernel (include/hive_kernels.cuh:175-180) */
+hive_status_t hive_insert(hive_table_t* t, const uint32_t* keys, const uint32_t* vals, size_t n,
+                          uint8_t* status, void* stream);
+hive_status_t hive_lookup(hive_table_t* t, const uint32_t* keys, size_t n, uint32_t* vals_out,
+                          uint8_t* status, void* stream);
+hive_status_t hive_erase(hive_table_t* t, const uint32_t* keys, size_t n, uint8_t* status, void* stream);
+
+/* Mixed batch: types[i] in HIVE_OP_*.  All operations of one call are
+ * concurrent; the library executes them as lookups, then deletes, then
+ * inserts (replaces hive_mixed_kernel, include/hive_kernels.cuh:119-192).
+ * vals may be NULL (value = key + 1). out_vals receives lookup values. */
+hive_status_t hive_mixed(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                         size_t n, uint32_t* out_vals, uint8_t* status, void* stream);
+
+/* The reference's kernel interface over its 16-byte Operation records with
+ * uint64 0/1 results and value = key + 1:
+ *   hive_insert_ops -> hive_insert_kernel (include/hive_kernels.cuh:205-266)
+ *   hive_lookup_ops -> hive_lookup_kernel (include/hive_kernels.cuh:76-117)
+ *   hive_mixed_ops  -> hive_mixed_kernel  (include/hive_kernels.cuh:119-192) */
+hive_status_t hive_insert_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results, void* stream);
+hive_status_t hive_lookup_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results, void* stream);
+hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results, void* stream);
+
+/* ---- maintenance
+ * hive_resize: rehash every live entry (buckets and stash) into
+ *   new_num_slots / 8 buckets; replaces hive_hash_grow / hive_hash_shrink
+ *   (include/hive_hash_resize.cuh:138-215,340-420) with a rehash that keeps
+ *   lookups consistent.  Returns HIVE_ECAPACITY (table unchanged) if an entry
+ *   could not be placed.  Synchronizes the stream.
+ * hive_stash_drain: flatten live stash entries into out_kv (device, packed
+ *   kv), reset the stash; replaces compact_stash_kernel +
+ *   reinsert_stash_into_next_batch (include/hive_stash_table.cuh:33-155).
+ *   *n_out (host) receives the entry count.  Synchronizes the stream.
+ * hive_stash_reinsert: drain the stash and insert the drained entries back
+ *   (with their values) through the normal insert path.  Synchronizes. */
+hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream);
+hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream);
+hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* stream);
+
+/* ---- observation (synchronize the stream)
+ * hive_stats replaces count_occupancy_kernel + the LF computation
+ *   (include/hive_kernels.cuh:303-325, src/hive_dispatch.cu:166-177).
+ * hive_count_exact scans the table instead of reading the counters.
+ * hive_export_raw / hive_import_raw copy the raw layout to / from host
+ *   memory (arrays sized num_buckets*8, num_buckets*8, num_buckets). */
+hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream);
+hive_status_t hive_count_exact(hive_table_t* t, uint64_t* table_live, uint64_t* stash_live, void* stream);
+/* Probe statistics for a key set against the current table (no reference
+ * counterpart; used for the algorithmic-bytes accounting): the distinct
+ * candidate buckets an insert reads, the buckets a lookup reads before its
+ * early exit, and the lookups that also read the stash. */
+hive_status_t hive_probe_stats(hive_table_t* t, const uint32_t* keys, size_t n, uint64_t* candidate_buckets,
+                               uint64_t* lookup_bucket_reads, uint64_t* lookup_stash_reads, void* stream);
+hive_status_t hive_export_raw(hive_table_t* t, uint64_t* h_slots, uint64_t* h_stash, uint32_t* h_stash_count,
+                              void* stream);
+hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const uint64_t* h_stash,
+                              const uint32_t* h_stash_count, void* stream);
+uint64_t hive_num_buckets(const hive_table_t* t);
+
+/* ---- launch shape of the plain bulk kernels (autotuner output) */
+hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t* cfg);
+hive_status_t hive_get_launch(const hive_table_t* t, int op_type, hive_launch_t* cfg);
+
+/* ---- drop-in pair with the reference's host dispatch semantics.
+ * Same arguments as hash_table_kernel_dispatch / hash_table_kernel_dispatch_YCSB
+ * (include/hive_hash_table.cuh:50-77, src/hive_dispatch.cu:317-549,815-872),
+ * with double& -> double*, std::string -> const char*, bool -> int,
+ * HashTableDataLayout -> int.  Host buffers in and out; a table is created
+ * and destroyed per call as in the reference.  Unknown policy names return
+ * HIVE_EINVAL (the reference prints and falls back to Default2Hash).
+ * elapsed_ms receives the mean kernel time per timed iteration. */
+hive_status_t hive_hash_table_kernel_dispatch(const hive_operation_t* h_ops, size_t num_ops, size_t table_size,
+                                              size_t threads_per_block, size_t num_blocks, size_t num_iterations,
+                                              double* elapsed_ms, uint64_t* h_results, int verify, int layout,
+                                              const char* hash_policy);
+hive_status_t hive_hash_table_kernel_dispatch_ycsb(const hive_operation_t* h_prefill_ops,
+                                                   const hive_operation_t* h_workload_ops, size_t num_prefill_ops,
+                                                   size_t num_workload_ops, size_t table_size,
+                                                   size_t threads_per_block, size_t num_iterations,
+                                                   double* elapsed_ms, uint64_t* h_results, int verify, int layout);
+
+/* ---- measurement kernels (same-footprint ceilings; no reference counterpart)
+ * hive_ceiling_gather: n random 64-byte reads over a buffer of footprint
+ *   bytes, one per lane (the bucket-read pattern of lookups).
+ * hive_ceiling_cas: n random 64-bit CAS over the same footprint.
+ * buf must hold footprint bytes of device memory; out gets one u32 per
+ * block (anti-dead-code sink, may be NULL for cas). */
+hive_status_t hive_ceiling_gather(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed,
+                                  uint32_t* sink, void* stream);
+hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed, void* stream);
+
+const char* hive_status_string(hive_status_t s);
+const char* hive_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIVE_H */

@@ -1,0 +1,71 @@
+"""The multithreaded CPU path (oracle/hive_cpu_mt.cpp, the timed CPU
+baseline) against the sequential oracle -- CPU only."""
+import numpy as np
+import pytest
+
+from hivehashtable_amd import workloads as W
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+@pytest.mark.parametrize("policy", ["Default2Hash", "MurmurCityHash", "TripleHash"])
+def test_cpu_bulk_insert_lookup_matches_oracle(O, policy):
+    slots = 1 << 16
+    n = W.keys_for_lf(slots, 0.8)
+    keys = W.distinct_keys(n, 42)
+    vals = W.values(n, 43)
+    lk, present = W.lookup_set(keys, n, 0.5, 44)
+    o = O.Oracle(slots // 8, policy)
+    so = o.insert(keys, vals)
+    c = O.CpuTable(slots // 8, policy, threads=8)
+    sc, _ = c.insert(keys, vals)
+    ok_o = np.isin(so, [O.R_INSERTED, O.R_STASHED, O.R_UPDATED])
+    ok_c = np.isin(sc, [O.R_INSERTED, O.R_STASHED, O.R_UPDATED])
+    assert ok_o.all() and ok_c.all()
+    raw = c.export()
+    assert np.array_equal(raw.live_kv(), o.dump())
+    chk = O.Oracle(slots // 8, policy)
+    chk.load(raw)
+    assert chk.check_invariants()[0] == 0
+    v1, s1 = o.lookup(lk)
+    v2, s2, _ = c.lookup(lk)
+    assert np.array_equal(s1, s2) and np.array_equal(v1, v2)
+    assert np.array_equal(s2 == O.R_FOUND, present)
+
+
+def test_cpu_high_load_keeps_invariants(O):
+    slots = 1 << 14
+    keys = W.distinct_keys(slots, 5)
+    c = O.CpuTable(slots // 8, threads=8)
+    st, _ = c.insert(keys)
+    raw = c.export()
+    chk = O.Oracle(slots // 8)
+    chk.load(raw)
+    r, msg = chk.check_invariants()
+    assert r == 0, msg
+    ok = np.isin(st, [O.R_INSERTED, O.R_STASHED])
+    assert raw.live_kv().size == ok.sum()
+    v, s, _ = c.lookup(keys)
+    assert np.array_equal(s == O.R_FOUND, ok)
+
+
+def test_cpu_mixed_matches_phased_oracle(O):
+    slots = 1 << 15
+    pre = W.distinct_keys(8000, 9)
+    o = O.Oracle(slots // 8)
+    c = O.CpuTable(slots // 8, threads=8)
+    o.insert(pre)
+    c.insert(pre)
+    for types, keys, vals in W.mixed_batches(pre, 1 << 17, [5000, 3000], seed=3):
+        out_o, st_o = o.mixed(types, keys, vals)
+        out_c, st_c, _ = c.mixed(types, keys, vals)
+        # unique keys in a batch have one admissible outcome
+        uniq, cnt = np.unique(keys, return_counts=True)
+        single = np.isin(keys, uniq[cnt == 1])
+        norm = lambda s: np.where(s == O.R_STASHED, O.R_INSERTED, s)
+        assert np.array_equal(norm(st_o)[single], norm(st_c)[single])
+        assert np.array_equal(out_o[single], out_c[single])
+    assert np.array_equal(c.export().live_kv(), o.dump())
