@@ -1,0 +1,297 @@
+#!/usr/bin/env python3
+"""Headline benchmark: bulk insert + bulk lookup on a 2^26-slot Hive table.
+
+One step = clear the table, bulk-insert 60,397,977 distinct uniform u32 keys
+(load factor 0.9) with uniform u32 values, then bulk-look-up 2^26 keys of
+which 50% are present (Default2Hash).  Inputs are resident in HBM before the
+timed region.  value = (inserts + lookups) per second over the timed steps,
+in Mops/s, on one GPU.
+
+Also reported: per-kernel times (HIP events on the launch stream), the
+roofline of the dominant kernel against the 8 TB/s HBM peak, the measured
+same-footprint random-gather / CAS ceilings, and the multithreaded CPU path
+(oracle/hive_cpu_mt.cpp) timed on the host on one full step.
+
+    python bench.py [--steps K] [--warmup W] [--lf 0.9] [--log2-slots 26]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Mops/s insert+lookup at 2^26 slots LF 0.9; mixed-op p50/p99 batch latency"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--log2-slots", type=int, default=26)
+    p.add_argument("--lf", type=float, default=0.9)
+    p.add_argument("--exist", type=float, default=0.5)
+    p.add_argument("--policy", default="Default2Hash")
+    p.add_argument("--seed", type=int, default=44)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device copy of BYTES (reference point for the HBM byte counters)")
+    return p.parse_args()
+
+
+def copy_calibration(nbytes):
+    """Three plain device-to-device copies of nbytes: the counter reading of
+    one such dispatch is the scale for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "dispatches": 3}), flush=True)
+    return 0
+
+
+def git_sha():
+    try:
+        return subprocess.run(["git", "rev-parse", "--short", "HEAD"], cwd=ROOT, capture_output=True,
+                              text=True).stdout.strip() or None
+    except Exception:
+        return None
+
+
+def traffic_from_profiles(lf):
+    """HBM bytes per launch of the dominant kernel, from the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_bulk_2p26.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(f"lf{int(round(lf * 100)):03d}")
+    except Exception:
+        return None
+
+
+def cpu_baseline(keys, vals, lk, threads, gpu_found, gpu_vals, policy, nb):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    c = O.CpuTable(nb, policy, threads=threads)
+    st, t_ins = c.insert(keys, vals)
+    v, s, t_look = c.lookup(lk)
+    agree = bool(np.array_equal(s == O.R_FOUND, gpu_found) and np.array_equal(v, gpu_vals))
+    ops = keys.size + lk.size
+    del c
+    return {
+        "value": round(ops / (t_ins + t_look) / 1e6, 3),
+        "unit": "Mops/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one full step at the same size: {keys.size} inserts + {lk.size} lookups, "
+                   f"{threads} std::thread workers"),
+        "insert_mops": round(keys.size / t_ins / 1e6, 3),
+        "lookup_mops": round(lk.size / t_look / 1e6, 3),
+        "seconds": round(t_ins + t_look, 3),
+        "lookups_agree_with_gpu": agree,
+    }
+
+
+def main():
+    a = parse()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build runs on a single "
+              f"MI355X, see DESIGN.md for how the table would shard).", file=sys.stderr)
+        return 2
+    import torch
+    import hivehashtable_amd as H
+    from hivehashtable_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    H.lib()  # fails loudly if libhive.so is missing
+    slots = 1 << a.log2_slots
+    nb = slots // 8
+    n_ins = W.keys_for_lf(slots, a.lf)
+    n_look = slots
+    t0 = time.time()
+    keys = W.distinct_keys(n_ins, a.seed)
+    vals = W.values(n_ins, a.seed + 1000)
+    lk, present = W.lookup_set(keys, n_look, a.exist, a.seed + 2000)
+    log(f"inputs: {n_ins} inserts, {n_look} lookups ({a.exist:.0%} present) in {time.time() - t0:.1f}s")
+
+    dev = torch.device("cuda:0")
+    d_keys = torch.from_numpy(keys.view(np.int32)).to(dev)
+    d_vals = torch.from_numpy(vals.view(np.int32)).to(dev)
+    d_lk = torch.from_numpy(lk.view(np.int32)).to(dev)
+    d_st = torch.empty(n_ins, dtype=torch.uint8, device=dev)
+    d_lv = torch.empty(n_look, dtype=torch.int32, device=dev)
+    d_ls = torch.empty(n_look, dtype=torch.uint8, device=dev)
+    t = H.HiveTable(slots, a.policy)
+    if a.block:
+        for op in (0, 1, 2):
+            t.set_launch(op, a.block)
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        t.clear()
+        if ev is not None:
+            ev[1].record(stream)
+        t.insert(d_keys, d_vals, status=d_st)
+        if ev is not None:
+            ev[2].record(stream)
+        t.lookup(d_lk, vals=d_lv, status=d_ls)
+        if ev is not None:
+            ev[3].record(stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warmup {a.warmup} done")
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(a.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_step = wall / a.steps * 1e3
+    clear_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    ins_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    look_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in events]))
+    value = (n_ins + n_look) / (ms_step / 1e3) / 1e6
+    log(f"timed {a.steps} steps: {ms_step:.3f} ms/step (clear {clear_ms:.3f}, insert {ins_ms:.3f}, "
+        f"lookup {look_ms:.3f} ms)")
+
+    # correctness of the last step (size-independent properties)
+    st = d_st.cpu().numpy()
+    ls = d_ls.cpu().numpy()
+    lv = d_lv.cpu().numpy().view(np.uint32)
+    placed = np.isin(st, (1, 3))
+    found = ls == 4
+    ok_keys = set() if placed.all() else set(keys[~placed].tolist())
+    expect_found = present.copy()
+    if ok_keys:
+        expect_found &= ~np.isin(lk, np.array(sorted(ok_keys), np.uint32))
+    srt = np.argsort(keys)
+    pos = np.searchsorted(keys[srt], lk)
+    pos[pos >= keys.size] = 0
+    expect_val = np.where(found, vals[srt][pos], 0).astype(np.uint32)
+    verified = bool(np.array_equal(found, expect_found) and np.array_equal(lv[found], expect_val[found]))
+    log(f"verified={verified} placed={placed.mean():.6f} found={found.mean():.4f}")
+
+    # algorithmic bytes (probe statistics of this exact table and key sets)
+    cand, _, _ = t.probe_stats(d_keys)
+    _, lreads, lstash = t.probe_stats(d_lk)
+    ins_bytes = n_ins * (4 + 4 + 1) + 64 * cand + 8 * int(placed.sum())
+    look_bytes = n_look * (4 + 4 + 1) + 64 * lreads + (64 + 4) * lstash
+    ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
+    look_gbps = look_bytes / (look_ms / 1e3) / 1e9
+    dominant = "insert" if ins_ms >= look_ms else "lookup"
+    achieved = ins_gbps if dominant == "insert" else look_gbps
+
+    # same-footprint random-access ceilings
+    buf = torch.zeros(nb * 8, dtype=torch.int64, device=dev)
+    sink = torch.zeros(1 << 20, dtype=torch.int32, device=dev)
+    L = H.lib()
+    import ctypes as C
+    sptr = C.c_void_p(stream.cuda_stream)
+    n_g = lreads
+    L.hive_ceiling_gather(C.c_void_p(buf.data_ptr()), nb * 64, n_g, 7, C.c_void_p(sink.data_ptr()), sptr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    gms = []
+    for r in range(3):
+        e0.record(stream)
+        L.hive_ceiling_gather(C.c_void_p(buf.data_ptr()), nb * 64, n_g, 11 + r, C.c_void_p(sink.data_ptr()), sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        gms.append(e0.elapsed_time(e1))
+    gather_rate = n_g / (min(gms) / 1e3)  # random 64-byte reads per second
+    cms = []
+    for r in range(3):
+        e0.record(stream)
+        L.hive_ceiling_cas(C.c_void_p(buf.data_ptr()), nb * 64, n_ins, 13 + r, sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        cms.append(e0.elapsed_time(e1))
+    cas_rate = n_ins / (min(cms) / 1e3)
+    del buf, sink
+    look_read_rate = lreads / (look_ms / 1e3)
+    ins_read_rate = cand / (ins_ms / 1e3)
+
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline: {threads} threads, one full step")
+        cpu = cpu_baseline(keys, vals, lk, threads, found, lv, a.policy, nb)
+        log(f"cpu: {cpu['value']} Mops/s in {cpu['seconds']} s, agree={cpu['lookups_agree_with_gpu']}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"bulk insert {n_ins} distinct uniform u32 keys (LF {a.lf}) then bulk lookup {n_look} keys "
+                         f"({a.exist:.0%} present) on a 2^{a.log2_slots}-slot table, {a.policy}"),
+            "slots": slots, "load_factor": a.lf, "exist_ratio": a.exist, "policy": a.policy,
+            "inserts": n_ins, "lookups": n_look, "parallelism": "single GPU",
+        },
+        "insert_mops": round(n_ins / (ins_ms / 1e3) / 1e6, 3),
+        "lookup_mops": round(n_look / (look_ms / 1e3) / 1e6, 3),
+        "kernel_ms": {"clear": round(clear_ms, 4), "insert": round(ins_ms, 4), "lookup": round(look_ms, 4)},
+        "verified": verified,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 5),
+            "traffic": traffic_from_profiles(a.lf),
+            "algorithmic_bytes": {"insert": int(ins_bytes), "lookup": int(look_bytes)},
+            "achieved_gbps": {"insert": round(ins_gbps, 2), "lookup": round(look_gbps, 2)},
+        },
+        "ceiling": {
+            "random_gather_64B_per_s": round(gather_rate, 1),
+            "random_cas_u64_per_s": round(cas_rate, 1),
+            "lookup_bucket_reads_per_s": round(look_read_rate, 1),
+            "lookup_frac_of_gather_ceiling": round(look_read_rate / gather_rate, 4),
+            "insert_bucket_reads_per_s": round(ins_read_rate, 1),
+            "insert_frac_of_gather_ceiling": round(ins_read_rate / gather_rate, 4),
+            "footprint_bytes": nb * 64,
+        },
+        "cpu_baseline": cpu,
+        "git": git_sha(),
+    }
+    print(json.dumps(out), flush=True)
+    t.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

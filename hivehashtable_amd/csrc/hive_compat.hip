@@ -11,9 +11,11 @@
 //   last launch's 0/1 values for the first num_ops ops; elapsed_ms is the mean
 //   kernel time per timed launch.
 // Deliberate differences: the stash is cleared together with the table
-// (hiveHashTableClear leaves it, src/hive_memory.cu:203-218), unknown policy
-// names are an error, and the 30%-of-free-HBM pre-reservation
-// (include/hive_hash_table_struct.cuh:38-48) is not made.
+// (hiveHashTableClear leaves it, src/hive_memory.cu:203-218), an all-LOOKUP
+// batch does not drain the stash between iterations (the reference's drain
+// loses those entries there), unknown policy names are an error, and the
+// 30%-of-free-HBM pre-reservation (include/hive_hash_table_struct.cuh:38-48)
+// is not made.
 //
 // hive_hash_table_kernel_dispatch_ycsb restates hash_table_kernel_dispatch_YCSB
 // (src/hive_dispatch.cu:317-549): prefill with the bulk insert kernel, then 5
@@ -153,7 +155,10 @@ hive_status_t hive_hash_table_kernel_dispatch(const hive_operation_t* h_ops, siz
     float ms = 0.f;
     HK(hipEventElapsedTime(&ms, ev.a, ev.b));
     total_ms += ms;
-    if (it + 1 < iters) {  // stash entries become INSERT ops of the next launch
+    // stash entries become INSERT ops of the next launch.  Not for all-LOOKUP
+    // batches: hive_lookup_kernel treats the appended records as lookups, so
+    // in the reference the drained entries are simply lost.
+    if (it + 1 < iters && kind != Kind::Lookup) {
       size_t drained = 0;
       CK(hive_stash_drain(tb.t, static_cast<uint64_t*>(d_flat.p), stash_cap, &drained, s));
       if (drained)

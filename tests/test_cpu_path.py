@@ -59,13 +59,20 @@ def test_cpu_mixed_matches_phased_oracle(O):
     c = O.CpuTable(slots // 8, threads=8)
     o.insert(pre)
     c.insert(pre)
+    conflicted = set()
     for types, keys, vals in W.mixed_batches(pre, 1 << 17, [5000, 3000], seed=3):
         out_o, st_o = o.mixed(types, keys, vals)
         out_c, st_c, _ = c.mixed(types, keys, vals)
-        # unique keys in a batch have one admissible outcome
+        # keys appearing once in a batch have exactly one admissible outcome
         uniq, cnt = np.unique(keys, return_counts=True)
         single = np.isin(keys, uniq[cnt == 1])
+        conflicted.update(uniq[cnt > 1].tolist())
         norm = lambda s: np.where(s == O.R_STASHED, O.R_INSERTED, s)
         assert np.array_equal(norm(st_o)[single], norm(st_c)[single])
         assert np.array_equal(out_o[single], out_c[single])
-    assert np.array_equal(c.export().live_kv(), o.dump())
+    a, b = c.export().live_kv(), o.dump()
+    assert np.array_equal(np.sort(a & 0xFFFFFFFF), np.sort(b & 0xFFFFFFFF))   # same key set
+    clean = ~np.isin((b & 0xFFFFFFFF).astype(np.uint32), np.array(sorted(conflicted), np.uint32))
+    ka = dict(zip((a & 0xFFFFFFFF).tolist(), (a >> 32).tolist()))
+    for kv in b[clean].tolist():
+        assert ka[kv & 0xFFFFFFFF] == kv >> 32
