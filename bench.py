@@ -131,7 +131,7 @@ def main():
     t0 = time.time()
     keys = W.distinct_keys(n_ins, a.seed)
     vals = W.values(n_ins, a.seed + 1000)
-    lk, present = W.lookup_set(keys, n_look, a.exist, a.seed + 2000)
+    lk, present = W.lookup_set(keys, n_look, a.exist, a.seed + 2000, key_seed=a.seed)
     log(f"inputs: {n_ins} inserts, {n_look} lookups ({a.exist:.0%} present) in {time.time() - t0:.1f}s")
 
     dev = torch.device("cuda:0")
@@ -199,7 +199,7 @@ def main():
     # algorithmic bytes (probe statistics of this exact table and key sets)
     cand, _, _ = t.probe_stats(d_keys)
     _, lreads, lstash = t.probe_stats(d_lk)
-    ins_bytes = n_ins * (4 + 4 + 1) + 64 * cand + 8 * int(placed.sum())
+    ins_bytes = n_ins * (4 + 4 + 1 + 4) + 64 * cand + 8 * int(placed.sum())  # +4: hint word of H0
     look_bytes = n_look * (4 + 4 + 1) + 64 * lreads + (64 + 4) * lstash
     ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
     look_gbps = look_bytes / (look_ms / 1e3) / 1e9

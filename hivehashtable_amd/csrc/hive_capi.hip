@@ -49,14 +49,14 @@ struct hive_table {
 
 namespace {
 
-size_t flag_words(uint32_t nb) { return (static_cast<size_t>(nb) + 31) / 32; }
+size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
 size_t counter_words() { return static_cast<size_t>(2) * kCounterLanes * kCounterStride; }
 
 void free_view(TableView& v) {
   if (v.slots) (void)hipFree(v.slots);
   if (v.stash) (void)hipFree(v.stash);
   if (v.stash_count) (void)hipFree(v.stash_count);
-  if (v.stash_flag) (void)hipFree(v.stash_flag);
+  if (v.meta) (void)hipFree(v.meta);
   if (v.locks) (void)hipFree(v.locks);
   if (v.counters) (void)hipFree(v.counters);
   v = TableView{};
@@ -64,7 +64,7 @@ void free_view(TableView& v) {
 
 size_t view_bytes(uint32_t nb) {
   return static_cast<size_t>(nb) * kSlots * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
-         static_cast<size_t>(nb) * 4 * 2 + flag_words(nb) * 4 + counter_words() * 8;
+         static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_words() * 8;
 }
 
 hive_status_t alloc_view(TableView& v, uint32_t nb) {
@@ -74,7 +74,7 @@ hive_status_t alloc_view(TableView& v, uint32_t nb) {
   if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kSlots * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.stash_flag, flag_words(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
       (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
       (e = hipMalloc(&v.counters, counter_words() * 8)) != hipSuccess) {
     free_view(v);
@@ -91,13 +91,13 @@ unsigned fill_grid(size_t n) {
   return static_cast<unsigned>(g ? g : 1);
 }
 
-// zero buckets, stash pointers and flags, locks and counters (stash slots at or
+// zero buckets, stash pointers, hint bits, locks and counters (stash slots at or
 // above a bucket's count are never read, so the stash body is left as is).
 hive_status_t clear_view(const TableView& v, hipStream_t s) {
   const size_t ns = static_cast<size_t>(v.nb) * kSlots;
   k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
   k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
-  k_fill_u32<<<fill_grid(flag_words(v.nb)), 256, 0, s>>>(v.stash_flag, flag_words(v.nb), 0u);
+  k_fill_u32<<<fill_grid(meta_words(v.nb)), 256, 0, s>>>(v.meta, meta_words(v.nb), 0u);
   k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.locks, v.nb, 0u);
   k_fill_u64<<<fill_grid(counter_words()), 256, 0, s>>>(v.counters, counter_words(), 0ull);
   HIP_CHECK_LAUNCH();
@@ -402,7 +402,7 @@ hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, si
     HIP_CHECK_LAUNCH();
   }
   k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.stash_count, t->v.nb, 0u);
-  k_fill_u32<<<fill_grid(flag_words(t->v.nb)), 256, 0, s>>>(t->v.stash_flag, flag_words(t->v.nb), 0u);
+  k_clear_stash_bits<<<grid_for(meta_words(t->v.nb), 256), 256, 0, s>>>(t->v);
   HIP_CHECK_LAUNCH();
   uint64_t tab = 0, st = 0;
   if ((r = read_counters(t, s, &tab, &st)) != HIVE_OK) return r;
@@ -466,17 +466,32 @@ hive_status_t hive_probe_stats(hive_table_t* t, const uint32_t* keys, size_t n, 
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   if (n && !keys) return fail(HIVE_EINVAL, "null keys");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipMemsetAsync(t->scratch, 0, 24, s));
-  if (n) {
-    HIVE_POLICY_SWITCH(t->policy, (k_probe_stats<P><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, t->scratch)));
-    HIP_CHECK_LAUNCH();
+  constexpr size_t kWords = 64 * 8;
+  unsigned long long* acc = nullptr;
+  HIP_TRY(hipMalloc(&acc, kWords * 8));
+  std::vector<unsigned long long> h(kWords, 0ull);
+  hipError_t e = hipMemsetAsync(acc, 0, kWords * 8, s);
+  if (e == hipSuccess && n) {
+    switch (t->policy) {
+      case HIVE_POLICY_DEFAULT2HASH: k_probe_stats<PolicyDefault2><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+      case HIVE_POLICY_MURMURCITY: k_probe_stats<PolicyMurmurCity><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+      case HIVE_POLICY_TRIPLE: k_probe_stats<PolicyTriple><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+      case HIVE_POLICY_MURMURCITYBIT: k_probe_stats<PolicyMurmurCityBit><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+      case HIVE_POLICY_LOOKUP2: k_probe_stats<PolicyLookup2><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+      default: k_probe_stats<PolicyLookup3><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
+    }
+    e = hipGetLastError();
   }
-  unsigned long long h[3];
-  HIP_TRY(hipMemcpyAsync(h, t->scratch, 24, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  if (candidate_buckets) *candidate_buckets = h[0];
-  if (lookup_bucket_reads) *lookup_bucket_reads = h[1];
-  if (lookup_stash_reads) *lookup_stash_reads = h[2];
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), acc, kWords * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(acc);
+  if (e != hipSuccess) return fail(HIVE_EDEVICE, std::string("probe stats: ") + hipGetErrorString(e));
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  for (int l = 0; l < 64; ++l)
+    for (int f = 0; f < 4; ++f) tot[f] += h[l * 8 + f];
+  if (candidate_buckets) *candidate_buckets = tot[0];
+  if (lookup_bucket_reads) *lookup_bucket_reads = tot[1];
+  if (lookup_stash_reads) *lookup_stash_reads = tot[2];
   return HIVE_OK;
 }
 
@@ -505,7 +520,8 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
   HIP_TRY(hipMemcpyAsync(t->v.slots, h_slots, nb * kSlots * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash_count, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));
-  k_rebuild_flags<<<grid_for(flag_words(t->v.nb), 256), 256, 0, s>>>(t->v);
+  k_fill_u32<<<fill_grid(meta_words(t->v.nb)), 256, 0, s>>>(t->v.meta, meta_words(t->v.nb), 0u);
+  HIVE_POLICY_SWITCH(t->policy, (k_rebuild_meta<P><<<grid_for(nb * kSlots, 256), 256, 0, s>>>(t->v)));
   k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.locks, t->v.nb, 0u);
   HIP_CHECK_LAUNCH();
   uint64_t a = 0, b = 0;

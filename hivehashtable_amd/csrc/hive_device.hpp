@@ -29,6 +29,9 @@ constexpr int kCounterStride = 16;  // u64 words between counter lanes (128 B)
 constexpr uint8_t R_FAILED = 0, R_INSERTED = 1, R_UPDATED = 2, R_STASHED = 3, R_FOUND = 4,
                   R_NOT_FOUND = 5, R_ERASED = 6, R_INVALID_KEY = 7, R_NOOP = 8, R_PENDING = 0xFF;
 constexpr int OP_INSERT = 0, OP_LOOKUP = 1, OP_DELETE = 2;
+// per-bucket hint bits in TableView::meta
+constexpr uint32_t kOverflowBit = 1u;  // a key whose H0 is this bucket may live in another candidate bucket
+constexpr uint32_t kStashBit = 2u;     // this bucket's stash holds pushed entries (count > 0)
 
 struct Op16 {  // the reference's Operation record (include/utils.h:18-21)
   int32_t type;
@@ -40,7 +43,7 @@ struct TableView {
   unsigned long long* slots;        // nb * 8
   unsigned long long* stash;        // nb * 8
   uint32_t* stash_count;            // nb
-  uint32_t* stash_flag;             // (nb + 31) / 32
+  uint32_t* meta;                   // (nb + 15) / 16 words, 2 bits per bucket: kOverflowBit, kStashBit
   uint32_t* locks;                  // nb
   unsigned long long* counters;     // [2][kCounterLanes][kCounterStride]: table, stash
   uint32_t nb;
@@ -246,8 +249,21 @@ __device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[kSl
   return m;
 }
 
-__device__ __forceinline__ bool stash_flag_set(const TableView& t, uint32_t b) {
-  return (t.stash_flag[b >> 5] >> (b & 31)) & 1u;
+// ------------------------------------------------------------- hint bits
+// meta packs 2 bits per bucket, 16 buckets per 32-bit word (2 MiB for 2^23
+// buckets, small enough to stay L2-resident under the random bucket stream).
+//  kOverflowBit of bucket c is set before any key whose H0 is c is moved out of
+//  c by a cuckoo kick.  A key leaves H0 only that way (Stage B places into H0;
+//  the kick that moves a victim to a non-primary bucket sets the bit of the
+//  victim's H0), so while the bit is clear no key with H0 == c lives in another
+//  candidate bucket and probes of H1/H2 can be skipped.  The bit is
+//  conservative: deletes never clear it; clear() and resize() rebuild it.
+//  kStashBit of c is set when stash[c] receives its first entry (the
+//  reference's freeMask bit 31, include/hive_hash_insert.cuh:512-514).
+__device__ __forceinline__ uint32_t meta_bits(uint32_t word, uint32_t b) { return (word >> ((b & 15) * 2)) & 3u; }
+__device__ __forceinline__ uint32_t meta_word(const TableView& t, uint32_t b) { return t.meta[b >> 4]; }
+__device__ __forceinline__ void meta_set(const TableView& t, uint32_t b, uint32_t bit) {
+  __hip_atomic_fetch_or(t.meta + (b >> 4), bit << ((b & 15) * 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ unsigned long long* bucket_ptr(const TableView& t, uint32_t b) {
@@ -257,9 +273,18 @@ __device__ __forceinline__ unsigned long long* stash_ptr(const TableView& t, uin
   return t.stash + static_cast<size_t>(b) * kStashSlots;
 }
 
+template <class P>
+__device__ __forceinline__ bool is_dup(const uint32_t (&b)[P::NH], int i) {
+  bool dup = false;
+#pragma unroll
+  for (int j = 0; j < i; ++j) dup |= b[j] == b[i];
+  return dup;
+}
+
 // ------------------------------------------------------------------ insert
 // Fast path of one insert, run by every lane with plain loads:
 //   Stage A  (include/hive_hash_insert.cuh:911-930)  update in any candidate
+//            (H1/H2 are read only when H0's overflow bit is set)
 //   Stage B  (include/hive_hash_insert.cuh:271-310)  claim lowest empty H0 slot
 // Returns R_PENDING when the stash must be consulted, the canonical bucket is
 // full, or a CAS revealed a concurrent change; the op then takes slow_insert.
@@ -268,21 +293,33 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
   if (key == 0) return R_INVALID_KEY;
   const unsigned long long kv = kv_pack(key, val);
   uint32_t b[P::NH];
-  unsigned long long s[P::NH][kSlots];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-#pragma unroll
-  for (int i = 0; i < P::NH; ++i) load_bucket(bucket_ptr(t, b[i]), s[i]);
-#pragma unroll
-  for (int i = 0; i < P::NH; ++i) {
-    uint32_t m = match_mask(s[i], key);
+  unsigned long long s0[kSlots];
+  load_bucket(bucket_ptr(t, b[0]), s0);
+  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+  {
+    const uint32_t m = match_mask(s0, key);
     if (m) {
       const int j = __builtin_ctz(m);
-      return cas64(bucket_ptr(t, b[i]) + j, s[i][j], kv) == s[i][j] ? R_UPDATED : R_PENDING;
+      return cas64(bucket_ptr(t, b[0]) + j, s0[j], kv) == s0[j] ? R_UPDATED : R_PENDING;
     }
   }
-  if (stash_flag_set(t, b[0])) return R_PENDING;  // Stage A' runs under the lock
-  uint32_t e = empty_mask(s[0]);
+  if (hint & kOverflowBit) {
+#pragma unroll
+    for (int i = 1; i < P::NH; ++i) {
+      if (is_dup<P>(b, i)) continue;
+      unsigned long long s[kSlots];
+      load_bucket(bucket_ptr(t, b[i]), s);
+      const uint32_t m = match_mask(s, key);
+      if (m) {
+        const int j = __builtin_ctz(m);
+        return cas64(bucket_ptr(t, b[i]) + j, s[j], kv) == s[j] ? R_UPDATED : R_PENDING;
+      }
+    }
+  }
+  if (hint & kStashBit) return R_PENDING;  // Stage A' runs under the lock
+  uint32_t e = empty_mask(s0);
   unsigned long long* cb = bucket_ptr(t, b[0]);
   while (e) {
     const int j = __builtin_ctz(e);
@@ -309,6 +346,32 @@ __device__ __forceinline__ int locked_update(unsigned long long* bk, int nslots,
     return -1;
   }
   return 0;
+}
+
+// Move the victim held in cb[vs] into slot ad[d] (already claimed with a copy
+// of `victim`), replacing it with kv.  If a concurrent update changed the
+// victim's value in the source slot, the copy is refreshed to the new value
+// and the swap retried, so the destination slot never empties again.
+// Returns true when kv is in place.
+__device__ __forceinline__ bool finish_move(unsigned long long* src, unsigned long long* dst,
+                                            unsigned long long victim, unsigned long long kv) {
+  for (int spin = 0; spin < 64; ++spin) {
+    const unsigned long long seen = cas64(src, victim, kv);
+    if (seen == victim) return true;
+    if (seen == 0ull || kv_key(seen) != kv_key(victim)) break;
+    // source holds a newer value of the same key: carry it into the copy
+    const unsigned long long prev = cas64(dst, victim, seen);
+    if (prev != victim) break;
+    victim = seen;
+  }
+  // withdraw the copy (only reachable if the source stopped holding the victim)
+  unsigned long long cur = victim;
+  for (int spin = 0; spin < 64; ++spin) {
+    const unsigned long long seen = cas64(dst, cur, 0ull);
+    if (seen == cur || seen == 0ull || kv_key(seen) != kv_key(victim)) break;
+    cur = seen;
+  }
+  return false;
 }
 
 // The whole protocol for one key while holding lock(H0(key)):
@@ -356,12 +419,16 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       const int vs = kick % kSlots;
       const unsigned long long victim = cs[vs];
       if (victim == 0ull) continue;
-      const uint32_t a = alt_bucket<P>(kv_key(victim), c, t.nb);
+      const uint32_t vkey = kv_key(victim);
+      const uint32_t a = alt_bucket<P>(vkey, c, t.nb);
       if (a == c) continue;
       unsigned long long* ab = bucket_ptr(t, a);
       unsigned long long as[kSlots];
       load_bucket_sync(ab, as);
       uint32_t ae = empty_mask(as);
+      if (!ae) continue;
+      const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
+      if (a != vh0) meta_set(t, vh0, kOverflowBit);  // before the victim can leave its H0
       int d = -1;
       while (ae) {
         const int j = __builtin_ctz(ae);
@@ -369,15 +436,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
         ae &= ae - 1;
       }
       if (d < 0) continue;
-      if (cas64(cb + vs, victim, kv) == victim) return R_INSERTED;
-      // the victim's slot changed (a concurrent update of the victim key):
-      // withdraw the copy and start over
-      unsigned long long cur = victim;
-      for (int spin = 0; spin < 64; ++spin) {
-        const unsigned long long seen = cas64(ab + d, cur, 0ull);
-        if (seen == cur || seen == 0ull || kv_key(seen) != kv_key(victim)) break;
-        cur = seen;
-      }
+      if (finish_move(cb + vs, ab + d, victim, kv)) return R_INSERTED;
       retry = true;
       break;
     }
@@ -385,7 +444,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     if (cnt < kStashSlots) {
       __hip_atomic_store(sb + cnt, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(t.stash_count + c, cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_or(t.stash_flag + (c >> 5), 1u << (c & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      meta_set(t, c, kStashBit);
       return R_STASHED;
     }
     return R_FAILED;
@@ -414,7 +473,9 @@ __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
 }
 
 // ------------------------------------------------------------------ lookup
-// include/hive_hash_lookup.cuh:126-155; returns R_FOUND / R_NOT_FOUND.
+// include/hive_hash_lookup.cuh:126-155: H0, then the other candidates (only
+// if H0's overflow bit is set), then stash[H0] (if its stash bit is set).
+// The hint word is read only after an H0 miss.  Returns R_FOUND / R_NOT_FOUND.
 template <class P>
 __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, uint32_t& val) {
   val = 0;
@@ -422,21 +483,30 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-#pragma unroll
-  for (int i = 0; i < P::NH; ++i) {
-    bool dup = false;
-#pragma unroll
-    for (int j = 0; j < i; ++j) dup |= b[j] == b[i];
-    if (dup) continue;
+  {
     unsigned long long s[kSlots];
-    load_bucket(bucket_ptr(t, b[i]), s);
+    load_bucket(bucket_ptr(t, b[0]), s);
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
   }
-  if (stash_flag_set(t, b[0])) {
+  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+  if (hint & kOverflowBit) {
+#pragma unroll
+    for (int i = 1; i < P::NH; ++i) {
+      if (is_dup<P>(b, i)) continue;
+      unsigned long long s[kSlots];
+      load_bucket(bucket_ptr(t, b[i]), s);
+      const uint32_t m = match_mask(s, key);
+      if (m) {
+        val = kv_val(s[__builtin_ctz(m)]);
+        return R_FOUND;
+      }
+    }
+  }
+  if (hint & kStashBit) {
     uint32_t cnt = t.stash_count[b[0]];
     if (cnt > kStashSlots) cnt = kStashSlots;
     const unsigned long long* sb = stash_ptr(t, b[0]);
@@ -452,9 +522,9 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
 }
 
 // ------------------------------------------------------------------- erase
-// include/hive_hash_delete.cuh:168-194: lowest match in every candidate bucket,
-// then the canonical stash (count unchanged).  removed_tab / removed_stash
-// receive the number of copies removed.
+// include/hive_hash_delete.cuh:168-194: lowest match in every candidate bucket
+// (the others only if H0's overflow bit is set), then the canonical stash
+// (count unchanged).  removed_tab / removed_stash receive the copies removed.
 __device__ __forceinline__ int erase_in(unsigned long long* bk, int nslots, uint32_t key) {
   for (int j = 0; j < nslots; ++j) {
     unsigned long long cur = bk[j];
@@ -478,9 +548,13 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+  removed_tab += erase_in(bucket_ptr(t, b[0]), kSlots, key);
+  if (hint & kOverflowBit) {
 #pragma unroll
-  for (int i = 0; i < P::NH; ++i) removed_tab += erase_in(bucket_ptr(t, b[i]), kSlots, key);
-  if (stash_flag_set(t, b[0])) {
+    for (int i = 1; i < P::NH; ++i) removed_tab += erase_in(bucket_ptr(t, b[i]), kSlots, key);
+  }
+  if (hint & kStashBit) {
     uint32_t cnt = t.stash_count[b[0]];
     if (cnt > kStashSlots) cnt = kStashSlots;
     removed_stash = erase_in(stash_ptr(t, b[0]), static_cast<int>(cnt), key);

@@ -235,16 +235,26 @@ __global__ void k_count_exact(TableView t, unsigned long long* out) {
   }
 }
 
-__global__ void k_rebuild_flags(TableView t) {
-  const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t nw = (static_cast<size_t>(t.nb) + 31) / 32;
-  if (w >= nw) return;
-  uint32_t bits = 0;
-  for (int j = 0; j < 32; ++j) {
-    const size_t b = w * 32 + j;
-    if (b < t.nb && t.stash_count[b] > 0) bits |= 1u << j;
+// Rebuild the hint bits from a table's content (after import): stash bit from
+// each stash count, overflow bit of H0(key) for every entry stored outside
+// its H0.  meta must be zeroed first.  One thread per slot.
+template <class P>
+__global__ void k_rebuild_meta(TableView t) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(t.nb) * kSlots) return;
+  const uint32_t b = static_cast<uint32_t>(i / kSlots);
+  const unsigned long long kv = t.slots[i];
+  if (kv != 0ull) {
+    const uint32_t h0 = bucket_of<P>(0, kv_key(kv), t.nb);
+    if (h0 != b) meta_set(t, h0, kOverflowBit);
   }
-  t.stash_flag[w] = bits;
+  if (i % kSlots == 0 && t.stash_count[b] > 0) meta_set(t, b, kStashBit);
+}
+
+// drop every stash bit, keep the overflow bits (stash drain)
+__global__ void k_clear_stash_bits(TableView t) {
+  const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w < (static_cast<size_t>(t.nb) + 15) / 16) t.meta[w] &= 0x55555555u;
 }
 
 // stash drain (include/hive_stash_table.cuh:33-64; holes are skipped)
@@ -272,43 +282,52 @@ __global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned l
 }
 
 // ---------------------------------------------------------------- probe stats
-// Per key: the distinct candidate buckets an insert's fast path reads, the
-// buckets a lookup reads before its early exit, and whether the lookup also
-// reads the stash.  out[0..2] accumulate the three totals.
+// Per key, following the probe order of fast_insert / lookup_one exactly:
+//   out[0] buckets an insert's fast path reads (H0, plus the other distinct
+//          candidates when H0's overflow bit is set)
+//   out[1] buckets a lookup reads before its early exit
+//   out[2] lookups that also read the stash
+//   out[3] lookups that read the hint word (H0 misses)
+// Block totals go to out[(blockIdx % 64) * 8 + field] (spread counters).
 template <class P>
 __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t* keys, size_t n,
                                                      unsigned long long* out) {
+  __shared__ int blk[4];
+  if (threadIdx.x < 4) blk[threadIdx.x] = 0;
+  __syncthreads();
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int cand = 0, reads = 0, stash = 0;
+  int v[4] = {0, 0, 0, 0};
   if (i < n && keys[i] != 0) {
     const uint32_t key = keys[i];
     uint32_t b[P::NH];
 #pragma unroll
     for (int j = 0; j < P::NH; ++j) b[j] = bucket_of<P>(j, key, t.nb);
-    bool found = false;
+    const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+    v[0] = 1;
+    v[1] = 1;
+    unsigned long long s[kSlots];
+    load_bucket(bucket_ptr(t, b[0]), s);
+    bool found = match_mask(s, key) != 0;
+    if (!found) v[3] = 1;
 #pragma unroll
-    for (int j = 0; j < P::NH; ++j) {
-      bool dup = false;
-#pragma unroll
-      for (int q = 0; q < j; ++q) dup |= b[q] == b[j];
-      if (dup) continue;
-      ++cand;
+    for (int j = 1; j < P::NH; ++j) {
+      if (!(hint & kOverflowBit) || is_dup<P>(b, j)) continue;
+      ++v[0];
       if (found) continue;
-      ++reads;
-      unsigned long long s[kSlots];
+      ++v[1];
       load_bucket(bucket_ptr(t, b[j]), s);
       found = match_mask(s, key) != 0;
     }
-    if (!found && stash_flag_set(t, b[0])) stash = 1;
+    if (!found && (hint & kStashBit)) v[2] = 1;
   }
-  cand = wave_sum(cand);
-  reads = wave_sum(reads);
-  stash = wave_sum(stash);
-  if ((threadIdx.x & 63) == 0) {
-    if (cand) atomicAdd(out, static_cast<unsigned long long>(cand));
-    if (reads) atomicAdd(out + 1, static_cast<unsigned long long>(reads));
-    if (stash) atomicAdd(out + 2, static_cast<unsigned long long>(stash));
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int w = wave_sum(v[f]);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(&blk[f], w);
   }
+  __syncthreads();
+  if (threadIdx.x < 4 && blk[threadIdx.x])
+    atomicAdd(out + (blockIdx.x & 63) * 8 + threadIdx.x, static_cast<unsigned long long>(blk[threadIdx.x]));
 }
 
 // ---------------------------------------------------------------- ceilings

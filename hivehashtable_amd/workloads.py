@@ -19,23 +19,46 @@ def keys_for_lf(num_slots, lf):
     return int(num_slots * lf)
 
 
-def distinct_keys(n, seed, exclude=None, lo=KEY_LO, hi=KEY_HI):
-    """n distinct u32 keys, uniform in [lo, hi], in draw order; none in `exclude`."""
-    rng = np.random.default_rng(seed)
+def _bijection_params(seed):
+    rng = np.random.default_rng([seed, 0x5EED])
+    mult = (rng.integers(1, 2**31, size=3, dtype=np.uint64) * 2 + 1).astype(np.uint64)  # odd multipliers
+    xors = rng.integers(0, 2**32, size=3, dtype=np.uint64)
+    return mult, xors
+
+
+def key_stream(seed, start, count):
+    """f(start), ..., f(start+count-1) for a seeded bijection f of u32, so the
+    outputs are distinct for distinct indices: rounds of xor with a constant,
+    xor-shift right and multiplication by an odd constant (each invertible
+    modulo 2^32)."""
+    mult, xors = _bijection_params(seed)
+    x = np.arange(start, start + count, dtype=np.uint64) & np.uint64(0xFFFFFFFF)
+    m32 = np.uint64(0xFFFFFFFF)
+    for r in range(3):
+        x ^= xors[r]
+        x ^= x >> np.uint64(16)
+        x = (x * mult[r]) & m32
+        x ^= x >> np.uint64(13)
+    return x.astype(np.uint32)
+
+
+def distinct_keys(n, seed, exclude=None, start=0):
+    """n distinct u32 keys in [1, 2^32-2], a seeded pseudo-random permutation
+    prefix (sampling without replacement).  Keys in `exclude` are skipped
+    (small sets only); `start` continues the same stream, so
+    distinct_keys(m, seed, start=n) never repeats distinct_keys(n, seed)."""
     out = np.empty(0, np.uint32)
     excl = None if exclude is None else np.unique(np.asarray(exclude, np.uint32))
+    pos = start
     while out.size < n:
         need = n - out.size
-        draw = rng.integers(lo, hi + 1, size=int(need * 1.05) + 1024, dtype=np.uint64).astype(np.uint32)
-        cand = np.concatenate([out, draw])
-        _, first = np.unique(cand, return_index=True)
-        keep = np.zeros(cand.size, bool)
-        keep[first] = True
+        take = need + (need >> 6) + 64
+        cand = key_stream(seed, pos, take)
+        pos += take
+        keep = (cand >= KEY_LO) & (cand <= KEY_HI)
         if excl is not None and excl.size:
-            pos = np.searchsorted(excl, cand)
-            pos[pos >= excl.size] = 0
-            keep &= excl[pos] != cand
-        out = cand[keep]
+            keep &= ~np.isin(cand, excl, assume_unique=False)
+        out = np.concatenate([out, cand[keep]])
     return np.ascontiguousarray(out[:n])
 
 
@@ -43,16 +66,24 @@ def values(n, seed):
     return np.random.default_rng(seed).integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
 
 
-def lookup_set(inserted, n, exist_ratio, seed):
+def lookup_set(inserted, n, exist_ratio, seed, key_seed=None):
     """n lookup keys: round(n*exist_ratio) drawn from `inserted`, the rest fresh
-    absent keys, shuffled (src/hashTableBenchmark.cc:289-431).  Returns
-    (keys, expect_present bool mask)."""
+    absent keys, shuffled (src/hashTableBenchmark.cc:289-431).  When
+    `inserted` is distinct_keys(len, key_seed), the absent keys continue that
+    stream (absent by construction); otherwise they are filtered against it.
+    Returns (keys, expect_present bool mask)."""
     rng = np.random.default_rng(seed)
     n_hit = int(round(n * exist_ratio))
     hit = inserted[rng.integers(0, inserted.size, size=n_hit)] if n_hit else np.empty(0, np.uint32)
-    miss = distinct_keys(n - n_hit, seed + 7919, exclude=inserted) if n - n_hit else np.empty(0, np.uint32)
+    n_miss = n - n_hit
+    if not n_miss:
+        miss = np.empty(0, np.uint32)
+    elif key_seed is not None:
+        miss = distinct_keys(n_miss, key_seed, start=inserted.size + (inserted.size >> 6) + 64)
+    else:
+        miss = distinct_keys(n_miss, seed + 7919, exclude=inserted)
     keys = np.concatenate([hit, miss]).astype(np.uint32)
-    present = np.concatenate([np.ones(n_hit, bool), np.zeros(n - n_hit, bool)])
+    present = np.concatenate([np.ones(n_hit, bool), np.zeros(n_miss, bool)])
     perm = rng.permutation(n)
     return np.ascontiguousarray(keys[perm]), present[perm]
 
@@ -63,7 +94,7 @@ def cfg1(seed=42):
     n = keys_for_lf(slots, 0.8)
     keys = distinct_keys(n, seed)
     vals = values(n, seed + 1)
-    lk, present = lookup_set(keys, n, 0.5, seed + 2)
+    lk, present = lookup_set(keys, n, 0.5, seed + 2, key_seed=seed)
     return slots, keys, vals, lk, present
 
 
