@@ -196,6 +196,10 @@ def main():
     verified = bool(np.array_equal(found, expect_found) and np.array_equal(lv[found], expect_val[found]))
     log(f"verified={verified} placed={placed.mean():.6f} found={found.mean():.4f}")
 
+    tstats = t.stats()
+    slow_frac = tstats["slow_path_ops"] / n_ins
+    log(f"insert slow path: {tstats['slow_path_ops']} ops ({slow_frac:.4f}), stash entries {tstats['stash_live']}")
+
     # algorithmic bytes (probe statistics of this exact table and key sets)
     cand, _, _ = t.probe_stats(d_keys)
     _, lreads, lstash = t.probe_stats(d_lk)
@@ -265,6 +269,8 @@ def main():
         "lookup_mops": round(n_look / (look_ms / 1e3) / 1e6, 3),
         "kernel_ms": {"clear": round(clear_ms, 4), "insert": round(ins_ms, 4), "lookup": round(look_ms, 4)},
         "verified": verified,
+        "insert_slow_path_frac": round(slow_frac, 5),
+        "stash_entries": tstats["stash_live"],
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,

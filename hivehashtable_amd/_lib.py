@@ -41,6 +41,7 @@ class Stats(C.Structure):
         ("load_factor", C.c_double),
         ("device_bytes", C.c_uint64),
         ("policy", C.c_int),
+        ("slow_path_ops", C.c_uint64),
     ]
 
 

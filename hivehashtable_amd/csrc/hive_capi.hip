@@ -50,7 +50,7 @@ struct hive_table {
 namespace {
 
 size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
-size_t counter_words() { return static_cast<size_t>(2) * kCounterLanes * kCounterStride; }
+size_t counter_words() { return static_cast<size_t>(3) * kCounterLanes * kCounterStride; }
 
 void free_view(TableView& v) {
   if (v.slots) (void)hipFree(v.slots);
@@ -165,17 +165,19 @@ hive_status_t set_device(const hive_table* t) {
   return HIVE_OK;
 }
 
-hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_t* st) {
+hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_t* st, uint64_t* slow = nullptr) {
   std::vector<unsigned long long> h(counter_words());
   HIP_TRY(hipMemcpyAsync(h.data(), t->v.counters, counter_words() * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  unsigned long long a = 0, b = 0;
+  unsigned long long a = 0, b = 0, c = 0;
   for (int l = 0; l < kCounterLanes; ++l) {
     a += h[(0 * kCounterLanes + l) * kCounterStride];
     b += h[(1 * kCounterLanes + l) * kCounterStride];
+    c += h[(2 * kCounterLanes + l) * kCounterStride];
   }
   *tab = a;
   *st = b;
+  if (slow) *slow = c;
   return HIVE_OK;
 }
 
@@ -437,8 +439,9 @@ hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   if (!out) return fail(HIVE_EINVAL, "null out");
-  uint64_t tab = 0, st = 0;
-  if ((r = read_counters(t, static_cast<hipStream_t>(stream), &tab, &st)) != HIVE_OK) return r;
+  uint64_t tab = 0, st = 0, slow = 0;
+  if ((r = read_counters(t, static_cast<hipStream_t>(stream), &tab, &st, &slow)) != HIVE_OK) return r;
+  out->slow_path_ops = slow;
   out->num_buckets = t->v.nb;
   out->slots_per_bucket = kSlots;
   out->capacity = static_cast<uint64_t>(t->v.nb) * kSlots;

@@ -45,7 +45,7 @@ struct TableView {
   uint32_t* stash_count;            // nb
   uint32_t* meta;                   // (nb + 15) / 16 words, 2 bits per bucket: kOverflowBit, kStashBit
   uint32_t* locks;                  // nb
-  unsigned long long* counters;     // [2][kCounterLanes][kCounterStride]: table, stash
+  unsigned long long* counters;     // [3][kCounterLanes][kCounterStride]: table, stash, slow-path ops
   uint32_t nb;
 };
 
@@ -215,8 +215,8 @@ __device__ __forceinline__ bool try_lock(uint32_t* l) {
                                               __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void unlock(uint32_t* l) {
-  wait_vmem();
-  __hip_atomic_exchange(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  wait_vmem();  // the critical section's atomics and coherent stores are performed
+  __hip_atomic_store(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // plain 64-byte bucket read (four 16-byte loads)
@@ -230,9 +230,18 @@ __device__ __forceinline__ void load_bucket(const unsigned long long* p, unsigne
   }
 }
 
+// coherent 64-byte bucket read: four 16-byte volatile loads, which compile to
+// system-coherent loads (sc0 sc1) that bypass stale L2 lines; each aligned
+// 8-byte slot inside a 16-byte access is read whole.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+  const volatile u64x2* q = reinterpret_cast<const volatile u64x2*>(p);
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) s[j] = ld_sync(p + j);
+  for (int j = 0; j < kSlots / 2; ++j) {
+    const u64x2 v = q[j];
+    s[2 * j] = v.x;
+    s[2 * j + 1] = v.y;
+  }
 }
 
 __device__ __forceinline__ uint32_t match_mask(const unsigned long long (&s)[kSlots], uint32_t key) {
@@ -331,12 +340,16 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
   return R_PENDING;
 }
 
-// Stage A on one bucket with coherent reads: 1 updated, 0 absent, -1 raced.
+// Stage A on one bucket with coherent reads (all eight slots requested at
+// once, then scanned): 1 updated, 0 absent, -1 raced.
 __device__ __forceinline__ int locked_update(unsigned long long* bk, int nslots, uint32_t key,
                                              unsigned long long kv) {
-  for (int j = 0; j < nslots; ++j) {
-    unsigned long long cur = ld_sync(bk + j);
-    if (cur == 0ull || kv_key(cur) != key) continue;
+  unsigned long long s[kSlots];
+  load_bucket_sync(bk, s);
+  uint32_t m = match_mask(s, key) & ((1u << nslots) - 1u);
+  if (m) {
+    const int j = __builtin_ctz(m);
+    unsigned long long cur = s[j];
     for (int spin = 0; spin < 64; ++spin) {
       const unsigned long long seen = cas64(bk + j, cur, kv);
       if (seen == cur) return 1;
@@ -388,23 +401,43 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
   unsigned long long* sb = stash_ptr(t, c);
   for (int attempt = 0; attempt < 256; ++attempt) {
     bool retry = false;
-#pragma unroll
-    for (int i = 0; i < P::NH; ++i) {
-      if (retry) break;
-      const int r = locked_update(bucket_ptr(t, bucket_of<P>(i, key, t.nb)), kSlots, key, kv);
-      if (r > 0) return R_UPDATED;
-      if (r < 0) retry = true;
-    }
-    if (retry) continue;
-    uint32_t cnt = ld_sync32(t.stash_count + c);
-    if (cnt > kStashSlots) cnt = kStashSlots;
-    {
-      const int r = locked_update(sb, static_cast<int>(cnt), key, kv);
-      if (r > 0) return R_UPDATED;
-      if (r < 0) continue;
-    }
+    // one coherent round: the canonical bucket and its hint bits.  Only
+    // holders of this lock move keys whose H0 is c out of c, or push to
+    // stash[c], so the hint read here is exact for this key.
     unsigned long long cs[kSlots];
     load_bucket_sync(cb, cs);
+    const uint32_t hint = meta_bits(ld_sync32(t.meta + (c >> 4)), c);
+    {
+      const uint32_t m = match_mask(cs, key);
+      if (m) {
+        const int j = __builtin_ctz(m);
+        if (cas64(cb + j, cs[j], kv) == cs[j]) return R_UPDATED;
+        continue;
+      }
+    }
+    if (hint & kOverflowBit) {
+      uint32_t b[P::NH];
+#pragma unroll
+      for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+#pragma unroll
+      for (int i = 1; i < P::NH; ++i) {
+        if (retry || is_dup<P>(b, i)) continue;
+        const int r = locked_update(bucket_ptr(t, b[i]), kSlots, key, kv);
+        if (r > 0) return R_UPDATED;
+        if (r < 0) retry = true;
+      }
+    }
+    if (retry) continue;
+    uint32_t cnt = 0;
+    if (hint & kStashBit) {
+      cnt = ld_sync32(t.stash_count + c);
+      if (cnt > kStashSlots) cnt = kStashSlots;
+      if (cnt) {
+        const int r = locked_update(sb, static_cast<int>(cnt), key, kv);
+        if (r > 0) return R_UPDATED;
+        if (r < 0) continue;
+      }
+    }
     uint32_t e = empty_mask(cs);
     while (e) {
       const int j = __builtin_ctz(e);
@@ -423,12 +456,16 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       const uint32_t a = alt_bucket<P>(vkey, c, t.nb);
       if (a == c) continue;
       unsigned long long* ab = bucket_ptr(t, a);
+      // slots of the alternate bucket are only ever claimed by CAS 0 -> kv, so
+      // a stale plain view can only overstate its empty slots; the CAS decides
       unsigned long long as[kSlots];
-      load_bucket_sync(ab, as);
+      load_bucket(ab, as);
       uint32_t ae = empty_mask(as);
       if (!ae) continue;
       const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
-      if (a != vh0) meta_set(t, vh0, kOverflowBit);  // before the victim can leave its H0
+      // before the victim can leave its H0 (already set when vh0 == c and the
+      // coherent hint read above saw it)
+      if (a != vh0 && !(vh0 == c && (hint & kOverflowBit))) meta_set(t, vh0, kOverflowBit);
       int d = -1;
       while (ae) {
         const int j = __builtin_ctz(ae);

@@ -69,21 +69,21 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// Called by every thread of the block (contains __syncthreads).
-__device__ __forceinline__ void block_count(const TableView& t, int dtab, int dstash, int* blk) {
+// Per-wavefront counter update: every lane of the wave must call it (no
+// barrier, so waves whose work is done can retire independently).  Group 0
+// counts bucket entries, 1 stash entries, 2 ops that took the insert slow path.
+__device__ __forceinline__ void wave_count(const TableView& t, int dtab, int dstash, int dslow = 0) {
   const int wt = wave_sum(dtab);
   const int ws = wave_sum(dstash);
+  const int wl = wave_sum(dslow);
   if ((threadIdx.x & 63) == 0) {
-    if (wt) atomicAdd(&blk[0], wt);
-    if (ws) atomicAdd(&blk[1], ws);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int lane = blockIdx.x & (kCounterLanes - 1);
-    if (blk[0]) atomicAdd(t.counters + (0 * kCounterLanes + lane) * kCounterStride,
-                          static_cast<unsigned long long>(static_cast<long long>(blk[0])));
-    if (blk[1]) atomicAdd(t.counters + (1 * kCounterLanes + lane) * kCounterStride,
-                          static_cast<unsigned long long>(static_cast<long long>(blk[1])));
+    const int lane = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kCounterLanes - 1);
+    if (wt) atomicAdd(t.counters + (0 * kCounterLanes + lane) * kCounterStride,
+                      static_cast<unsigned long long>(static_cast<long long>(wt)));
+    if (ws) atomicAdd(t.counters + (1 * kCounterLanes + lane) * kCounterStride,
+                      static_cast<unsigned long long>(static_cast<long long>(ws)));
+    if (wl) atomicAdd(t.counters + (2 * kCounterLanes + lane) * kCounterStride,
+                      static_cast<unsigned long long>(static_cast<long long>(wl)));
   }
 }
 
@@ -96,11 +96,7 @@ template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase) {
   __shared__ uint32_t queue[kMaxBlock];
   __shared__ uint32_t qn;
-  __shared__ int blk[2];
-  if (threadIdx.x == 0) {
-    qn = 0;
-    blk[0] = blk[1] = 0;
-  }
+  if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
   const size_t i = base + threadIdx.x;
@@ -117,6 +113,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   }
   __syncthreads();
   const uint32_t pending = qn;
+  int dslow = 0;
   if (threadIdx.x < pending) {  // LDS-staged slow path on packed lanes
     const size_t idx = base + queue[threadIdx.x];
     const uint32_t key = src.key(idx);
@@ -124,8 +121,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
     out.put(idx, r, 0);
     dtab += (r == R_INSERTED);
     dstash += (r == R_STASHED);
+    dslow = 1;
   }
-  block_count(t, dtab, dstash, blk);
+  wave_count(t, dtab, dstash, dslow);
 }
 
 // ---------------------------------------------------------------- lookup
@@ -148,16 +146,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
 // ---------------------------------------------------------------- erase
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_t n, Out out, int phase) {
-  __shared__ int blk[2];
-  if (threadIdx.x == 0) blk[0] = blk[1] = 0;
-  __syncthreads();
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int rt = 0, rs = 0;
   if (i < n && (phase < 0 || src.type(i, OP_DELETE) == OP_DELETE)) {
     const uint8_t r = erase_one<P>(t, src.key(i), rt, rs);
     out.put(i, r, 0);
   }
-  block_count(t, -rt, -rs, blk);
+  wave_count(t, -rt, -rs);
 }
 
 // ---------------------------------------------------------------- rehash
@@ -167,11 +162,7 @@ template <class P>
 __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView src, unsigned long long* failed) {
   __shared__ unsigned long long queue[kMaxBlock];
   __shared__ uint32_t qn;
-  __shared__ int blk[2];
-  if (threadIdx.x == 0) {
-    qn = 0;
-    blk[0] = blk[1] = 0;
-  }
+  if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t ntab = static_cast<size_t>(src.nb) * kSlots;
@@ -198,7 +189,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
     dstash += (r == R_STASHED);
     if (r == R_FAILED) atomicAdd(failed, 1ull);
   }
-  block_count(dst, dtab, dstash, blk);
+  wave_count(dst, dtab, dstash);
 }
 
 // ---------------------------------------------------------------- utility

@@ -12,7 +12,9 @@
  *                   (8 slots x 8 B = one 64-byte bucket, densely packed)
  *   stash[nb * 8]   u64, one 8-slot overflow bucket per canonical bucket
  *   stash_count[nb] u32 monotonic push pointer of each stash bucket
- *   stash_flag[nb/32] u32 bitmap, bit set <=> stash_count > 0
+ *   meta[nb/16]     u32, 2 hint bits per bucket: overflow (a key whose H0 is
+ *                   this bucket may sit in another candidate bucket) and
+ *                   stash (stash_count > 0)
  *   locks[nb]       u32 per-canonical-bucket lock of the insert slow path
  * Key 0 is the empty sentinel and is rejected by every operation
  * (include/utils.h:40-47 of the reference).
@@ -81,6 +83,7 @@ typedef struct {
   double load_factor;        /* table_live / capacity, as count_occupancy_kernel (include/hive_kernels.cuh:303-325) */
   uint64_t device_bytes;     /* device memory held by the handle */
   int policy;
+  uint64_t slow_path_ops;    /* inserts that took the locked slow path since the last clear/import */
 } hive_stats_t;
 
 /* Launch shape of the plain bulk kernels.  0 = library default / autotuned. */
