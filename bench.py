@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
+    p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
+    p.add_argument("--mixed-batches", type=int, default=1000)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (reference point for the HBM byte counters)")
     return p.parse_args()
@@ -82,6 +84,79 @@ def traffic_from_profiles(lf):
         return d.get(f"lf{int(round(lf * 100)):03d}")
     except Exception:
         return None
+
+
+def clean_ops(types, keys, off):
+    """Ops with exactly one admissible outcome: the key occurs once in its batch
+    and had no concurrent inserts in an earlier batch."""
+    nb = len(off) - 1
+    batch = np.repeat(np.arange(nb), np.diff(off).astype(np.int64))
+    comb = (batch.astype(np.uint64) << np.uint64(32)) | keys.astype(np.uint64)
+    _, inv, cnt = np.unique(comb, return_inverse=True, return_counts=True)
+    dup = cnt[inv] > 1
+    dk, db = keys[dup], batch[dup]
+    order = np.lexsort((db, dk))
+    dk, db = dk[order], db[order]
+    first = np.ones(dk.size, bool)
+    first[1:] = dk[1:] != dk[:-1]
+    ukeys, ufirst = dk[first], db[first]
+    pos = np.searchsorted(ukeys, keys)
+    pos[pos >= max(ukeys.size, 1)] = 0
+    fd = np.where((ukeys.size > 0) & (ukeys[pos] == keys) if ukeys.size else np.zeros(keys.size, bool),
+                  ufirst[pos] if ukeys.size else 0, nb)
+    return (~dup) & (batch < fd)
+
+
+def mixed_leg(a, H, W, torch, dev):
+    """2^26 slots preloaded to LF 0.5; a stream of uneven mixed batches
+    (log-uniform 2^10..2^20 ops, 50/40/10 insert/lookup/delete over a 2^27 key
+    space) run as per-batch launches, one captured hipGraph, and the
+    persistent work-queue kernel (shapes from the autotuner)."""
+    slots = 1 << a.log2_slots
+    t0 = time.time()
+    pre = W.distinct_keys(slots // 2, a.seed + 3000, bits=27)  # preload from the same 2^27 key space
+    sizes = W.log_uniform_sizes(a.mixed_batches, 1 << 10, 1 << 20, a.seed + 3001)
+    types, keys, vals, off = W.mixed_stream(pre, 1 << 27, sizes, seed=a.seed + 3002)
+    clean = clean_ops(types, keys, off)
+    n_ops = int(off[-1])
+    log(f"mixed stream: {len(sizes)} batches, {n_ops} ops, {clean.mean():.4f} single-outcome, {time.time() - t0:.1f}s")
+    d_pre = torch.from_numpy(pre.view(np.int32)).to(dev)
+    d_ty = torch.from_numpy(types).to(dev)
+    d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
+    d_v = torch.from_numpy(vals.view(np.int32)).to(dev)
+    d_clean = torch.from_numpy(clean).to(dev)
+    t = H.HiveTable(slots, a.policy)
+    t.insert(d_pre)
+    ns = min(100, len(sizes))
+    tune = t.autotune_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1])
+    log(f"autotune: {tune}")
+    out = {"batches": len(sizes), "ops": n_ops, "preload_keys": int(pre.size), "key_space": 1 << 27,
+           "batch_ops_min_max": [int(sizes.min()), int(sizes.max())], "mix_insert_lookup_delete": [0.5, 0.4, 0.1],
+           "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}, "modes": {}}
+    ref = None
+    agree = True
+    names = {0: "launches", 1: "graph", 2: "persistent"}
+    for mode in (0, 1, 2):
+        t.clear()
+        t.insert(d_pre)
+        torch.cuda.synchronize()
+        gv, gs, done, setup = t.mixed_stream(d_ty, d_k, d_v, off, mode)
+        lat = np.diff(np.concatenate([[0.0], done])) / 1e3  # per-batch service time, us
+        total_ms = done[-1] / 1e6
+        st = torch.where(gs == 3, torch.ones_like(gs), gs)
+        if ref is None:
+            ref = (st, gv)
+        else:
+            same = bool(torch.equal(st[d_clean], ref[0][d_clean]) and torch.equal(gv[d_clean], ref[1][d_clean]))
+            agree &= same
+        out["modes"][names[mode]] = {
+            "total_ms": round(total_ms, 3), "mops": round(n_ops / total_ms / 1e3, 2),
+            "p50_batch_us": round(float(np.percentile(lat, 50)), 2), "p99_batch_us": round(float(np.percentile(lat, 99)), 2),
+            "setup_ms": round(setup, 3)}
+        log(f"mixed {names[mode]}: {out['modes'][names[mode]]}")
+    out["modes_agree"] = agree
+    t.close()
+    return out
 
 
 def cpu_baseline(keys, vals, lk, threads, gpu_found, gpu_vals, policy, nb):
@@ -239,6 +314,10 @@ def main():
     look_read_rate = lreads / (look_ms / 1e3)
     ins_read_rate = cand / (ins_ms / 1e3)
 
+    mixed = None
+    if not a.no_mixed:
+        mixed = mixed_leg(a, H, W, torch, dev)
+
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -291,6 +370,7 @@ def main():
             "insert_frac_of_gather_ceiling": round(ins_read_rate / gather_rate, 4),
             "footprint_bytes": nb * 64,
         },
+        "mixed_stream": mixed,
         "cpu_baseline": cpu,
         "git": git_sha(),
     }

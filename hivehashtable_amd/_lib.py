@@ -15,6 +15,7 @@ HEADER = os.path.join(ROOT, "include", "hive.h")
 HIVE_OK, HIVE_EINVAL, HIVE_ENOMEM, HIVE_EDEVICE, HIVE_ECAPACITY, HIVE_ERANGE = range(6)
 R_FAILED, R_INSERTED, R_UPDATED, R_STASHED, R_FOUND, R_NOT_FOUND, R_ERASED, R_INVALID_KEY, R_NOOP = range(9)
 OP_INSERT, OP_LOOKUP, OP_DELETE, OP_ATOMIC_INC, OP_NONE = range(5)
+STREAM_LAUNCHES, STREAM_GRAPH, STREAM_PERSISTENT = range(3)
 POLICIES = {
     "Default2Hash": 0,
     "MurmurCityHash": 1,
@@ -47,6 +48,12 @@ class Stats(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("block_threads", C.c_uint32), ("tile_ops", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("best_mode", C.c_int), ("launches_ms", C.c_double), ("graph_ms", C.c_double),
+                ("persistent_ms", C.c_double), ("candidates", C.c_uint32)]
 
 
 class Operation(C.Structure):
@@ -96,6 +103,9 @@ def lib():
         "hive_get_launch": [p, i32, C.POINTER(Launch)],
         "hive_hash_table_kernel_dispatch": [p, sz, sz, sz, sz, sz, C.POINTER(C.c_double), p, i32, i32, C.c_char_p],
         "hive_hash_table_kernel_dispatch_ycsb": [p, p, sz, sz, sz, sz, sz, C.POINTER(C.c_double), p, i32, i32],
+        "hive_mixed_stream": [p, p, p, p, p, sz, p, p, i32, p, C.POINTER(C.c_double), p],
+        "hive_set_stream_shape": [p, u32, u32],
+        "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
         "hive_ceiling_gather": [p, u64, u64, u32, p, p],
         "hive_ceiling_cas": [p, u64, u64, u32, p],
     }

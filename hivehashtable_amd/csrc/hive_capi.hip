@@ -11,159 +11,15 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hive.h"
-#include "hive_kernels.hpp"
+#include "hive_internal.hpp"
 
 using namespace hive;
 
-namespace {
-
+namespace hive {
 thread_local std::string g_last_error;
-
-hive_status_t fail(hive_status_t s, const std::string& what) {
-  g_last_error = what;
-  return s;
 }
-
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(HIVE_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-  } while (0)
-
-#define HIP_CHECK_LAUNCH() HIP_TRY(hipGetLastError())
-
-constexpr uint32_t kDefaultBlock = 256;
-
-}  // namespace
-
-struct hive_table {
-  int device = 0;
-  int policy = 0;
-  TableView v{};
-  size_t bytes = 0;
-  hive_launch_t launch[3] = {{kDefaultBlock, 1}, {kDefaultBlock, 1}, {kDefaultBlock, 1}};
-  unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
-};
 
 namespace {
-
-size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
-size_t counter_words() { return static_cast<size_t>(3) * kCounterLanes * kCounterStride; }
-
-void free_view(TableView& v) {
-  if (v.slots) (void)hipFree(v.slots);
-  if (v.stash) (void)hipFree(v.stash);
-  if (v.stash_count) (void)hipFree(v.stash_count);
-  if (v.meta) (void)hipFree(v.meta);
-  if (v.locks) (void)hipFree(v.locks);
-  if (v.counters) (void)hipFree(v.counters);
-  v = TableView{};
-}
-
-size_t view_bytes(uint32_t nb) {
-  return static_cast<size_t>(nb) * kSlots * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
-         static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_words() * 8;
-}
-
-hive_status_t alloc_view(TableView& v, uint32_t nb) {
-  v = TableView{};
-  v.nb = nb;
-  hipError_t e = hipSuccess;
-  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kSlots * 8)) != hipSuccess ||
-      (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
-      (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.counters, counter_words() * 8)) != hipSuccess) {
-    free_view(v);
-    (void)hipGetLastError();
-    return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
-                std::string("hipMalloc: ") + hipGetErrorString(e));
-  }
-  return HIVE_OK;
-}
-
-unsigned fill_grid(size_t n) {
-  size_t g = (n + 255) / 256;
-  if (g > 8192) g = 8192;
-  return static_cast<unsigned>(g ? g : 1);
-}
-
-// zero buckets, stash pointers, hint bits, locks and counters (stash slots at or
-// above a bucket's count are never read, so the stash body is left as is).
-hive_status_t clear_view(const TableView& v, hipStream_t s) {
-  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
-  k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
-  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
-  k_fill_u32<<<fill_grid(meta_words(v.nb)), 256, 0, s>>>(v.meta, meta_words(v.nb), 0u);
-  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.locks, v.nb, 0u);
-  k_fill_u64<<<fill_grid(counter_words()), 256, 0, s>>>(v.counters, counter_words(), 0ull);
-  HIP_CHECK_LAUNCH();
-  return HIVE_OK;
-}
-
-#define HIVE_POLICY_SWITCH(policy, BODY)                   \
-  switch (policy) {                                        \
-    case HIVE_POLICY_DEFAULT2HASH: { using P = PolicyDefault2; BODY; } break;       \
-    case HIVE_POLICY_MURMURCITY: { using P = PolicyMurmurCity; BODY; } break;       \
-    case HIVE_POLICY_TRIPLE: { using P = PolicyTriple; BODY; } break;               \
-    case HIVE_POLICY_MURMURCITYBIT: { using P = PolicyMurmurCityBit; BODY; } break; \
-    case HIVE_POLICY_LOOKUP2: { using P = PolicyLookup2; BODY; } break;             \
-    case HIVE_POLICY_LOOKUP3: { using P = PolicyLookup3; BODY; } break;             \
-    default: return fail(HIVE_EINVAL, "unknown policy");   \
-  }
-
-unsigned grid_for(size_t n, uint32_t block) { return static_cast<unsigned>((n + block - 1) / block); }
-
-bool valid_block(uint32_t b) { return b >= 64 && b <= static_cast<uint32_t>(kMaxBlock) && b % 64 == 0; }
-
-template <class Src, class Out>
-hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
-  if (n == 0) return HIVE_OK;
-  const uint32_t blk = t->launch[OP_INSERT].block_threads;
-  HIVE_POLICY_SWITCH(t->policy, (k_insert<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
-  HIP_CHECK_LAUNCH();
-  return HIVE_OK;
-}
-
-template <class Src, class Out>
-hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
-  if (n == 0) return HIVE_OK;
-  const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
-  HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
-  HIP_CHECK_LAUNCH();
-  return HIVE_OK;
-}
-
-template <class Src, class Out>
-hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
-  if (n == 0) return HIVE_OK;
-  const uint32_t blk = t->launch[OP_DELETE].block_threads;
-  HIVE_POLICY_SWITCH(t->policy, (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
-  HIP_CHECK_LAUNCH();
-  return HIVE_OK;
-}
-
-// a mixed batch = three stream-ordered phases: lookups, deletes, inserts
-template <class Src, class Out>
-hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_t s) {
-  hive_status_t r;
-  if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;
-  if ((r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
-  return launch_insert(t, src, n, out, OP_INSERT, s);
-}
-
-hive_status_t check_table(const hive_table* t) {
-  if (!t) return fail(HIVE_EINVAL, "null table");
-  return HIVE_OK;
-}
-
-hive_status_t set_device(const hive_table* t) {
-  HIP_TRY(hipSetDevice(t->device));
-  return HIVE_OK;
-}
 
 hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_t* st, uint64_t* slow = nullptr) {
   std::vector<unsigned long long> h(counter_words());

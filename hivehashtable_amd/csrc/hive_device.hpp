@@ -7,11 +7,13 @@
 //    wrote them during the same kernel; kernel boundaries make all writes
 //    visible.  Every fast path therefore reads with plain loads and confirms
 //    with CAS (performed at the coherence point, returns the true old value).
-//  * slow paths read with agent-scope relaxed atomic loads (global_load sc1),
-//    which bypass stale L2 lines, and write only with atomics.
+//  * slow paths read buckets with 16-byte volatile loads (sc0 sc1) and words
+//    with agent-scope relaxed atomic loads (sc1), which bypass stale L2 lines,
+//    and write only with atomics or coherent (sc1) stores.
 //  * no acquire/release fences at agent scope: they emit buffer_inv/buffer_wbl2
 //    over the whole L2.  The lock release waits for the critical section's
-//    memory operations with s_waitcnt vmcnt(0) and releases with an atomic swap.
+//    memory operations with s_waitcnt vmcnt(0) and releases with a coherent
+//    store.
 #pragma once
 
 #include <hip/hip_runtime.h>

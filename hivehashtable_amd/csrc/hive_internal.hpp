@@ -1,0 +1,165 @@
+// hive_internal.hpp -- host-side internals shared by the translation units of
+// libhive.so: the table handle, error plumbing and the launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hive.h"
+#include "hive_kernels.hpp"
+
+namespace hive {
+
+extern thread_local std::string g_last_error;
+
+inline hive_status_t fail(hive_status_t s, const std::string& what) {
+  g_last_error = what;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::hive::fail(HIVE_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HIP_CHECK_LAUNCH() HIP_TRY(hipGetLastError())
+
+constexpr uint32_t kDefaultBlock = 256;
+
+}  // namespace hive
+
+struct hive_table {
+  int device = 0;
+  int policy = 0;
+  hive::TableView v{};
+  size_t bytes = 0;
+  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1}, {hive::kDefaultBlock, 1}, {hive::kDefaultBlock, 1}};
+  unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
+  uint32_t stream_tile = 0;              // persistent kernel: ops per tile (0 = sized per batch)
+  uint32_t stream_blocks_per_cu = 0;     // persistent kernel: resident blocks per CU (0 = default)
+};
+
+namespace hive {
+
+inline size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
+inline size_t counter_words() { return static_cast<size_t>(3) * kCounterLanes * kCounterStride; }
+inline unsigned grid_for(size_t n, uint32_t block) { return static_cast<unsigned>((n + block - 1) / block); }
+inline bool valid_block(uint32_t b) { return b >= 64 && b <= static_cast<uint32_t>(kMaxBlock) && b % 64 == 0; }
+inline unsigned fill_grid(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  return static_cast<unsigned>(g ? g : 1);
+}
+
+#define HIVE_POLICY_SWITCH(policy, BODY)                                                  \
+  switch (policy) {                                                                       \
+    case HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::PolicyDefault2; BODY; } break;       \
+    case HIVE_POLICY_MURMURCITY: { using P = ::hive::PolicyMurmurCity; BODY; } break;       \
+    case HIVE_POLICY_TRIPLE: { using P = ::hive::PolicyTriple; BODY; } break;               \
+    case HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::PolicyMurmurCityBit; BODY; } break; \
+    case HIVE_POLICY_LOOKUP2: { using P = ::hive::PolicyLookup2; BODY; } break;             \
+    case HIVE_POLICY_LOOKUP3: { using P = ::hive::PolicyLookup3; BODY; } break;             \
+    default: return ::hive::fail(HIVE_EINVAL, "unknown policy");                          \
+  }
+
+template <class Src, class Out>
+hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s,
+                            BatchClock clk = BatchClock{nullptr, nullptr}) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_INSERT].block_threads;
+  HIVE_POLICY_SWITCH(t->policy,
+                     (k_insert<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase, clk)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+template <class Src, class Out>
+hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
+  HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+template <class Src, class Out>
+hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
+  if (n == 0) return HIVE_OK;
+  const uint32_t blk = t->launch[OP_DELETE].block_threads;
+  HIVE_POLICY_SWITCH(t->policy, (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+// a mixed batch = three stream-ordered phases: lookups, deletes, inserts
+template <class Src, class Out>
+hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_t s,
+                           BatchClock clk = BatchClock{nullptr, nullptr}) {
+  hive_status_t r;
+  if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;
+  if ((r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
+  return launch_insert(t, src, n, out, OP_INSERT, s, clk);
+}
+
+inline void free_view(TableView& v) {
+  if (v.slots) (void)hipFree(v.slots);
+  if (v.stash) (void)hipFree(v.stash);
+  if (v.stash_count) (void)hipFree(v.stash_count);
+  if (v.meta) (void)hipFree(v.meta);
+  if (v.locks) (void)hipFree(v.locks);
+  if (v.counters) (void)hipFree(v.counters);
+  v = TableView{};
+}
+
+inline size_t view_bytes(uint32_t nb) {
+  return static_cast<size_t>(nb) * kSlots * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
+         static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_words() * 8;
+}
+
+inline hive_status_t alloc_view(TableView& v, uint32_t nb) {
+  v = TableView{};
+  v.nb = nb;
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kSlots * 8)) != hipSuccess ||
+      (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
+      (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
+      (e = hipMalloc(&v.counters, counter_words() * 8)) != hipSuccess) {
+    free_view(v);
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return HIVE_OK;
+}
+
+// zero buckets, stash pointers, hint bits, locks and counters (stash slots at or
+// above a bucket's count are never read, so the stash body is left as is).
+inline hive_status_t clear_view(const TableView& v, hipStream_t s) {
+  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
+  k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
+  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
+  k_fill_u32<<<fill_grid(meta_words(v.nb)), 256, 0, s>>>(v.meta, meta_words(v.nb), 0u);
+  k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.locks, v.nb, 0u);
+  k_fill_u64<<<fill_grid(counter_words()), 256, 0, s>>>(v.counters, counter_words(), 0ull);
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+
+inline hive_status_t check_table(const hive_table* t) {
+  if (!t) return fail(HIVE_EINVAL, "null table");
+  return HIVE_OK;
+}
+
+inline hive_status_t set_device(const hive_table* t) {
+  HIP_TRY(hipSetDevice(t->device));
+  return HIVE_OK;
+}
+
+}  // namespace hive

@@ -89,11 +89,29 @@ __device__ __forceinline__ void wave_count(const TableView& t, int dtab, int dst
 
 constexpr int kMaxBlock = 1024;
 
+// Optional completion clock of a launch: the last block to finish stamps the
+// device wall clock (100 MHz on gfx950) into *ts.  Used to time batch
+// sequences without extra launches.
+struct BatchClock {
+  unsigned int* ctr;
+  unsigned long long* ts;
+  __device__ __forceinline__ void finish() const {
+    if (ctr == nullptr) return;
+    wait_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(ctr, 1u) == gridDim.x - 1)
+      __hip_atomic_store(ts, static_cast<unsigned long long>(wall_clock64()), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+};
+
+
 // ---------------------------------------------------------------- insert
 // phase < 0: every op is an insert.  phase == OP_INSERT: only ops whose type
 // is INSERT (the insert phase of a mixed batch).
 template <class P, class Src, class Out>
-__global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase) {
+__global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase,
+                                                      BatchClock clk = BatchClock{nullptr, nullptr}) {
   __shared__ uint32_t queue[kMaxBlock];
   __shared__ uint32_t qn;
   if (threadIdx.x == 0) qn = 0;
@@ -124,6 +142,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
     dslow = 1;
   }
   wave_count(t, dtab, dstash, dslow);
+  clk.finish();
 }
 
 // ---------------------------------------------------------------- lookup
@@ -193,20 +212,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
 }
 
 // ---------------------------------------------------------------- utility
-__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+static __global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x)
     p[i] = v;
 }
 
-__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+static __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x)
     p[i] = v;
 }
 
 // exact occupancy (include/hive_kernels.cuh:303-325, counting kv != 0)
-__global__ void k_count_exact(TableView t, unsigned long long* out) {
+static __global__ void k_count_exact(TableView t, unsigned long long* out) {
   const size_t ntab = static_cast<size_t>(t.nb) * kSlots;
   unsigned long long ct = 0, cs = 0;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < ntab;
@@ -243,13 +262,13 @@ __global__ void k_rebuild_meta(TableView t) {
 }
 
 // drop every stash bit, keep the overflow bits (stash drain)
-__global__ void k_clear_stash_bits(TableView t) {
+static __global__ void k_clear_stash_bits(TableView t) {
   const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w < (static_cast<size_t>(t.nb) + 15) / 16) t.meta[w] &= 0x55555555u;
 }
 
 // stash drain (include/hive_stash_table.cuh:33-64; holes are skipped)
-__global__ void k_stash_count_live(TableView t, unsigned long long* total) {
+static __global__ void k_stash_count_live(TableView t, unsigned long long* total) {
   const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int c = 0;
   if (b < t.nb) {
@@ -261,7 +280,7 @@ __global__ void k_stash_count_live(TableView t, unsigned long long* total) {
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, static_cast<unsigned long long>(c));
 }
 
-__global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned long long* pos) {
+static __global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned long long* pos) {
   const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= t.nb) return;
   uint32_t cnt = t.stash_count[b];
@@ -327,7 +346,7 @@ __device__ __forceinline__ uint32_t ceiling_index(size_t i, uint32_t seed) {
 }
 
 // n random 64-byte reads (one per lane), same load shape as a bucket probe
-__global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned long long* buf, uint32_t nbuckets, size_t n,
+static __global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned long long* buf, uint32_t nbuckets, size_t n,
                                                         uint32_t seed, uint32_t* sink) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -341,7 +360,7 @@ __global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned long long
 }
 
 // n random 64-bit CAS (one per lane)
-__global__ __launch_bounds__(256) void k_ceiling_cas(unsigned long long* buf, size_t nwords, size_t n, uint32_t seed) {
+static __global__ __launch_bounds__(256) void k_ceiling_cas(unsigned long long* buf, size_t nwords, size_t n, uint32_t seed) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t w = (static_cast<size_t>(ceiling_index(i, seed)) * 8u + (i & 7)) % nwords;

@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (Launch, Operation, Stats, POLICIES, check, lib, OP_INSERT, OP_LOOKUP, OP_DELETE)
+from ._lib import (Launch, Operation, Stats, Tune, POLICIES, check, lib, OP_INSERT, OP_LOOKUP, OP_DELETE)
 
 
 def _torch():
@@ -120,6 +120,43 @@ class HiveTable:
         check(self.L.hive_mixed(self.h, _ptr(ty), _ptr(k), _ptr(v), k.numel(), _ptr(out_vals), _ptr(status),
                                 _stream_ptr(stream)), "hive_mixed")
         return out_vals, status
+
+    def mixed_stream(self, types, keys, vals, offsets, mode=2, out_vals=None, status=None, stream=None):
+        """Run consecutive mixed batches (batch b = ops[offsets[b]:offsets[b+1]]).
+
+        mode: 0 per-batch launches, 1 captured hipGraph, 2 persistent kernel.
+        Returns (out_vals, status, batch_done_ns numpy array, setup_ms)."""
+        torch = _torch()
+        ty = _dev(types, torch.uint8)
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        nb = off.size - 1
+        if out_vals is None:
+            out_vals = torch.empty(k.numel(), dtype=torch.int32, device=k.device)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        done = np.zeros(max(nb, 1), np.float64)
+        setup = C.c_double(0.0)
+        check(self.L.hive_mixed_stream(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p), nb,
+                                       _ptr(out_vals), _ptr(status), int(mode), done.ctypes.data_as(C.c_void_p),
+                                       C.byref(setup), _stream_ptr(stream)), "hive_mixed_stream")
+        return out_vals, status, done[:nb], setup.value
+
+    def autotune_stream(self, types, keys, vals, offsets, stream=None):
+        """Time launch shapes on a scratch copy of this table; keep the fastest."""
+        torch = _torch()
+        ty = _dev(types, torch.uint8)
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        res = Tune()
+        check(self.L.hive_autotune_stream(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
+                                          off.size - 1, C.byref(res), _stream_ptr(stream)), "hive_autotune_stream")
+        return {f: getattr(res, f) for f, _ in Tune._fields_}
+
+    def set_stream_shape(self, tile_ops=0, blocks_per_cu=0):
+        check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu)), "hive_set_stream_shape")
 
     # --------------------------------------------- reference record ops
     def _ops(self, fn, ops, results, stream):

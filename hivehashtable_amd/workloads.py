@@ -26,27 +26,37 @@ def _bijection_params(seed):
     return mult, xors
 
 
-def key_stream(seed, start, count):
-    """f(start), ..., f(start+count-1) for a seeded bijection f of u32, so the
-    outputs are distinct for distinct indices: rounds of xor with a constant,
-    xor-shift right and multiplication by an odd constant (each invertible
-    modulo 2^32)."""
+def key_stream(seed, start, count, bits=32):
+    """f(start), ..., f(start+count-1) for a seeded bijection f of [0, 2^bits),
+    so the outputs are distinct for distinct indices: rounds of xor with a
+    constant, xor-shift right and multiplication by an odd constant, each
+    invertible modulo 2^bits."""
     mult, xors = _bijection_params(seed)
-    x = np.arange(start, start + count, dtype=np.uint64) & np.uint64(0xFFFFFFFF)
-    m32 = np.uint64(0xFFFFFFFF)
+    mask = np.uint64((1 << bits) - 1)
+    x = np.arange(start, start + count, dtype=np.uint64) & mask
+    s1, s2 = np.uint64(bits // 2), np.uint64(bits // 2 - 3)
     for r in range(3):
-        x ^= xors[r]
-        x ^= x >> np.uint64(16)
-        x = (x * mult[r]) & m32
-        x ^= x >> np.uint64(13)
+        x ^= xors[r] & mask
+        x ^= x >> s1
+        x = (x * mult[r]) & mask
+        x ^= x >> s2
     return x.astype(np.uint32)
 
 
-def distinct_keys(n, seed, exclude=None, start=0):
-    """n distinct u32 keys in [1, 2^32-2], a seeded pseudo-random permutation
-    prefix (sampling without replacement).  Keys in `exclude` are skipped
-    (small sets only); `start` continues the same stream, so
-    distinct_keys(m, seed, start=n) never repeats distinct_keys(n, seed)."""
+def distinct_keys(n, seed, exclude=None, start=0, bits=32):
+    """n distinct u32 keys, a seeded pseudo-random permutation prefix
+    (sampling without replacement): in [1, 2^32-2] for bits=32, in
+    [1, 2^bits] for bits < 32.  Keys in `exclude` are skipped (small sets
+    only); `start` continues the same stream, so distinct_keys(m, seed,
+    start=n) never repeats distinct_keys(n, seed)."""
+    if bits < 32:
+        if start + n > (1 << bits):
+            raise ValueError("key space exhausted")
+        cand = key_stream(seed, start, n + (0 if exclude is None else 0), bits).astype(np.uint64) + 1
+        out = cand.astype(np.uint32)
+        if exclude is not None:
+            raise ValueError("exclude is only supported for bits=32")
+        return np.ascontiguousarray(out)
     out = np.empty(0, np.uint32)
     excl = None if exclude is None else np.unique(np.asarray(exclude, np.uint32))
     pos = start
@@ -105,25 +115,43 @@ def log_uniform_sizes(count, lo, hi, seed):
     return np.clip(np.round(np.exp(e)).astype(np.int64), lo, hi)
 
 
-def mixed_batches(live_keys, key_space, sizes, mix=(0.5, 0.4, 0.1), seed=0):
-    """Mixed batches: each op INSERT/LOOKUP/DELETE with probabilities `mix`;
-    insert and lookup keys uniform over [1, key_space]; deletes drawn from
-    keys inserted so far (the preload plus earlier batches).  Yields
-    (types u8, keys u32, vals u32) per batch."""
+def mixed_stream(live_keys, key_space, sizes, mix=(0.5, 0.4, 0.1), seed=0):
+    """A stream of mixed batches, all at once (vectorized).
+
+    Each op is INSERT/LOOKUP/DELETE with probabilities `mix`; insert and
+    lookup keys are uniform over [1, key_space]; a delete draws uniformly from
+    the keys inserted before its batch (the preload `live_keys` plus inserts
+    of earlier batches).  Returns (types u8, keys u32, vals u32, offsets u64
+    of len(sizes) + 1)."""
     rng = np.random.default_rng(seed)
-    pool = list(np.asarray(live_keys, np.uint32))
-    pool_arr = np.asarray(pool, np.uint32)
-    for sz in sizes:
-        types = rng.choice(3, size=int(sz), p=list(mix)).astype(np.uint8)
-        keys = rng.integers(1, key_space + 1, size=int(sz), dtype=np.uint64).astype(np.uint32)
-        dl = types == 2
-        if dl.any() and pool_arr.size:
-            keys[dl] = pool_arr[rng.integers(0, pool_arr.size, size=int(dl.sum()))]
-        vals = rng.integers(0, 2**32, size=int(sz), dtype=np.uint64).astype(np.uint32)
-        ins = keys[types == 0]
-        if ins.size:
-            pool_arr = np.concatenate([pool_arr, ins])
-        yield types, keys, vals
+    sizes = np.asarray(sizes, np.int64)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(offsets[-1])
+    types = rng.choice(3, size=total, p=list(mix)).astype(np.uint8)
+    keys = rng.integers(1, key_space + 1, size=total, dtype=np.uint64).astype(np.uint32)
+    vals = rng.integers(0, 2**32, size=total, dtype=np.uint64).astype(np.uint32)
+    live = np.asarray(live_keys, np.uint32)
+    ins_pos = np.flatnonzero(types == 0)
+    pool = np.concatenate([live, keys[ins_pos]])
+    batch_of = np.repeat(np.arange(sizes.size), sizes)
+    # inserts of batches before each batch
+    ins_per_batch = np.bincount(batch_of[ins_pos], minlength=sizes.size)
+    ins_before = np.concatenate([[0], np.cumsum(ins_per_batch)[:-1]])
+    dpos = np.flatnonzero(types == 2)
+    if dpos.size:
+        avail = live.size + ins_before[batch_of[dpos]]
+        has = avail > 0
+        pick = (rng.random(dpos.size) * np.maximum(avail, 1)).astype(np.int64)
+        keys[dpos[has]] = pool[pick[has]]
+    return types, keys, vals, offsets
+
+
+def mixed_batches(live_keys, key_space, sizes, mix=(0.5, 0.4, 0.1), seed=0):
+    """mixed_stream, yielded batch by batch as (types, keys, vals)."""
+    types, keys, vals, off = mixed_stream(live_keys, key_space, sizes, mix, seed)
+    for b in range(len(off) - 1):
+        lo, hi = int(off[b]), int(off[b + 1])
+        yield types[lo:hi], keys[lo:hi], vals[lo:hi]
 
 
 def zipf_sampler(n_items, theta, seed):

@@ -131,6 +131,47 @@ hive_status_t hive_insert_ops(hive_table_t* t, const hive_operation_t* ops, size
 hive_status_t hive_lookup_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results, void* stream);
 hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results, void* stream);
 
+/* ---- streams of mixed batches (no reference counterpart: its harness
+ * launches and synchronizes one hive_mixed_kernel per batch,
+ * src/hive_dispatch.cu:119-164).  Batch b is ops [h_offsets[b],
+ * h_offsets[b+1]) of the concatenated device arrays; every batch keeps the
+ * hive_mixed order (lookups, deletes, inserts) and batch b+1 runs after batch
+ * b, so all modes return the same results.
+ *   HIVE_STREAM_LAUNCHES   three kernels per batch queued on the stream
+ *   HIVE_STREAM_GRAPH      the same launches captured once into a hipGraph
+ *                          and replayed (capture + instantiate in *setup_ms)
+ *   HIVE_STREAM_PERSISTENT one resident kernel that claims tiles of
+ *                          (batch, phase) from a device work queue
+ * h_offsets: host array of nbatches + 1 offsets.  batch_done_ns (host, may be
+ * NULL) receives each batch's completion time in ns after the stream start
+ * (device wall clock).  Synchronizes the stream. */
+enum { HIVE_STREAM_LAUNCHES = 0, HIVE_STREAM_GRAPH = 1, HIVE_STREAM_PERSISTENT = 2 };
+hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                                const uint64_t* h_offsets, size_t nbatches, uint32_t* out_vals, uint8_t* status,
+                                int mode, double* batch_done_ns, double* setup_ms, void* stream);
+/* Shape of the persistent kernel: ops per tile (multiple of 256, 0 = about
+ * one tile per resident block, 256..4096) and resident blocks per CU (0 = 4). */
+hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu);
+
+/* Autotuner for the stream forms (no reference counterpart): times the
+ * sample stream on a scratch copy of the table -- so at the table's current
+ * load factor, leaving the table unchanged -- for per-batch launches at
+ * 128..1024 threads per block, the captured graph, and the persistent kernel
+ * over tile sizes {auto, 256, 1024, 4096} x {2, 4, 8} blocks per CU, then
+ * stores the fastest launch shapes in the handle.  Synchronizes. */
+typedef struct {
+  uint32_t block_threads;  /* chosen threads per block of the plain kernels */
+  uint32_t tile_ops;       /* chosen persistent tile (0 = sized per batch) */
+  uint32_t blocks_per_cu;  /* chosen persistent residency */
+  int best_mode;           /* HIVE_STREAM_* fastest on the sample */
+  double launches_ms;      /* best sample time per mode (device clock) */
+  double graph_ms;
+  double persistent_ms;
+  uint32_t candidates;     /* configurations timed */
+} hive_tune_t;
+hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                                   const uint64_t* h_offsets, size_t nbatches, hive_tune_t* out, void* stream);
+
 /* ---- maintenance
  * hive_resize: rehash every live entry (buckets and stash) into
  *   new_num_slots / 8 buckets; replaces hive_hash_grow / hive_hash_shrink

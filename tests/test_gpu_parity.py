@@ -353,3 +353,70 @@ def test_full_size_lf090_properties(H):
     assert (st2[ok] == 2).all()
     newly = int(np.isin(st2[~ok], (1, 3)).sum())
     assert sum(t.count_exact()) == a + b + newly
+
+
+def _clean_ops(types, keys, off):
+    """Ops with one admissible outcome: key unique in its batch and never
+    inserted twice concurrently in an earlier batch."""
+    nb = len(off) - 1
+    batch = np.repeat(np.arange(nb), np.diff(off).astype(np.int64))
+    comb = (batch.astype(np.uint64) << np.uint64(32)) | keys.astype(np.uint64)
+    _, inv, cnt = np.unique(comb, return_inverse=True, return_counts=True)
+    dup = cnt[inv] > 1
+    first_dup = {}
+    for k, b in zip(keys[dup].tolist(), batch[dup].tolist()):
+        if k not in first_dup or b < first_dup[k]:
+            first_dup[k] = b
+    fd = np.array([first_dup.get(k, nb) for k in keys.tolist()])
+    return (~dup) & (batch < fd)
+
+
+def test_stream_modes_agree_with_phased_oracle(H, O):
+    print("[case] mixed stream: launches / graph / persistent", flush=True)
+    slots = 1 << 18
+    pre = W.distinct_keys(slots // 2, 91)
+    sizes = W.log_uniform_sizes(40, 1 << 6, 1 << 14, 92)
+    sizes[5] = 0  # an empty batch
+    types, keys, vals, off = W.mixed_stream(pre, 1 << 19, sizes, seed=93)
+    o = O.Oracle(slots // 8)
+    o.insert(pre)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for b in range(len(sizes)):
+        lo, hi = int(off[b]), int(off[b + 1])
+        ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    clean = _clean_ops(types, keys, off)
+    norm = lambda s: np.where(s == 3, 1, s)
+    keyset = np.sort(o.dump() & 0xFFFFFFFF)
+    prev = None
+    for mode in (0, 1, 2):
+        t = H.HiveTable(slots)
+        t.insert(pre)
+        gv, gs, done, setup = t.mixed_stream(types, keys, vals, off, mode)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        assert done.size == len(sizes) and (np.diff(done) >= 0).all() and done[-1] > 0
+        assert np.array_equal(norm(gs)[clean], norm(os_)[clean]), mode
+        assert np.array_equal(gv[clean], ov[clean]), mode
+        raw, _ = check_gpu_state(t, O, "Default2Hash")
+        assert np.array_equal(np.sort(raw.live_kv() & 0xFFFFFFFF), keyset), mode
+        if prev is not None:
+            assert np.array_equal(norm(gs)[clean], norm(prev)[clean])
+        prev = gs
+
+
+def test_stream_autotune_leaves_table_unchanged(H, O):
+    print("[case] stream autotune", flush=True)
+    slots = 1 << 16
+    pre = W.distinct_keys(slots // 2, 95)
+    sizes = W.log_uniform_sizes(12, 1 << 6, 1 << 12, 96)
+    types, keys, vals, off = W.mixed_stream(pre, 1 << 18, sizes, seed=97)
+    t = H.HiveTable(slots)
+    t.insert(pre)
+    before = gpu_raw(t, O).live_kv()
+    res = t.autotune_stream(types, keys, vals, off)
+    assert res["candidates"] == 17
+    assert res["block_threads"] in (128, 256, 512, 1024)
+    assert res["tile_ops"] in (0, 256, 1024, 4096) and res["blocks_per_cu"] in (2, 4, 8)
+    assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
+    assert np.array_equal(gpu_raw(t, O).live_kv(), before)
+    assert t.get_launch(0) == (res["block_threads"], 1)
