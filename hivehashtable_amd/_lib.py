@@ -47,12 +47,12 @@ class Stats(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32)]
+    _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32), ("inline_slow_path", C.c_uint32)]
 
 
 class Tune(C.Structure):
-    _fields_ = [("block_threads", C.c_uint32), ("tile_ops", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("best_mode", C.c_int), ("launches_ms", C.c_double), ("graph_ms", C.c_double),
+    _fields_ = [("block_threads", C.c_uint32), ("inline_slow_path", C.c_uint32), ("tile_ops", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int), ("launches_ms", C.c_double), ("graph_ms", C.c_double),
                 ("persistent_ms", C.c_double), ("candidates", C.c_uint32)]
 
 
@@ -104,7 +104,7 @@ def lib():
         "hive_hash_table_kernel_dispatch": [p, sz, sz, sz, sz, sz, C.POINTER(C.c_double), p, i32, i32, C.c_char_p],
         "hive_hash_table_kernel_dispatch_ycsb": [p, p, sz, sz, sz, sz, sz, C.POINTER(C.c_double), p, i32, i32],
         "hive_mixed_stream": [p, p, p, p, p, sz, p, p, i32, p, C.POINTER(C.c_double), p],
-        "hive_set_stream_shape": [p, u32, u32],
+        "hive_set_stream_shape": [p, u32, u32, u32],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
         "hive_ceiling_gather": [p, u64, u64, u32, p, p],
         "hive_ceiling_cas": [p, u64, u64, u32, p],

@@ -396,11 +396,17 @@ __device__ __forceinline__ bool finish_move(unsigned long long* src, unsigned lo
 //   Stage D stash push      include/hive_hash_insert.cuh:489-517
 // The victim is copied into its alternate bucket (claimed by CAS 0 -> victim)
 // before its source slot is CAS-replaced, so it is never absent from both.
+// pre_victim / pre_alt: slot 0 of H0 as read before the lock and its
+// alternate bucket (~0u if none); that bucket is read in the same round as the
+// coherent reload of H0, so the first kick usually needs no extra round trip.
 template <class P>
-__device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val, uint32_t c) {
+__device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val, uint32_t c,
+                                 unsigned long long pre_victim, uint32_t pre_alt) {
   const unsigned long long kv = kv_pack(key, val);
   unsigned long long* cb = bucket_ptr(t, c);
   unsigned long long* sb = stash_ptr(t, c);
+  unsigned long long pre[kSlots];
+  bool have_pre = false;
   for (int attempt = 0; attempt < 256; ++attempt) {
     bool retry = false;
     // one coherent round: the canonical bucket and its hint bits.  Only
@@ -409,6 +415,10 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     unsigned long long cs[kSlots];
     load_bucket_sync(cb, cs);
     const uint32_t hint = meta_bits(ld_sync32(t.meta + (c >> 4)), c);
+    if (attempt == 0 && pre_alt != ~0u) {
+      load_bucket(bucket_ptr(t, pre_alt), pre);
+      have_pre = true;
+    }
     {
       const uint32_t m = match_mask(cs, key);
       if (m) {
@@ -461,7 +471,12 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       // slots of the alternate bucket are only ever claimed by CAS 0 -> kv, so
       // a stale plain view can only overstate its empty slots; the CAS decides
       unsigned long long as[kSlots];
-      load_bucket(ab, as);
+      if (kick == 0 && have_pre && victim == pre_victim && a == pre_alt) {
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) as[j] = pre[j];
+      } else {
+        load_bucket(ab, as);
+      }
       uint32_t ae = empty_mask(as);
       if (!ae) continue;
       const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
@@ -497,11 +512,20 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
 template <class P>
 __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
   const uint32_t c = bucket_of<P>(0, key, t.nb);
+  // plain read of H0 in the same round as the first lock attempt: names the
+  // first victim, whose alternate bucket is then fetched with the coherent reload
+  unsigned long long ps[kSlots];
+  load_bucket(bucket_ptr(t, c), ps);
   uint8_t result = R_PENDING;
   uint32_t backoff = 1;
   while (result == R_PENDING) {
     if (try_lock(t.locks + c)) {
-      result = locked_insert<P>(t, key, val, c);
+      uint32_t pa = ~0u;
+      if (ps[0] != 0ull) {
+        pa = alt_bucket<P>(kv_key(ps[0]), c, t.nb);
+        if (pa == c) pa = ~0u;
+      }
+      result = locked_insert<P>(t, key, val, c, ps[0], pa);
       unlock(t.locks + c);
     } else {
       for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);

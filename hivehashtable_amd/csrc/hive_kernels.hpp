@@ -109,37 +109,55 @@ struct BatchClock {
 // ---------------------------------------------------------------- insert
 // phase < 0: every op is an insert.  phase == OP_INSERT: only ops whose type
 // is INSERT (the insert phase of a mixed batch).
-template <class P, class Src, class Out>
+// Inline = false: ops whose fast path cannot finish are staged in an LDS queue
+//   and, after one block barrier, run their slow path on packed lanes
+//   (fewer partially idle wavefronts: throughput).
+// Inline = true: each lane continues straight into its slow path, no barrier
+//   (shortest critical path: latency of small batches).
+template <class P, class Src, class Out, bool Inline>
 __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase,
                                                       BatchClock clk = BatchClock{nullptr, nullptr}) {
-  __shared__ uint32_t queue[kMaxBlock];
-  __shared__ uint32_t qn;
-  if (threadIdx.x == 0) qn = 0;
-  __syncthreads();
   const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
   const size_t i = base + threadIdx.x;
-  int dtab = 0, dstash = 0;
-  if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-    const uint32_t key = src.key(i);
-    const uint8_t r = fast_insert<P>(t, key, src.val(i, key));
-    if (r == R_PENDING) {
-      queue[atomicAdd(&qn, 1u)] = threadIdx.x;
-    } else {
+  int dtab = 0, dstash = 0, dslow = 0;
+  if constexpr (Inline) {
+    if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
+      const uint32_t key = src.key(i);
+      const uint32_t val = src.val(i, key);
+      uint8_t r = fast_insert<P>(t, key, val);
+      if (r == R_PENDING) {
+        r = slow_insert<P>(t, key, val);
+        dslow = 1;
+      }
       out.put(i, r, 0);
       dtab += (r == R_INSERTED);
+      dstash += (r == R_STASHED);
     }
-  }
-  __syncthreads();
-  const uint32_t pending = qn;
-  int dslow = 0;
-  if (threadIdx.x < pending) {  // LDS-staged slow path on packed lanes
-    const size_t idx = base + queue[threadIdx.x];
-    const uint32_t key = src.key(idx);
-    const uint8_t r = slow_insert<P>(t, key, src.val(idx, key));
-    out.put(idx, r, 0);
-    dtab += (r == R_INSERTED);
-    dstash += (r == R_STASHED);
-    dslow = 1;
+  } else {
+    __shared__ uint32_t queue[kMaxBlock];
+    __shared__ uint32_t qn;
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
+      const uint32_t key = src.key(i);
+      const uint8_t r = fast_insert<P>(t, key, src.val(i, key));
+      if (r == R_PENDING) {
+        queue[atomicAdd(&qn, 1u)] = threadIdx.x;
+      } else {
+        out.put(i, r, 0);
+        dtab += (r == R_INSERTED);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < qn) {  // LDS-staged slow path on packed lanes
+      const size_t idx = base + queue[threadIdx.x];
+      const uint32_t key = src.key(idx);
+      const uint8_t r = slow_insert<P>(t, key, src.val(idx, key));
+      out.put(idx, r, 0);
+      dtab += (r == R_INSERTED);
+      dstash += (r == R_STASHED);
+      dslow = 1;
+    }
   }
   wave_count(t, dtab, dstash, dslow);
   clk.finish();

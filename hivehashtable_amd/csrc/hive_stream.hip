@@ -29,99 +29,124 @@ struct StreamArgs {
   const uint32_t* vals;
   uint32_t* out_vals;
   uint8_t* status;
-  const unsigned long long* batch_off;  // nbatch + 1 op offsets
-  const uint32_t* tile_ops;             // ops per tile of each batch
-  const uint32_t* phase_start;          // 3 * nbatch + 1 tile prefix over (batch, phase)
-  uint32_t nphase;                      // 3 * nbatch
-  unsigned int* ticket;                 // next tile to claim
-  unsigned int* done_tiles;             // per phase: tiles completed
-  unsigned long long* ts;               // nbatch + 1: ts[0] stream start, ts[b + 1] batch b done
-  unsigned int* error;                  // set when a dependency wait gives up
-  uint32_t max_tile;                    // largest tile_ops (LDS queue entries)
-  uint32_t phases_in_lds;               // phase_start copied into LDS after the queue
+  const uint4* tile_desc;      // per ticket: {phase q, first op, end op, batch}
+  uint32_t ntiles;             // tickets in the stream
+  unsigned int* ticket;        // next tile to claim
+  unsigned int* tiles_left;    // per phase, one 128-byte line each: tiles not yet completed
+  unsigned int* phases_done;   // phases 0 .. phases_done-1 are complete
+  unsigned long long* ts;      // nbatch + 1: ts[0] stream start, ts[b + 1] batch b done
+  unsigned int* error;         // set when a dependency wait gives up
+  unsigned long long wait_ticks;  // wall-clock ticks one dependency wait may take before it gives up
 };
 
 constexpr uint32_t kStreamBlock = 256;
-constexpr uint32_t kSpinLimit = 1u << 26;  // ~seconds of s_sleep: a stuck dependency ends the kernel
-
-// Persistent kernel.  Phases are numbered q = 3*b + {0 lookup, 1 delete, 2
-// insert}.  A block claims the next tile ticket; tickets are handed out in
-// phase order, so every tile of the phases before the claimed one is owned by
-// a running or finished block and the wait for phase q-1 always ends.  A
-// completed phase q-1 implies every earlier phase completed, so each block
-// keeps the highest phase it has seen complete and waits (with a dependency
-// poll) and acquires (buffer_inv sc1, dropping L1/L2 lines that other XCDs'
-// writes made stale -- no kernel boundary separates phases) only when a
-// ticket needs a later one.  The tile-to-phase map is read from LDS.
-constexpr uint32_t kLdsPhases = 8192;  // phase_start entries cached in LDS (32 KiB at most)
+constexpr uint32_t kDoneStride = 32;        // u32 words per phase counter (own 128-byte line)
+constexpr double kWaitLimitSeconds = 10.0;  // bounded waits: a stuck dependency ends the kernel
 constexpr uint32_t kMaxTileOps = 4096;
 
-template <class P>
+// One thread per ticket: its descriptor, and (first tile of a phase) the
+// phase's count of tiles left.
+__global__ void k_tile_desc(const uint32_t* phase_start, uint32_t nphase, const unsigned long long* batch_off,
+                            const uint32_t* tile_ops, uint4* desc, uint32_t ntiles, unsigned int* tiles_left) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ntiles) return;
+  uint32_t lo = 0, hi = nphase;  // phase_start[lo] <= k < phase_start[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (phase_start[mid] <= k) lo = mid; else hi = mid;
+  }
+  const uint32_t q = lo, b = q / 3, tops = tile_ops[b];
+  const unsigned long long bend = batch_off[b + 1];
+  const unsigned long long first = batch_off[b] + static_cast<unsigned long long>(k - phase_start[q]) * tops;
+  const unsigned long long end = first + tops < bend ? first + tops : bend;
+  desc[k] = make_uint4(q, static_cast<uint32_t>(first), static_cast<uint32_t>(end), b);
+  if (k == phase_start[q]) tiles_left[static_cast<size_t>(q) * kDoneStride] = phase_start[q + 1] - phase_start[q];
+}
+
+// Wait until phase r is complete.  Phases complete in order, and the last
+// tile of a phase raises phases_done, so a block waiting far ahead backs off
+// in proportion to its distance on that one watermark; only a block waiting
+// on the phase that is running polls that phase's own tile counter.
+__device__ bool wait_phase(const StreamArgs& a, uint32_t r) {
+  const unsigned long long t0 = wall_clock64();
+  uint32_t spins = 0;
+  for (;;) {
+    const uint32_t pd = ld_sync32(a.phases_done);
+    if (pd > r) return true;
+    if (pd == r) {
+      while (ld_sync32(a.tiles_left + static_cast<size_t>(r) * kDoneStride) != 0u) {
+        if ((++spins & 255u) == 0 && (ld_sync32(a.error) != 0u || wall_clock64() - t0 > a.wait_ticks)) return false;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      return true;
+    }
+    if (ld_sync32(a.error) != 0u || wall_clock64() - t0 > a.wait_ticks) return false;
+    for (uint32_t d = min(r - pd, 16u); d > 0; --d) __builtin_amdgcn_s_sleep(32);
+  }
+}
+
+// Persistent kernel.  Phases are numbered q = 3*b + {0 lookup, 1 delete, 2
+// insert}.  A block claims tile tickets one ahead of the tile it works on;
+// tickets are handed out in phase order, so the lowest unfinished tile always
+// has an owner whose earlier tile is done and whose dependencies (all lower
+// tickets) are complete: every wait ends.  A completed phase q-1 implies
+// every earlier phase completed, so a block waits (and acquires: buffer_inv
+// sc1 drops L1/L2 lines that other XCDs' writes made stale -- no kernel
+// boundary separates phases) only when a ticket needs a later phase than it
+// has seen complete.  The first chunk of a tile's inputs (read-only) is
+// loaded before the wait.
+template <class P, bool Inline>
 __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs a) {
-  __shared__ uint32_t s_k, s_q, s_abort, qn;
-  __shared__ int s_known;  // highest phase this block has seen complete (and acquired after)
-  extern __shared__ uint32_t dyn_lds[];  // [max_tile] slow-path queue, then [nphase + 1] phase map
-  uint32_t* queue = dyn_lds;
-  uint32_t* lds_phase = dyn_lds + a.max_tile;
-  const bool phase_in_lds = a.phases_in_lds != 0;
-  if (phase_in_lds)
-    for (uint32_t i = threadIdx.x; i <= a.nphase; i += kStreamBlock) lds_phase[i] = a.phase_start[i];
-  if (threadIdx.x == 0) s_known = -1;
-  __syncthreads();
-  const uint32_t* ps = phase_in_lds ? lds_phase : a.phase_start;
+  __shared__ uint4 s_desc;
+  __shared__ uint32_t s_abort, s_fresh, qn;
+  extern __shared__ uint32_t queue[];  // staged slow path: one entry per op of the largest tile
   int dtab = 0, dstash = 0, dslow = 0;
-  const uint32_t total = ps[a.nphase];
+  int known = -1;  // thread 0: highest phase this block has seen complete
+  uint32_t next_k = 0;
+  uint4 next_d = make_uint4(0, 0, 0, 0);
+  if (threadIdx.x == 0) {
+    next_k = atomicAdd(a.ticket, 1u);
+    if (next_k < a.ntiles) next_d = a.tile_desc[next_k];
+  }
   for (;;) {
     if (threadIdx.x == 0) {
-      const uint32_t k = atomicAdd(a.ticket, 1u);
-      uint32_t q = 0, abort = 0;
-      if (k < total) {
-        uint32_t lo = 0, hi = a.nphase;  // ps[lo] <= k < ps[hi]
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (ps[mid] <= k) lo = mid; else hi = mid;
-        }
-        q = lo;
-        if (static_cast<int>(q) - 1 > s_known) {
-          const uint32_t need = ps[q] - ps[q - 1];
-          uint32_t spins = 0;
-          while (ld_sync32(a.done_tiles + q - 1) < need) {
-            if (ld_sync32(a.error) != 0u || ++spins > kSpinLimit) {
-              atomicOr(a.error, 1u);
-              abort = 1;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-          }
-          if (!abort) s_known = -2 - static_cast<int>(q - 1);  // new: acquire below, then record
-        }
-      }
-      s_k = k;
-      s_q = q;
-      s_abort = abort;
+      s_desc = next_k < a.ntiles ? next_d : make_uint4(0xFFFFFFFFu, 0, 0, 0);
       qn = 0;
     }
     __syncthreads();
-    const uint32_t k = s_k, q = s_q;
-    if (k >= total || s_abort) break;
-    if (s_known < -1) {  // this block just saw a later phase complete
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __syncthreads();
-      if (threadIdx.x == 0) s_known = -2 - s_known;
+    const uint4 d = s_desc;
+    if (d.x == 0xFFFFFFFFu) break;
+    const uint32_t q = d.x, b = d.w, p = q % 3;
+    const size_t lo = d.y, hi = d.z;
+    // inputs of the first chunk: immutable, so loaded ahead of the dependency wait
+    size_t i = lo + threadIdx.x;
+    int ty = i < hi ? static_cast<int>(a.types[i]) : -1;
+    uint32_t key = i < hi ? a.keys[i] : 0u;
+    uint32_t val = (i < hi && a.vals) ? a.vals[i] : key + 1u;
+    if (threadIdx.x == 0) {
+      next_k = atomicAdd(a.ticket, 1u);
+      uint32_t abort = 0, fresh = 0;
+      if (static_cast<int>(q) - 1 > known) {
+        if (wait_phase(a, q - 1)) {
+          known = static_cast<int>(q) - 1;
+          fresh = 1;
+        } else {
+          atomicOr(a.error, 1u);
+          abort = 1;
+        }
+      }
+      if (next_k < a.ntiles) next_d = a.tile_desc[next_k];
+      s_abort = abort;
+      s_fresh = fresh;
     }
-    const uint32_t b = q / 3, p = q % 3;
-    const uint32_t tile = k - ps[q];
-    const size_t bend = a.batch_off[b + 1];
-    const uint32_t tops = a.tile_ops[b];
-    const size_t lo = a.batch_off[b] + static_cast<size_t>(tile) * tops;
-    const size_t hi = lo + tops < bend ? lo + tops : bend;
-    for (size_t base = lo; base < hi; base += kStreamBlock) {
-      const size_t i = base + threadIdx.x;
-      const int ty = i < hi ? static_cast<int>(a.types[i]) : -1;
+    __syncthreads();
+    if (s_abort) break;
+    if (s_fresh) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (size_t base = lo;;) {
       if (p == 0) {
         if (ty == OP_LOOKUP) {
           uint32_t v;
-          const uint8_t r = lookup_one<P>(t, a.keys[i], v);
+          const uint8_t r = lookup_one<P>(t, key, v);
           if (a.status) a.status[i] = r;
           if (a.out_vals) a.out_vals[i] = v;
         } else if (ty > OP_DELETE || ty < -1) {
@@ -131,46 +156,62 @@ __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs
       } else if (p == 1) {
         if (ty == OP_DELETE) {
           int rt, rs;
-          const uint8_t r = erase_one<P>(t, a.keys[i], rt, rs);
+          const uint8_t r = erase_one<P>(t, key, rt, rs);
           if (a.status) a.status[i] = r;
           if (a.out_vals) a.out_vals[i] = 0;
           dtab -= rt;
           dstash -= rs;
         }
       } else if (ty == OP_INSERT) {
-        const uint32_t key = a.keys[i];
-        const uint8_t r = fast_insert<P>(t, key, a.vals ? a.vals[i] : key + 1u);
+        uint8_t r = fast_insert<P>(t, key, val);
         if (r == R_PENDING) {
-          queue[atomicAdd(&qn, 1u)] = static_cast<uint32_t>(i - lo);
-        } else {
+          if constexpr (Inline) {
+            r = slow_insert<P>(t, key, val);
+            ++dslow;
+            dstash += (r == R_STASHED);
+          } else {
+            queue[atomicAdd(&qn, 1u)] = static_cast<uint32_t>(i - lo);
+          }
+        }
+        if (r != R_PENDING) {
           if (a.status) a.status[i] = r;
           if (a.out_vals) a.out_vals[i] = 0;
           dtab += (r == R_INSERTED);
         }
       }
+      base += kStreamBlock;
+      if (base >= hi) break;
+      i = base + threadIdx.x;
+      ty = i < hi ? static_cast<int>(a.types[i]) : -1;
+      key = i < hi ? a.keys[i] : 0u;
+      val = (i < hi && a.vals) ? a.vals[i] : key + 1u;
     }
-    if (p == 2) {  // LDS-staged slow path of the whole tile on packed lanes
-      __syncthreads();
-      const uint32_t pend = qn;
-      for (uint32_t j = threadIdx.x; j < pend; j += kStreamBlock) {
-        const size_t idx = lo + queue[j];
-        const uint32_t key = a.keys[idx];
-        const uint8_t r = slow_insert<P>(t, key, a.vals ? a.vals[idx] : key + 1u);
-        if (a.status) a.status[idx] = r;
-        if (a.out_vals) a.out_vals[idx] = 0;
-        dtab += (r == R_INSERTED);
-        dstash += (r == R_STASHED);
-        ++dslow;
+    if constexpr (!Inline) {
+      if (p == 2) {  // LDS-staged slow path of the whole tile on packed lanes
+        __syncthreads();
+        const uint32_t pend = qn;
+        for (uint32_t j = threadIdx.x; j < pend; j += kStreamBlock) {
+          const size_t idx = lo + queue[j];
+          const uint32_t k2 = a.keys[idx];
+          const uint8_t r = slow_insert<P>(t, k2, a.vals ? a.vals[idx] : k2 + 1u);
+          if (a.status) a.status[idx] = r;
+          if (a.out_vals) a.out_vals[idx] = 0;
+          dtab += (r == R_INSERTED);
+          dstash += (r == R_STASHED);
+          ++dslow;
+        }
       }
     }
     wait_vmem();  // this thread's table writes are performed before the tile is published
     __syncthreads();
     if (threadIdx.x == 0) {
-      const uint32_t need = ps[q + 1] - ps[q];
-      const uint32_t prev = atomicAdd(a.done_tiles + q, 1u);
-      if (p == 2 && prev + 1 == need)
-        __hip_atomic_store(a.ts + b + 1, static_cast<unsigned long long>(wall_clock64()), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t prev = atomicSub(a.tiles_left + static_cast<size_t>(q) * kDoneStride, 1u);
+      if (prev == 1u) {
+        atomicMax(a.phases_done, q + 1);
+        if (p == 2)
+          __hip_atomic_store(a.ts + b + 1, static_cast<unsigned long long>(wall_clock64()), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
   wave_count(t, dtab, dstash, dslow);
@@ -217,14 +258,17 @@ hive_status_t enqueue_launches(hive_table_t* t, const uint8_t* types, const uint
 
 extern "C" {
 
-hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu) {
+hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu,
+                                    uint32_t inline_slow_path) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK) return r;
   if (tile_ops && (tile_ops % kStreamBlock != 0 || tile_ops > kMaxTileOps))
     return fail(HIVE_EINVAL, "tile_ops must be 0 or a multiple of 256 up to 4096");
   if (blocks_per_cu > 16) return fail(HIVE_EINVAL, "blocks_per_cu must be 0..16");
+  if (inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
   t->stream_tile = tile_ops;
   t->stream_blocks_per_cu = blocks_per_cu;
+  t->stream_inline = inline_slow_path;
   return HIVE_OK;
 }
 
@@ -289,8 +333,18 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
       if (rc != HIVE_OK) return rc;
     }
   } else {
-    // persistent kernel: per-batch tile size, phase prefix, device queue state
-    const unsigned bpc = t->stream_blocks_per_cu ? t->stream_blocks_per_cu : 4;
+    // persistent kernel: per-batch tile size, phase prefix, per-ticket descriptors
+    if (h_offsets[nbatches] > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "persistent stream: more than 2^32-1 ops");
+    const void* kfn = nullptr;
+    if (t->stream_inline) {
+      HIVE_POLICY_SWITCH(t->policy, (kfn = reinterpret_cast<const void*>(k_stream<P, true>)));
+    } else {
+      HIVE_POLICY_SWITCH(t->policy, (kfn = reinterpret_cast<const void*>(k_stream<P, false>)));
+    }
+    int occ = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kStreamBlock, t->stream_inline ? 0 : kMaxTileOps * 4));
+    unsigned bpc = t->stream_blocks_per_cu ? t->stream_blocks_per_cu : 4;
+    if (occ > 0 && bpc > static_cast<unsigned>(occ)) bpc = static_cast<unsigned>(occ);  // every block resident
     const unsigned grid = static_cast<unsigned>(ncu) * bpc;
     std::vector<uint32_t> tile_ops(nbatches), phase_start(3 * nbatches + 1);
     uint64_t acc = 0;
@@ -300,7 +354,7 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
       if (!tops) {  // about one tile per resident block, 256..4096 ops
         uint64_t want = (n + grid - 1) / grid;
         want = (want + kStreamBlock - 1) / kStreamBlock * kStreamBlock;
-        tops = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(want, kStreamBlock), 4096));
+        tops = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(want, kStreamBlock), kMaxTileOps));
       }
       tile_ops[b] = tops;
       const uint64_t tiles = n ? (n + tops - 1) / tops : 1;  // an empty batch still completes through one tile
@@ -311,32 +365,39 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
     }
     if (acc >= 0xFFFFFFFFull - grid) return fail(HIVE_EINVAL, "stream too long for one persistent launch");
     phase_start[3 * nbatches] = static_cast<uint32_t>(acc);
+    const uint32_t ntiles = static_cast<uint32_t>(acc), nphase = static_cast<uint32_t>(3 * nbatches);
     unsigned long long* d_off = nullptr;
     uint32_t *d_tile = nullptr, *d_phase = nullptr;
-    unsigned int *d_ticket = nullptr, *d_done = nullptr;
+    uint4* d_desc = nullptr;
+    unsigned int *d_ctl = nullptr, *d_left = nullptr;
     HIP_TRY(ar.alloc(&d_off, nbatches + 1));
     HIP_TRY(ar.alloc(&d_tile, nbatches));
-    HIP_TRY(ar.alloc(&d_phase, 3 * nbatches + 1));
-    HIP_TRY(ar.alloc(&d_ticket, 2));
-    HIP_TRY(ar.alloc(&d_done, 3 * nbatches));
+    HIP_TRY(ar.alloc(&d_phase, nphase + 1));
+    HIP_TRY(ar.alloc(&d_desc, ntiles));
+    HIP_TRY(ar.alloc(&d_ctl, 3 * kDoneStride));  // ticket, phases_done, error: one line each
+    HIP_TRY(ar.alloc(&d_left, static_cast<size_t>(nphase) * kDoneStride));
     HIP_TRY(hipMemcpyAsync(d_off, h_offsets, (nbatches + 1) * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_tile, tile_ops.data(), nbatches * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_phase, phase_start.data(), (3 * nbatches + 1) * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(d_ticket, 0, 8, s));
-    HIP_TRY(hipMemsetAsync(d_done, 0, 3 * nbatches * 4, s));
+    HIP_TRY(hipMemcpyAsync(d_phase, phase_start.data(), (nphase + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(d_ctl, 0, 3 * kDoneStride * 4, s));
+    k_tile_desc<<<(ntiles + 255) / 256, 256, 0, s>>>(d_phase, nphase, d_off, d_tile, d_desc, ntiles, d_left);
+    HIP_CHECK_LAUNCH();
     HIP_TRY(hipStreamSynchronize(s));
     if (setup_ms)
       *setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     const uint32_t max_tile = *std::max_element(tile_ops.begin(), tile_ops.end());
-    const uint32_t in_lds = 3 * nbatches + 1 <= kLdsPhases ? 1u : 0u;
-    const size_t lds = (static_cast<size_t>(max_tile) + (in_lds ? 3 * nbatches + 1 : 0)) * 4;
-    StreamArgs args{types, keys, vals, out_vals, status, d_off, d_tile, d_phase,
-                    static_cast<uint32_t>(3 * nbatches), d_ticket, d_done, ts, d_ticket + 1, max_tile, in_lds};
+    StreamArgs args{types, keys, vals, out_vals, status, d_desc, ntiles, d_ctl, d_left,
+                    d_ctl + kDoneStride, ts, d_ctl + 2 * kDoneStride,
+                    static_cast<unsigned long long>(kWaitLimitSeconds * 1e3 * khz)};
     k_stamp<<<1, 64, 0, s>>>(ts);
-    HIVE_POLICY_SWITCH(t->policy, (k_stream<P><<<grid, kStreamBlock, lds, s>>>(t->v, args)));
+    if (t->stream_inline) {
+      HIVE_POLICY_SWITCH(t->policy, (k_stream<P, true><<<grid, kStreamBlock, 0, s>>>(t->v, args)));
+    } else {
+      HIVE_POLICY_SWITCH(t->policy, (k_stream<P, false><<<grid, kStreamBlock, max_tile * 4, s>>>(t->v, args)));
+    }
     HIP_CHECK_LAUNCH();
     unsigned int err = 0;
-    HIP_TRY(hipMemcpyAsync(&err, d_ticket + 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&err, d_ctl + 2 * kDoneStride, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (err) return fail(HIVE_EDEVICE, "persistent stream: a phase dependency did not complete");
   }
@@ -400,41 +461,47 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   best.launches_ms = best.graph_ms = best.persistent_ms = 1e300;
   uint32_t cands = 0;
   const uint32_t blocks[] = {128, 256, 512, 1024};
-  for (uint32_t blk : blocks) {
-    for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{blk, 1};
-    double ms = 0;
-    if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
-    ++cands;
-    if (ms < best.launches_ms) {
-      best.launches_ms = ms;
-      best.block_threads = blk;
+  for (uint32_t inl = 0; inl < 2; ++inl)
+    for (uint32_t blk : blocks) {
+      for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{blk, 1, inl};
+      double ms = 0;
+      if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
+      ++cands;
+      if (ms < best.launches_ms) {
+        best.launches_ms = ms;
+        best.block_threads = blk;
+        best.inline_slow_path = inl;
+      }
     }
-  }
-  for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{best.block_threads, 1};
+  for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path};
   if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms)) != HIVE_OK) return r;
   ++cands;
   const uint32_t tiles[] = {0, 256, 1024, 4096};
   const uint32_t bpcs[] = {2, 4, 8};
-  for (uint32_t tile : tiles)
-    for (uint32_t bpc : bpcs) {
-      tmp.stream_tile = tile;
-      tmp.stream_blocks_per_cu = bpc;
-      double ms = 0;
-      if ((r = run(HIVE_STREAM_PERSISTENT, &ms)) != HIVE_OK) return r;
-      ++cands;
-      if (ms < best.persistent_ms) {
-        best.persistent_ms = ms;
-        best.tile_ops = tile;
-        best.blocks_per_cu = bpc;
+  for (uint32_t inl = 0; inl < 2; ++inl)
+    for (uint32_t tile : tiles)
+      for (uint32_t bpc : bpcs) {
+        tmp.stream_tile = tile;
+        tmp.stream_blocks_per_cu = bpc;
+        tmp.stream_inline = inl;
+        double ms = 0;
+        if ((r = run(HIVE_STREAM_PERSISTENT, &ms)) != HIVE_OK) return r;
+        ++cands;
+        if (ms < best.persistent_ms) {
+          best.persistent_ms = ms;
+          best.tile_ops = tile;
+          best.blocks_per_cu = bpc;
+          best.stream_inline = inl;
+        }
       }
-    }
   best.candidates = cands;
   best.best_mode = HIVE_STREAM_PERSISTENT;
   if (best.launches_ms < best.persistent_ms && best.launches_ms <= best.graph_ms) best.best_mode = HIVE_STREAM_LAUNCHES;
   else if (best.graph_ms < best.persistent_ms) best.best_mode = HIVE_STREAM_GRAPH;
-  for (int op = 0; op < 3; ++op) t->launch[op] = hive_launch_t{best.block_threads, 1};
+  for (int op = 0; op < 3; ++op) t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path};
   t->stream_tile = best.tile_ops;
   t->stream_blocks_per_cu = best.blocks_per_cu;
+  t->stream_inline = best.stream_inline;
   *out = best;
   return HIVE_OK;
 }

@@ -155,8 +155,9 @@ class HiveTable:
                                           off.size - 1, C.byref(res), _stream_ptr(stream)), "hive_autotune_stream")
         return {f: getattr(res, f) for f, _ in Tune._fields_}
 
-    def set_stream_shape(self, tile_ops=0, blocks_per_cu=0):
-        check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu)), "hive_set_stream_shape")
+    def set_stream_shape(self, tile_ops=0, blocks_per_cu=0, inline_slow_path=0):
+        check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu), int(inline_slow_path)),
+              "hive_set_stream_shape")
 
     # --------------------------------------------- reference record ops
     def _ops(self, fn, ops, results, stream):
@@ -233,14 +234,14 @@ class HiveTable:
         check(self.L.hive_import_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
                                      cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_import_raw")
 
-    def set_launch(self, op_type, block_threads, keys_per_lane=1):
-        cfg = Launch(block_threads, keys_per_lane)
+    def set_launch(self, op_type, block_threads, keys_per_lane=1, inline_slow_path=0):
+        cfg = Launch(block_threads, keys_per_lane, inline_slow_path)
         check(self.L.hive_set_launch(self.h, op_type, C.byref(cfg)), "hive_set_launch")
 
     def get_launch(self, op_type):
         cfg = Launch()
         check(self.L.hive_get_launch(self.h, op_type, C.byref(cfg)), "hive_get_launch")
-        return cfg.block_threads, cfg.keys_per_lane
+        return cfg.block_threads, cfg.keys_per_lane, cfg.inline_slow_path
 
 
 def make_ops(types, keys):

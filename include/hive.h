@@ -88,8 +88,11 @@ typedef struct {
 
 /* Launch shape of the plain bulk kernels.  0 = library default / autotuned. */
 typedef struct {
-  uint32_t block_threads; /* threads per block: 64..1024, multiple of 64 */
-  uint32_t keys_per_lane; /* 1, 2 or 4 keys processed per lane */
+  uint32_t block_threads;    /* threads per block: 64..1024, multiple of 64 */
+  uint32_t keys_per_lane;    /* keys processed per lane (1) */
+  uint32_t inline_slow_path; /* inserts: 0 = stage slow-path ops in LDS behind one
+                                block barrier (throughput), 1 = each lane runs its
+                                own slow path at once (latency) */
 } hive_launch_t;
 
 /* ---- lifecycle: replaces hiveHashTableCreate / Destroy / Clear
@@ -150,19 +153,24 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
                                 const uint64_t* h_offsets, size_t nbatches, uint32_t* out_vals, uint8_t* status,
                                 int mode, double* batch_done_ns, double* setup_ms, void* stream);
 /* Shape of the persistent kernel: ops per tile (multiple of 256, 0 = about
- * one tile per resident block, 256..4096) and resident blocks per CU (0 = 4). */
-hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu);
+ * one tile per resident block, 256..4096), resident blocks per CU (0 = 4),
+ * and whether insert slow paths run inline (1) or staged per tile (0). */
+hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu,
+                                    uint32_t inline_slow_path);
 
 /* Autotuner for the stream forms (no reference counterpart): times the
  * sample stream on a scratch copy of the table -- so at the table's current
  * load factor, leaving the table unchanged -- for per-batch launches at
- * 128..1024 threads per block, the captured graph, and the persistent kernel
- * over tile sizes {auto, 256, 1024, 4096} x {2, 4, 8} blocks per CU, then
- * stores the fastest launch shapes in the handle.  Synchronizes. */
+ * 128..1024 threads per block x {staged, inline} insert slow path, the
+ * captured graph, and the persistent kernel over tile sizes {auto, 256, 1024,
+ * 4096} x {2, 4, 8} blocks per CU x {staged, inline}, then stores the fastest
+ * launch shapes in the handle.  Synchronizes. */
 typedef struct {
   uint32_t block_threads;  /* chosen threads per block of the plain kernels */
+  uint32_t inline_slow_path; /* chosen insert slow-path placement of the plain kernels */
   uint32_t tile_ops;       /* chosen persistent tile (0 = sized per batch) */
   uint32_t blocks_per_cu;  /* chosen persistent residency */
+  uint32_t stream_inline;  /* chosen insert slow-path placement of the persistent kernel */
   int best_mode;           /* HIVE_STREAM_* fastest on the sample */
   double launches_ms;      /* best sample time per mode (device clock) */
   double graph_ms;

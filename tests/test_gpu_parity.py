@@ -311,11 +311,11 @@ def test_launch_shapes_give_identical_results(H, O):
     keys = W.distinct_keys(int(slots * 0.85), 71)
     lk, _ = W.lookup_set(keys, keys.size, 0.5, 72)
     ref = None
-    for blk in (64, 256, 1024):
+    for blk, inl in ((64, 0), (256, 0), (1024, 0), (64, 1), (512, 1)):
         t = H.HiveTable(slots)
         for op in (0, 1, 2):
-            t.set_launch(op, blk)
-        assert t.get_launch(1) == (blk, 1)
+            t.set_launch(op, blk, inline_slow_path=inl)
+        assert t.get_launch(1) == (blk, 1, inl)
         t.insert(keys)
         gv, gs = t.lookup(lk)
         cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
@@ -389,8 +389,15 @@ def test_stream_modes_agree_with_phased_oracle(H, O):
     norm = lambda s: np.where(s == 3, 1, s)
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
     prev = None
-    for mode in (0, 1, 2):
+    # (mode, plain-kernel slow path inline, persistent tile ops, blocks per CU, persistent inline)
+    shapes = [(0, 0, 0, 0, 0), (0, 1, 0, 0, 0), (1, 0, 0, 0, 0), (2, 0, 0, 0, 0), (2, 0, 256, 8, 1),
+              (2, 0, 1024, 2, 0), (2, 0, 0, 4, 1)]
+    for mode, inl, tile, bpc, sinl in shapes:
+        print(f"[case] stream shape mode={mode} inline={inl} tile={tile} bpc={bpc} stream_inline={sinl}", flush=True)
         t = H.HiveTable(slots)
+        for op in (0, 1, 2):
+            t.set_launch(op, 256, inline_slow_path=inl)
+        t.set_stream_shape(tile, bpc, sinl)
         t.insert(pre)
         gv, gs, done, setup = t.mixed_stream(types, keys, vals, off, mode)
         gv, gs = np_(gv, np.uint32), np_(gs)
@@ -414,9 +421,10 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     t.insert(pre)
     before = gpu_raw(t, O).live_kv()
     res = t.autotune_stream(types, keys, vals, off)
-    assert res["candidates"] == 17
-    assert res["block_threads"] in (128, 256, 512, 1024)
+    assert res["candidates"] == 33
+    assert res["block_threads"] in (128, 256, 512, 1024) and res["inline_slow_path"] in (0, 1)
     assert res["tile_ops"] in (0, 256, 1024, 4096) and res["blocks_per_cu"] in (2, 4, 8)
+    assert res["stream_inline"] in (0, 1)
     assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
-    assert t.get_launch(0) == (res["block_threads"], 1)
+    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"])
