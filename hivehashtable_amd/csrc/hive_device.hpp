@@ -137,36 +137,49 @@ __device__ __forceinline__ uint32_t mix_crc32(uint32_t key) {
 }
 
 // ---------------------------------------------------------------- policies
+// kCoherentReads = false: table reads outside locked sections are plain
+// (cached) loads, valid because a kernel boundary separates every phase.
+struct PolicyBase {
+  static constexpr bool kCoherentReads = false;
+};
 // include/HashPolicies.cuh:10-65
-struct PolicyDefault2 {
+struct PolicyDefault2 : PolicyBase {
   static constexpr int NH = 2;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_hash1(k) : mix_hash2(k); }
 };
-struct PolicyMurmurCity {
+struct PolicyMurmurCity : PolicyBase {
   static constexpr int NH = 2;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_murmur(k) : mix_city(k); }
 };
-struct PolicyTriple {
+struct PolicyTriple : PolicyBase {
   static constexpr int NH = 3;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
     return i == 0 ? mix_hash1(k) : (i == 1 ? mix_hash2(k) : mix_city(k));
   }
 };
-struct PolicyMurmurCityBit {
+struct PolicyMurmurCityBit : PolicyBase {
   static constexpr int NH = 3;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
     return i == 0 ? mix_murmur(k) : (i == 1 ? mix_city(k) : mix_hash1(k));
   }
 };
-struct PolicyLookup2 {
+struct PolicyLookup2 : PolicyBase {
   static constexpr int NH = 2;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) { return i == 0 ? mix_crc64(k) : mix_crc32(k); }
 };
-struct PolicyLookup3 {
+struct PolicyLookup3 : PolicyBase {
   static constexpr int NH = 3;
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
     return i == 0 ? mix_crc64(k) : (i == 1 ? mix_crc32(k) : mix_hash1(k));
   }
+};
+
+// The same hashes with every table read a device-coherent load: for code that
+// runs with no kernel boundary after other XCDs' writes (the persistent stream
+// kernel), so that it needs no L2 invalidation before a phase.
+template <class P>
+struct CoherentReads : P {
+  static constexpr bool kCoherentReads = true;
 };
 
 template <class P>
@@ -246,6 +259,23 @@ __device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, un
   }
 }
 
+// table reads outside locked sections: plain, or coherent under CoherentReads
+template <class P>
+__device__ __forceinline__ void read_bucket(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+  if constexpr (P::kCoherentReads) load_bucket_sync(p, s);
+  else load_bucket(p, s);
+}
+template <class P>
+__device__ __forceinline__ uint32_t read_u32(const uint32_t* p) {
+  if constexpr (P::kCoherentReads) return ld_sync32(p);
+  else return *p;
+}
+template <class P>
+__device__ __forceinline__ unsigned long long read_u64(const unsigned long long* p) {
+  if constexpr (P::kCoherentReads) return ld_sync(p);
+  else return *p;
+}
+
 __device__ __forceinline__ uint32_t match_mask(const unsigned long long (&s)[kSlots], uint32_t key) {
   uint32_t m = 0;
 #pragma unroll
@@ -272,7 +302,8 @@ __device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[kSl
 //  kStashBit of c is set when stash[c] receives its first entry (the
 //  reference's freeMask bit 31, include/hive_hash_insert.cuh:512-514).
 __device__ __forceinline__ uint32_t meta_bits(uint32_t word, uint32_t b) { return (word >> ((b & 15) * 2)) & 3u; }
-__device__ __forceinline__ uint32_t meta_word(const TableView& t, uint32_t b) { return t.meta[b >> 4]; }
+template <class P>
+__device__ __forceinline__ uint32_t meta_word(const TableView& t, uint32_t b) { return read_u32<P>(t.meta + (b >> 4)); }
 __device__ __forceinline__ void meta_set(const TableView& t, uint32_t b, uint32_t bit) {
   __hip_atomic_fetch_or(t.meta + (b >> 4), bit << ((b & 15) * 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -307,8 +338,8 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
   unsigned long long s0[kSlots];
-  load_bucket(bucket_ptr(t, b[0]), s0);
-  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+  read_bucket<P>(bucket_ptr(t, b[0]), s0);
+  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
   {
     const uint32_t m = match_mask(s0, key);
     if (m) {
@@ -321,7 +352,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
       unsigned long long s[kSlots];
-      load_bucket(bucket_ptr(t, b[i]), s);
+      read_bucket<P>(bucket_ptr(t, b[i]), s);
       const uint32_t m = match_mask(s, key);
       if (m) {
         const int j = __builtin_ctz(m);
@@ -416,7 +447,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     load_bucket_sync(cb, cs);
     const uint32_t hint = meta_bits(ld_sync32(t.meta + (c >> 4)), c);
     if (attempt == 0 && pre_alt != ~0u) {
-      load_bucket(bucket_ptr(t, pre_alt), pre);
+      read_bucket<P>(bucket_ptr(t, pre_alt), pre);
       have_pre = true;
     }
     {
@@ -475,7 +506,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) as[j] = pre[j];
       } else {
-        load_bucket(ab, as);
+        read_bucket<P>(ab, as);
       }
       uint32_t ae = empty_mask(as);
       if (!ae) continue;
@@ -515,7 +546,7 @@ __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
   // plain read of H0 in the same round as the first lock attempt: names the
   // first victim, whose alternate bucket is then fetched with the coherent reload
   unsigned long long ps[kSlots];
-  load_bucket(bucket_ptr(t, c), ps);
+  read_bucket<P>(bucket_ptr(t, c), ps);
   uint8_t result = R_PENDING;
   uint32_t backoff = 1;
   while (result == R_PENDING) {
@@ -548,20 +579,20 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
   {
     unsigned long long s[kSlots];
-    load_bucket(bucket_ptr(t, b[0]), s);
+    read_bucket<P>(bucket_ptr(t, b[0]), s);
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
   }
-  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
   if (hint & kOverflowBit) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
       unsigned long long s[kSlots];
-      load_bucket(bucket_ptr(t, b[i]), s);
+      read_bucket<P>(bucket_ptr(t, b[i]), s);
       const uint32_t m = match_mask(s, key);
       if (m) {
         val = kv_val(s[__builtin_ctz(m)]);
@@ -570,11 +601,11 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
     }
   }
   if (hint & kStashBit) {
-    uint32_t cnt = t.stash_count[b[0]];
+    uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
     if (cnt > kStashSlots) cnt = kStashSlots;
     const unsigned long long* sb = stash_ptr(t, b[0]);
     for (uint32_t j = 0; j < cnt; ++j) {
-      const unsigned long long cur = sb[j];
+      const unsigned long long cur = read_u64<P>(sb + j);
       if (cur != 0ull && kv_key(cur) == key) {
         val = kv_val(cur);
         return R_FOUND;
@@ -588,17 +619,35 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
 // include/hive_hash_delete.cuh:168-194: lowest match in every candidate bucket
 // (the others only if H0's overflow bit is set), then the canonical stash
 // (count unchanged).  removed_tab / removed_stash receive the copies removed.
-__device__ __forceinline__ int erase_in(unsigned long long* bk, int nslots, uint32_t key) {
-  for (int j = 0; j < nslots; ++j) {
-    unsigned long long cur = bk[j];
-    if (cur == 0ull || kv_key(cur) != key) continue;
-    for (int spin = 0; spin < 64; ++spin) {
-      const unsigned long long seen = cas64(bk + j, cur, 0ull);
-      if (seen == cur) return 1;
-      if (seen == 0ull || kv_key(seen) != key) return 0;
-      cur = seen;
-    }
-    return 0;
+// remove the copy of key held in slot p (read as cur); a concurrent change to
+// another key, or to empty, means there is nothing left to remove here
+__device__ __forceinline__ int erase_slot(unsigned long long* p, unsigned long long cur, uint32_t key) {
+  for (int spin = 0; spin < 64; ++spin) {
+    const unsigned long long seen = cas64(p, cur, 0ull);
+    if (seen == cur) return 1;
+    if (seen == 0ull || kv_key(seen) != key) return 0;
+    cur = seen;
+  }
+  return 0;
+}
+
+// lowest matching slot of a bucket: one 64-byte read, then the CAS
+template <class P>
+__device__ __forceinline__ int erase_bucket(unsigned long long* bk, uint32_t key) {
+  unsigned long long s[kSlots];
+  read_bucket<P>(bk, s);
+  const uint32_t m = match_mask(s, key);
+  if (!m) return 0;
+  const int j = __builtin_ctz(m);
+  return erase_slot(bk + j, s[j], key);
+}
+
+// lowest matching entry among the first cnt stash entries
+template <class P>
+__device__ __forceinline__ int erase_stash(unsigned long long* sb, uint32_t cnt, uint32_t key) {
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const unsigned long long cur = read_u64<P>(sb + j);
+    if (cur != 0ull && kv_key(cur) == key) return erase_slot(sb + j, cur, key);
   }
   return 0;
 }
@@ -611,16 +660,16 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
-  removed_tab += erase_in(bucket_ptr(t, b[0]), kSlots, key);
+  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+  removed_tab += erase_bucket<P>(bucket_ptr(t, b[0]), key);
   if (hint & kOverflowBit) {
 #pragma unroll
-    for (int i = 1; i < P::NH; ++i) removed_tab += erase_in(bucket_ptr(t, b[i]), kSlots, key);
+    for (int i = 1; i < P::NH; ++i) removed_tab += erase_bucket<P>(bucket_ptr(t, b[i]), key);
   }
   if (hint & kStashBit) {
-    uint32_t cnt = t.stash_count[b[0]];
+    uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
     if (cnt > kStashSlots) cnt = kStashSlots;
-    removed_stash = erase_in(stash_ptr(t, b[0]), static_cast<int>(cnt), key);
+    removed_stash = erase_stash<P>(stash_ptr(t, b[0]), cnt, key);
   }
   return (removed_tab + removed_stash) ? R_ERASED : R_NOT_FOUND;
 }

@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
     uint32_t b[P::NH];
 #pragma unroll
     for (int j = 0; j < P::NH; ++j) b[j] = bucket_of<P>(j, key, t.nb);
-    const uint32_t hint = meta_bits(meta_word(t, b[0]), b[0]);
+    const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
     v[0] = 1;
     v[1] = 1;
     unsigned long long s[kSlots];

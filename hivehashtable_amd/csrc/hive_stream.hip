@@ -90,15 +90,18 @@ __device__ bool wait_phase(const StreamArgs& a, uint32_t r) {
 // tickets are handed out in phase order, so the lowest unfinished tile always
 // has an owner whose earlier tile is done and whose dependencies (all lower
 // tickets) are complete: every wait ends.  A completed phase q-1 implies
-// every earlier phase completed, so a block waits (and acquires: buffer_inv
-// sc1 drops L1/L2 lines that other XCDs' writes made stale -- no kernel
-// boundary separates phases) only when a ticket needs a later phase than it
-// has seen complete.  The first chunk of a tile's inputs (read-only) is
-// loaded before the wait.
-template <class P, bool Inline>
+// every earlier phase completed, so a block waits only when a ticket needs a
+// later phase than it has seen complete.  No kernel boundary separates phases,
+// so every table read is a device-coherent load (CoherentReads): no L1/L2 line
+// that another XCD's writes made stale is ever read, and no cache invalidation
+// is needed between phases.  Every finished tile's writes are performed
+// (s_waitcnt vmcnt(0)) before its phase counter drops.  The first chunk of a
+// tile's inputs (read-only) is loaded before the wait.
+template <class P0, bool Inline>
 __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs a) {
+  using P = CoherentReads<P0>;
   __shared__ uint4 s_desc;
-  __shared__ uint32_t s_abort, s_fresh, qn;
+  __shared__ uint32_t s_abort, qn;
   extern __shared__ uint32_t queue[];  // staged slow path: one entry per op of the largest tile
   int dtab = 0, dstash = 0, dslow = 0;
   int known = -1;  // thread 0: highest phase this block has seen complete
@@ -125,11 +128,10 @@ __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs
     uint32_t val = (i < hi && a.vals) ? a.vals[i] : key + 1u;
     if (threadIdx.x == 0) {
       next_k = atomicAdd(a.ticket, 1u);
-      uint32_t abort = 0, fresh = 0;
+      uint32_t abort = 0;
       if (static_cast<int>(q) - 1 > known) {
         if (wait_phase(a, q - 1)) {
           known = static_cast<int>(q) - 1;
-          fresh = 1;
         } else {
           atomicOr(a.error, 1u);
           abort = 1;
@@ -137,11 +139,9 @@ __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs
       }
       if (next_k < a.ntiles) next_d = a.tile_desc[next_k];
       s_abort = abort;
-      s_fresh = fresh;
     }
     __syncthreads();
     if (s_abort) break;
-    if (s_fresh) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     for (size_t base = lo;;) {
       if (p == 0) {
         if (ty == OP_LOOKUP) {
