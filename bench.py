@@ -49,6 +49,10 @@ def parse():
     p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
     p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
     p.add_argument("--mixed-batches", type=int, default=1000)
+    p.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    p.add_argument("--serve-requests", type=int, default=1000)
+    p.add_argument("--serve-offered-mops", type=float, default=4000.0,
+                   help="offered load of the open-loop serving runs, million ops per second")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (reference point for the HBM byte counters)")
     return p.parse_args()
@@ -156,6 +160,68 @@ def mixed_leg(a, H, W, torch, dev):
         log(f"mixed {names[mode]}: {out['modes'][names[mode]]}")
     out["modes_agree"] = agree
     t.close()
+    return out
+
+
+def serve_leg(a, H, W, torch, dev):
+    """Requests served with continuous batching (hive_serve_trace): a table
+    preloaded with 2^26 distinct keys; request sizes log-uniform in
+    [2^8, 2^18] ops; Poisson arrivals at the offered rate, then the same
+    requests all at once (saturation); keys Zipfian (theta 0.99).  Mixes A
+    (50% reads / 50% updates), B (95/5), C (reads) on 2^27 slots; D (95%
+    reads of the latest keys / 5% new inserts) on a table sized just under
+    LF 0.9, so that admission resizes it in place."""
+    import math
+    n_pre = 1 << 26
+    t0 = time.time()
+    pre = W.distinct_keys(n_pre, a.seed + 5000)
+    pre_v = W.values(n_pre, a.seed + 5001)
+    d_pre = torch.from_numpy(pre.view(np.int32)).to(dev)
+    d_pre_v = torch.from_numpy(pre_v.view(np.int32)).to(dev)
+    fresh = W.distinct_keys(int(a.serve_requests * (1 << 18) * 0.06) + 1024, a.seed + 5000, start=n_pre + 2)
+    log(f"serve: preload {n_pre} keys, {fresh.size} fresh keys in {time.time() - t0:.1f}s")
+    out = {"preload_keys": n_pre, "requests": a.serve_requests, "request_ops": [256, 262144], "zipf_theta": 0.99,
+           "offered_mops": a.serve_offered_mops, "mixes": {}}
+
+    def run(t, ty, k, v, off, arr):
+        log(f"serve run: {off.size - 1} requests, {int(off[-1])} ops, last arrival {arr[-1] / 1e3:.2f} ms")
+        d_ty = torch.from_numpy(ty).to(dev)
+        d_k = torch.from_numpy(k.view(np.int32)).to(dev)
+        d_v = torch.from_numpy(v.view(np.int32)).to(dev)
+        torch.cuda.synchronize()
+        gv, gs, lat, rep = t.serve(d_ty, d_k, d_v, off, arr)
+        st = gs.cpu().numpy()
+        reads = ty == 1
+        res = {"p50_us": round(rep["p50_us"], 2), "p99_us": round(rep["p99_us"], 2), "max_us": round(rep["max_us"], 2),
+               "mops": round(rep["mops"], 2), "span_ms": round(rep["span_ms"], 3), "batches": rep["batches"],
+               "mean_batch_ops": round(rep["mean_batch_ops"], 1), "resizes": rep["resizes"],
+               "resize_ms": round(rep["resize_ms"], 3), "rejected_requests": rep["rejected_requests"],
+               "final_slots": rep["final_slots"],
+               "reads_found": round(float((st[reads] == 4).mean()) if reads.any() else 1.0, 6),
+               "writes_applied": round(float(np.isin(st[~reads], (1, 2, 3)).mean()) if (~reads).any() else 1.0, 6)}
+        return res
+
+    t = H.HiveTable(1 << 27, a.policy)
+    t.insert(d_pre, d_pre_v)
+    for mix in "ABC":
+        ty, k, v, off, arr = W.serving_trace(mix, pre, a.serve_requests, a.serve_offered_mops, a.seed + 5100 + ord(mix))
+        res = {"ops": int(off[-1]), "open": run(t, ty, k, v, off, arr),
+               "saturated": run(t, ty, k, v, off, np.zeros_like(arr))}
+        out["mixes"][mix] = res
+        log(f"serve {mix}: {res}")
+    t.close()
+    ty, k, v, off, arr = W.serving_trace("D", pre, a.serve_requests, a.serve_offered_mops, a.seed + 5100 + ord("D"),
+                                         fresh=fresh)
+    res = {"ops": int(off[-1])}
+    slots_d = int(math.ceil(n_pre / 0.899 / 8)) * 8
+    for name, arrivals in (("open", arr), ("saturated", np.zeros_like(arr))):
+        t = H.HiveTable(slots_d, a.policy)
+        t.insert(d_pre, d_pre_v)
+        res["initial_slots"] = slots_d
+        res[name] = run(t, ty, k, v, off, arrivals)
+        t.close()
+    out["mixes"]["D"] = res
+    log(f"serve D: {res}")
     return out
 
 
@@ -318,6 +384,10 @@ def main():
     if not a.no_mixed:
         mixed = mixed_leg(a, H, W, torch, dev)
 
+    serving = None
+    if not a.no_serve:
+        serving = serve_leg(a, H, W, torch, dev)
+
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -371,6 +441,7 @@ def main():
             "footprint_bytes": nb * 64,
         },
         "mixed_stream": mixed,
+        "serving": serving,
         "cpu_baseline": cpu,
         "git": git_sha(),
     }

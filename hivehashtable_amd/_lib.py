@@ -52,8 +52,21 @@ class Launch(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("inline_slow_path", C.c_uint32), ("tile_ops", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int), ("launches_ms", C.c_double), ("graph_ms", C.c_double),
-                ("persistent_ms", C.c_double), ("candidates", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int),
+                ("launches_ms", C.c_double), ("graph_ms", C.c_double), ("persistent_ms", C.c_double),
+                ("candidates", C.c_uint32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("max_batch_ops", C.c_uint64), ("max_inflight", C.c_uint32), ("max_load_factor", C.c_double),
+                ("growth", C.c_double), ("hbm_reserve_bytes", C.c_uint64), ("stash_headroom", C.c_double)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("ops", C.c_uint64),
+                ("rejected_requests", C.c_uint64), ("resizes", C.c_uint64), ("final_slots", C.c_uint64),
+                ("span_ms", C.c_double), ("mops", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double),
+                ("max_us", C.c_double), ("mean_batch_ops", C.c_double), ("resize_ms", C.c_double)]
 
 
 class Operation(C.Structure):
@@ -106,6 +119,7 @@ def lib():
         "hive_mixed_stream": [p, p, p, p, p, sz, p, p, i32, p, C.POINTER(C.c_double), p],
         "hive_set_stream_shape": [p, u32, u32, u32],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
+        "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
         "hive_ceiling_gather": [p, u64, u64, u32, p, p],
         "hive_ceiling_cas": [p, u64, u64, u32, p],
     }

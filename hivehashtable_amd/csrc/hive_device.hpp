@@ -217,6 +217,16 @@ __device__ __forceinline__ unsigned long long cas64(unsigned long long* p, unsig
   return expect;
 }
 
+// Update a slot read as `cur` (holding kv's key) to kv.  When the CAS loses to
+// a concurrent update of the same key, this update is ordered just before
+// that one -- overwritten, but applied -- so it succeeds without a retry: hot
+// keys updated by thousands of lanes in one batch cost one CAS each.  False
+// if the slot no longer holds the key (a cuckoo move replaced it).
+__device__ __forceinline__ bool update_slot(unsigned long long* p, unsigned long long cur, unsigned long long kv) {
+  const unsigned long long seen = cas64(p, cur, kv);
+  return seen == cur || kv_key(seen) == kv_key(kv);
+}
+
 // all of this lane's outstanding memory operations have completed
 __device__ __forceinline__ void wait_vmem() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -344,7 +354,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
     const uint32_t m = match_mask(s0, key);
     if (m) {
       const int j = __builtin_ctz(m);
-      return cas64(bucket_ptr(t, b[0]) + j, s0[j], kv) == s0[j] ? R_UPDATED : R_PENDING;
+      return update_slot(bucket_ptr(t, b[0]) + j, s0[j], kv) ? R_UPDATED : R_PENDING;
     }
   }
   if (hint & kOverflowBit) {
@@ -356,7 +366,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
       const uint32_t m = match_mask(s, key);
       if (m) {
         const int j = __builtin_ctz(m);
-        return cas64(bucket_ptr(t, b[i]) + j, s[j], kv) == s[j] ? R_UPDATED : R_PENDING;
+        return update_slot(bucket_ptr(t, b[i]) + j, s[j], kv) ? R_UPDATED : R_PENDING;
       }
     }
   }
@@ -367,7 +377,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
     const int j = __builtin_ctz(e);
     const unsigned long long seen = cas64(cb + j, 0ull, kv);
     if (seen == 0ull) return R_INSERTED;
-    if (kv_key(seen) == key) return R_PENDING;  // same key landed first: update under the lock
+    if (kv_key(seen) == key) return update_slot(cb + j, seen, kv) ? R_UPDATED : R_PENDING;  // it landed first
     e &= e - 1;
   }
   return R_PENDING;
@@ -382,14 +392,7 @@ __device__ __forceinline__ int locked_update(unsigned long long* bk, int nslots,
   uint32_t m = match_mask(s, key) & ((1u << nslots) - 1u);
   if (m) {
     const int j = __builtin_ctz(m);
-    unsigned long long cur = s[j];
-    for (int spin = 0; spin < 64; ++spin) {
-      const unsigned long long seen = cas64(bk + j, cur, kv);
-      if (seen == cur) return 1;
-      if (seen == 0ull || kv_key(seen) != key) return -1;
-      cur = seen;
-    }
-    return -1;
+    return update_slot(bk + j, s[j], kv) ? 1 : -1;
   }
   return 0;
 }
@@ -454,7 +457,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       const uint32_t m = match_mask(cs, key);
       if (m) {
         const int j = __builtin_ctz(m);
-        if (cas64(cb + j, cs[j], kv) == cs[j]) return R_UPDATED;
+        if (update_slot(cb + j, cs[j], kv)) return R_UPDATED;
         continue;
       }
     }

@@ -111,6 +111,24 @@ hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_
   return launch_insert(t, src, n, out, OP_INSERT, s, clk);
 }
 
+// device buffers freed together when a call returns
+struct DevArena {
+  std::vector<void*> ptrs;
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    if (e == hipSuccess) {
+      ptrs.push_back(q);
+      *p = static_cast<T*>(q);
+    }
+    return e;
+  }
+};
+
 inline void free_view(TableView& v) {
   if (v.slots) (void)hipFree(v.slots);
   if (v.stash) (void)hipFree(v.stash);

@@ -1,0 +1,259 @@
+// hive_serve.hip -- continuous batching of client requests over one resident
+// table: admission (in-place resize by free HBM, stash headroom), coalescing
+// of arrived requests into mixed batches, completion tracking per request.
+//
+// Reference analogue: the YCSB driver (src/hashTableBenchmark.cc:434-570,
+// src/hive_dispatch.cu:317-549) runs one pre-built workload per call on a
+// table created for that call; here the table stays resident and requests
+// are served as they arrive.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <vector>
+
+#include "hive_internal.hpp"
+
+using namespace hive;
+
+namespace {
+
+// inserts of each request: the admission test needs an upper bound of new keys
+__global__ void k_req_inserts(const uint8_t* types, const unsigned long long* off, uint32_t* counts) {
+  __shared__ uint32_t total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  const unsigned long long lo = off[blockIdx.x], hi = off[blockIdx.x + 1];
+  uint32_t c = 0;
+  for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) c += types[i] == OP_INSERT;
+  if (c) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// a rejected request's ops: status FAILED, value 0, table untouched
+__global__ void k_reject(uint8_t* status, uint32_t* vals, size_t lo, size_t hi) {
+  const size_t i = lo + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  if (status) status[i] = R_FAILED;
+  if (vals) vals[i] = 0;
+}
+
+using Clock = std::chrono::steady_clock;
+
+double us_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::micro>(Clock::now() - t0).count();
+}
+
+struct Batch {
+  size_t first, last;  // requests [first, last)
+  hipEvent_t ev;
+};
+
+double percentile(std::vector<double> v, double p) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const double pos = p / 100.0 * static_cast<double>(v.size() - 1);
+  const size_t lo = static_cast<size_t>(pos);
+  const size_t hi = std::min(lo + 1, v.size() - 1);
+  return v[lo] + (v[hi] - v[lo]) * (pos - static_cast<double>(lo));
+}
+
+struct EventPool {
+  std::vector<hipEvent_t> all, free;
+  ~EventPool() {
+    for (hipEvent_t e : all) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
+                                          const uint32_t* vals, const uint64_t* h_req_offsets,
+                                          const double* h_arrival_us, size_t nreq, uint32_t* out_vals,
+                                          uint8_t* status, const hive_serve_config_t* cfg, double* h_latency_us,
+                                          hive_serve_report_t* report, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (!h_req_offsets || (nreq && !h_arrival_us)) return fail(HIVE_EINVAL, "null request offsets or arrivals");
+  if (h_req_offsets[0] != 0) return fail(HIVE_EINVAL, "request offsets[0] must be 0");
+  for (size_t i = 0; i < nreq; ++i) {
+    if (h_req_offsets[i + 1] < h_req_offsets[i]) return fail(HIVE_EINVAL, "request offsets must be non-decreasing");
+    if (!(h_arrival_us[i] >= 0.0) || !std::isfinite(h_arrival_us[i]) || (i && h_arrival_us[i] < h_arrival_us[i - 1]))
+      return fail(HIVE_EINVAL, "arrivals must be finite, >= 0 and non-decreasing");
+  }
+  const uint64_t nops = h_req_offsets[nreq];
+  if (nops && (!keys || !types)) return fail(HIVE_EINVAL, "null keys or types");
+  hive_serve_config_t c = cfg ? *cfg : hive_serve_config_t{};
+  if (c.max_batch_ops == 0) c.max_batch_ops = 1ull << 22;
+  if (c.max_inflight == 0) c.max_inflight = 2;
+  if (c.max_load_factor == 0.0) c.max_load_factor = 0.9;
+  if (c.growth == 0.0) c.growth = 2.0;
+  if (c.hbm_reserve_bytes == 0) c.hbm_reserve_bytes = 1ull << 30;
+  if (c.stash_headroom == 0.0) c.stash_headroom = 0.5;
+  if (c.max_inflight > 8 || !(c.max_load_factor > 0.0 && c.max_load_factor <= 1.0) || !(c.growth > 1.0) ||
+      !(c.stash_headroom > 0.0 && c.stash_headroom <= 1.0))
+    return fail(HIVE_EINVAL, "serve config out of range");
+  hive_serve_report_t rep{};
+  rep.requests = nreq;
+  if (nreq == 0) {
+    rep.final_slots = static_cast<uint64_t>(t->v.nb) * kSlots;
+    if (report) *report = rep;
+    return HIVE_OK;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+
+  // ---- set-up, before the trace clock starts
+  std::vector<uint32_t> h_ins(nreq);
+  {
+    DevArena ar;
+    unsigned long long* d_off = nullptr;
+    uint32_t* d_cnt = nullptr;
+    HIP_TRY(ar.alloc(&d_off, nreq + 1));
+    HIP_TRY(ar.alloc(&d_cnt, nreq));
+    HIP_TRY(hipMemcpyAsync(d_off, h_req_offsets, (nreq + 1) * 8, hipMemcpyHostToDevice, s));
+    for (size_t lo = 0; lo < nreq; lo += 0x7FFFFFFF) {  // grid.x limit
+      const size_t cnt = std::min<size_t>(nreq - lo, 0x7FFFFFFF);
+      if (nops) k_req_inserts<<<static_cast<unsigned>(cnt), 256, 0, s>>>(types, d_off + lo, d_cnt + lo);
+      else HIP_TRY(hipMemsetAsync(d_cnt + lo, 0, cnt * 4, s));
+    }
+    HIP_CHECK_LAUNCH();
+    HIP_TRY(hipMemcpyAsync(h_ins.data(), d_cnt, nreq * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  hive_stats_t st{};
+  if ((r = hive_stats(t, &st, stream)) != HIVE_OK) return r;
+  uint64_t capacity = st.capacity;
+  uint64_t table_ub = st.table_live;  // upper bound of table entries
+  uint64_t stash_ub = st.stash_live;  // upper bound of stash entries
+  EventPool pool;
+  for (uint32_t i = 0; i < c.max_inflight; ++i) {
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    pool.all.push_back(e);
+    pool.free.push_back(e);
+  }
+  std::vector<double> lat(nreq, 0.0);
+  std::vector<uint8_t> rejected(nreq, 0);
+  std::deque<Batch> q;
+  size_t arrived = 0, launched = 0, completed = 0;
+  double last_done = 0.0;
+  const Clock::time_point t0 = Clock::now();
+
+  // retire finished batches in order; block = wait for every queued batch
+  auto retire = [&](bool block) -> hive_status_t {
+    while (!q.empty()) {
+      hipError_t e = block ? hipEventSynchronize(q.front().ev) : hipEventQuery(q.front().ev);
+      if (e == hipErrorNotReady) return HIVE_OK;
+      if (e != hipSuccess) return fail(HIVE_EDEVICE, "batch completion");
+      const double done = us_since(t0);
+      for (size_t i = q.front().first; i < q.front().last; ++i) lat[i] = done - h_arrival_us[i];
+      completed += q.front().last - q.front().first;
+      last_done = done;
+      pool.free.push_back(q.front().ev);
+      q.pop_front();
+    }
+    return HIVE_OK;
+  };
+  auto refresh = [&]() -> hive_status_t {  // exact counts; the queue is empty
+    hive_stats_t x{};
+    hive_status_t rr = hive_stats(t, &x, stream);
+    if (rr != HIVE_OK) return rr;
+    capacity = x.capacity;
+    table_ub = x.table_live;
+    stash_ub = x.stash_live;
+    return HIVE_OK;
+  };
+  auto fits = [&](uint64_t ins) {
+    return static_cast<double>(table_ub + ins) <= c.max_load_factor * static_cast<double>(capacity);
+  };
+
+  while (completed < nreq) {
+    const double now = us_since(t0);
+    while (arrived < nreq && h_arrival_us[arrived] <= now) ++arrived;
+    if ((r = retire(false)) != HIVE_OK) return r;
+    if (launched < arrived && q.size() < c.max_inflight) {
+      // coalesce every arrived request, up to max_batch_ops (at least one request)
+      size_t end = launched + 1;
+      uint64_t ins = h_ins[launched];
+      while (end < arrived && h_req_offsets[end + 1] - h_req_offsets[launched] <= c.max_batch_ops) ins += h_ins[end++];
+      if (ins && !fits(ins)) {
+        if ((r = retire(true)) != HIVE_OK || (r = refresh()) != HIVE_OK) return r;
+        if (!fits(ins)) {
+          // grow in place if free HBM allows it
+          double want = static_cast<double>(capacity) * c.growth;
+          while (static_cast<double>(table_ub + ins) > c.max_load_factor * want) want *= c.growth;
+          const uint64_t new_nb = static_cast<uint64_t>(std::ceil(want / kSlots));
+          size_t free_b = 0, total_b = 0;
+          HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+          if (new_nb <= 0xFFFFFFFFull &&
+              free_b >= view_bytes(static_cast<uint32_t>(new_nb)) + c.hbm_reserve_bytes) {
+            const Clock::time_point tr = Clock::now();
+            if ((r = hive_resize(t, new_nb * kSlots, stream)) != HIVE_OK) return r;
+            if ((r = refresh()) != HIVE_OK) return r;
+            rep.resize_ms += std::chrono::duration<double, std::milli>(Clock::now() - tr).count();
+            ++rep.resizes;
+          }
+        }
+        if (!fits(ins)) {
+          // no room to grow: keep the longest prefix whose inserts still fit,
+          // or that the stash can absorb
+          const uint64_t stash_cap = static_cast<uint64_t>(t->v.nb) * kStashSlots;
+          const double stash_room = c.stash_headroom * static_cast<double>(stash_cap);
+          size_t keep = launched;
+          uint64_t kins = 0;
+          while (keep < end && (fits(kins + h_ins[keep]) ||
+                                static_cast<double>(stash_ub + kins + h_ins[keep]) < stash_room)) {
+            kins += h_ins[keep++];
+          }
+          if (keep == launched) {  // the first request cannot be admitted: reject it
+            const size_t lo = h_req_offsets[launched], hi = h_req_offsets[launched + 1];
+            if (hi > lo) {
+              k_reject<<<static_cast<unsigned>((hi - lo + 255) / 256), 256, 0, s>>>(status, out_vals, lo, hi);
+              HIP_CHECK_LAUNCH();
+            }
+            rejected[launched] = 1;
+            lat[launched] = us_since(t0) - h_arrival_us[launched];
+            ++rep.rejected_requests;
+            ++completed;
+            ++launched;
+            continue;
+          }
+          if (!fits(kins)) stash_ub += kins;  // these inserts may all land in the stash
+          end = keep;
+          ins = kins;
+        }
+      }
+      const size_t o = h_req_offsets[launched], n = h_req_offsets[end] - o;
+      SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
+      OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
+      if ((r = launch_mixed(t, src, n, out, s)) != HIVE_OK) return r;
+      hipEvent_t ev = pool.free.back();
+      pool.free.pop_back();
+      HIP_TRY(hipEventRecord(ev, s));
+      q.push_back(Batch{launched, end, ev});
+      table_ub += ins;
+      ++rep.batches;
+      rep.ops += n;
+      launched = end;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if ((r = refresh()) != HIVE_OK) return r;
+  std::vector<double> served;
+  served.reserve(nreq);
+  for (size_t i = 0; i < nreq; ++i)
+    if (!rejected[i]) served.push_back(lat[i]);
+  rep.final_slots = capacity;
+  rep.span_ms = (last_done - h_arrival_us[0]) * 1e-3;
+  rep.mops = rep.span_ms > 0.0 ? static_cast<double>(rep.ops) / (rep.span_ms * 1e3) : 0.0;
+  rep.p50_us = percentile(served, 50.0);
+  rep.p99_us = percentile(served, 99.0);
+  rep.max_us = served.empty() ? 0.0 : *std::max_element(served.begin(), served.end());
+  rep.mean_batch_ops = rep.batches ? static_cast<double>(rep.ops) / static_cast<double>(rep.batches) : 0.0;
+  if (h_latency_us) std::copy(lat.begin(), lat.end(), h_latency_us);
+  if (report) *report = rep;
+  return HIVE_OK;
+}

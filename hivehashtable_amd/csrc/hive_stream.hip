@@ -217,23 +217,6 @@ __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs
   wave_count(t, dtab, dstash, dslow);
 }
 
-struct DevArena {
-  std::vector<void*> ptrs;
-  ~DevArena() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <class T>
-  hipError_t alloc(T** p, size_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
-    if (e == hipSuccess) {
-      ptrs.push_back(q);
-      *p = static_cast<T*>(q);
-    }
-    return e;
-  }
-};
-
 hive_status_t enqueue_launches(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                const uint64_t* off, size_t nb, uint32_t* out_vals, uint8_t* status,
                                unsigned int* ctrs, unsigned long long* ts, hipStream_t s) {

@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (Launch, Operation, Stats, Tune, POLICIES, check, lib, OP_INSERT, OP_LOOKUP, OP_DELETE)
+from ._lib import (Launch, Operation, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib, OP_INSERT,
+                   OP_LOOKUP, OP_DELETE)
 
 
 def _torch():
@@ -154,6 +155,35 @@ class HiveTable:
         check(self.L.hive_autotune_stream(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
                                           off.size - 1, C.byref(res), _stream_ptr(stream)), "hive_autotune_stream")
         return {f: getattr(res, f) for f, _ in Tune._fields_}
+
+    def serve(self, types, keys, vals, req_offsets, arrival_us, max_batch_ops=0, max_inflight=0, max_load_factor=0.0,
+              growth=0.0, hbm_reserve_bytes=0, stash_headroom=0.0, out_vals=None, status=None, stream=None):
+        """Replay a request trace with continuous batching (hive_serve_trace).
+
+        Request r = ops[req_offsets[r]:req_offsets[r+1]], arriving arrival_us[r]
+        after the call starts.  Returns (out_vals, status, latency_us, report)."""
+        torch = _torch()
+        ty = _dev(types, torch.uint8)
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        off = np.ascontiguousarray(req_offsets, np.uint64)
+        arr = np.ascontiguousarray(arrival_us, np.float64)
+        nreq = off.size - 1
+        if arr.size != nreq:
+            raise ValueError("one arrival time per request")
+        if out_vals is None:
+            out_vals = torch.empty(k.numel(), dtype=torch.int32, device=k.device)
+        if status is None:
+            status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
+        cfg = ServeConfig(int(max_batch_ops), int(max_inflight), float(max_load_factor), float(growth),
+                          int(hbm_reserve_bytes), float(stash_headroom))
+        lat = np.zeros(max(nreq, 1), np.float64)
+        rep = ServeReport()
+        check(self.L.hive_serve_trace(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
+                                      arr.ctypes.data_as(C.c_void_p), nreq, _ptr(out_vals), _ptr(status),
+                                      C.byref(cfg), lat.ctypes.data_as(C.c_void_p), C.byref(rep),
+                                      _stream_ptr(stream)), "hive_serve_trace")
+        return out_vals, status, lat[:nreq], {f: getattr(rep, f) for f, _ in ServeReport._fields_}
 
     def set_stream_shape(self, tile_ops=0, blocks_per_cu=0, inline_slow_path=0):
         check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu), int(inline_slow_path)),

@@ -10,6 +10,7 @@ import numpy as np
 
 KEY_LO = 1
 KEY_HI = 0xFFFFFFFE  # inclusive
+OP_INSERT_CODE, OP_LOOKUP_CODE, OP_DELETE_CODE = 0, 1, 2  # Operation.type (include/utils.h:10-16)
 
 LF_POINTS = (0.65, 0.80, 0.90, 0.95, 1.0)
 
@@ -47,8 +48,10 @@ def distinct_keys(n, seed, exclude=None, start=0, bits=32):
     """n distinct u32 keys, a seeded pseudo-random permutation prefix
     (sampling without replacement): in [1, 2^32-2] for bits=32, in
     [1, 2^bits] for bits < 32.  Keys in `exclude` are skipped (small sets
-    only); `start` continues the same stream, so distinct_keys(m, seed,
-    start=n) never repeats distinct_keys(n, seed)."""
+    only); `start` continues the same stream.  With bits=32 the stream holds
+    0 and 0xFFFFFFFF once each, so the first n keys come from positions below
+    n + 2 and distinct_keys(m, seed, start=n + 2) never repeats
+    distinct_keys(n, seed)."""
     if bits < 32:
         if start + n > (1 << bits):
             raise ValueError("key space exhausted")
@@ -180,3 +183,59 @@ def zipf_sampler(n_items, theta, seed):
         return np.clip(r, 0, n_items - 1)
 
     return sample
+
+
+# cfg5 request mixes: fractions of (read, update of a preloaded key, insert of
+# a new key).  A/B/C follow YCSB workloads A, B, C (Zipfian over the preloaded
+# keys); D is YCSB D (reads skewed to the latest inserted keys, new inserts).
+SERVE_MIXES = {"A": (0.5, 0.5, 0.0), "B": (0.95, 0.05, 0.0), "C": (1.0, 0.0, 0.0), "D": (0.95, 0.0, 0.05)}
+
+
+def serving_trace(mix, preload, n_requests, offered_mops, seed, theta=0.99, size_lo=1 << 8, size_hi=1 << 18,
+                  fresh=None):
+    """Client requests for the serving loop.
+
+    Request sizes are log-uniform in [size_lo, size_hi]; arrivals are a
+    Poisson process whose rate offers `offered_mops` million ops per second
+    on average (0: every request arrives at time 0, a closed-loop saturation
+    run).  Reads are LOOKUP ops, updates and new keys are INSERT ops.  Keys of
+    mixes A-C are Zipfian (theta) over `preload` (rank i = preload[i]); mix D
+    takes new keys from `fresh` in order and draws reads Zipfian over the keys
+    inserted before the request, most recent first.
+    Returns (types u8, keys u32, vals u32, req_offsets u64, arrival_us f64)."""
+    pr, pu, _ = SERVE_MIXES[mix]
+    rng = np.random.default_rng(seed)
+    sizes = log_uniform_sizes(n_requests, size_lo, size_hi, seed + 1)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(off[-1])
+    if offered_mops and offered_mops > 0:
+        rate = offered_mops * 1e6 / sizes.mean()  # requests per second
+        gaps = rng.exponential(1e6 / rate, n_requests)
+        arrivals = np.cumsum(gaps) - gaps[0]
+    else:
+        arrivals = np.zeros(n_requests)
+    u = rng.random(total)
+    kind = (u >= pr).astype(np.int8) + (u >= pr + pu).astype(np.int8)  # 0 read, 1 update, 2 new
+    types = np.where(kind == 0, OP_LOOKUP_CODE, OP_INSERT_CODE).astype(np.uint8)
+    vals = rng.integers(0, 2**32, size=total, dtype=np.uint64).astype(np.uint32)
+    pre = np.asarray(preload, np.uint32)
+    keys = np.empty(total, np.uint32)
+    if mix != "D":
+        z = zipf_sampler(pre.size, theta, seed + 2)
+        keys[:] = pre[z(total)]
+        return types, keys, vals, off, arrivals
+    new_pos = np.flatnonzero(kind == 2)
+    if fresh is None or len(fresh) < new_pos.size:
+        raise ValueError(f"mix D needs {new_pos.size} fresh keys")
+    fresh = np.asarray(fresh, np.uint32)[: new_pos.size]
+    keys[new_pos] = fresh
+    req_of = np.repeat(np.arange(n_requests), sizes)
+    new_per_req = np.bincount(req_of[new_pos], minlength=n_requests)
+    new_before = np.concatenate([[0], np.cumsum(new_per_req)[:-1]])
+    read_pos = np.flatnonzero(kind != 2)
+    known = pre.size + new_before[req_of[read_pos]]
+    z = zipf_sampler(pre.size + fresh.size, theta, seed + 2)
+    rank = np.minimum(z(read_pos.size), known - 1)
+    pool = np.concatenate([pre, fresh])
+    keys[read_pos] = pool[known - 1 - rank]
+    return types, keys, vals, off, arrivals

@@ -180,6 +180,52 @@ typedef struct {
 hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                    const uint64_t* h_offsets, size_t nbatches, hive_tune_t* out, void* stream);
 
+/* ---- serving: continuous batching over one resident table
+ * Replaces the reference's YCSB driver (src/hashTableBenchmark.cc:434-570,
+ * src/hive_dispatch.cu:317-549), which runs a pre-built workload as one
+ * batch per call.  hive_serve_trace replays a trace of client requests in
+ * real time: request r owns ops [h_req_offsets[r], h_req_offsets[r+1]) of
+ * the device arrays and arrives h_arrival_us[r] microseconds after the call
+ * starts (non-decreasing).  Whenever fewer than max_inflight batches are
+ * queued on the stream, every request that has arrived and is not yet
+ * launched (up to max_batch_ops) is coalesced into one mixed batch with
+ * hive_mixed semantics (lookups, then deletes, then inserts).  Requests
+ * whose keys do not meet other requests of the same batch get exactly the
+ * results they would get alone; ops of different requests on one key in one
+ * batch follow the batch's phase order.
+ * Admission: before a batch whose inserts could lift the load factor above
+ * max_load_factor, the table is resized in place by `growth` if free HBM
+ * keeps hbm_reserve_bytes after the new table; if not, requests that insert
+ * are admitted while stash use stays below stash_headroom of the stash
+ * capacity, and rejected otherwise (their ops get HIVE_R_FAILED, unexecuted).
+ * h_latency_us[r] (host, may be NULL) receives arrival-to-completion time
+ * (completion = the host observing the batch's event).  Synchronizes. */
+typedef struct {
+  uint64_t max_batch_ops;     /* ops coalesced into one batch at most (0 = 2^22) */
+  uint32_t max_inflight;      /* batches queued on the stream at once, 1..8 (0 = 2) */
+  double max_load_factor;     /* resize before a batch that could exceed it (0 = 0.9) */
+  double growth;              /* capacity multiplier of one resize, > 1 (0 = 2.0) */
+  uint64_t hbm_reserve_bytes; /* free HBM a resize must leave (0 = 1 GiB) */
+  double stash_headroom;      /* admit inserts without a resize while stash use < this (0 = 0.5) */
+} hive_serve_config_t;
+typedef struct {
+  uint64_t requests;
+  uint64_t batches;
+  uint64_t ops;               /* ops executed (rejected requests excluded) */
+  uint64_t rejected_requests;
+  uint64_t resizes;
+  uint64_t final_slots;
+  double span_ms;             /* first arrival to last completion */
+  double mops;                /* ops / span */
+  double p50_us, p99_us, max_us; /* request latency */
+  double mean_batch_ops;
+  double resize_ms;           /* time spent resizing (inside span) */
+} hive_serve_report_t;
+hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                               const uint64_t* h_req_offsets, const double* h_arrival_us, size_t nreq,
+                               uint32_t* out_vals, uint8_t* status, const hive_serve_config_t* cfg,
+                               double* h_latency_us, hive_serve_report_t* report, void* stream);
+
 /* ---- maintenance
  * hive_resize: rehash every live entry (buckets and stash) into
  *   new_num_slots / 8 buckets; replaces hive_hash_grow / hive_hash_shrink

@@ -1,0 +1,146 @@
+"""Serving loop (hive_serve_trace): continuous batching, admission, resize.
+
+Requests are given disjoint key sets, so every request must get exactly the
+results it gets alone -- and the oracle's phased execution of that request
+-- whatever batches the server coalesces it into.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from hivehashtable_amd import workloads as W
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+@pytest.fixture(scope="module")
+def H():
+    import hivehashtable_amd as H
+    return H
+
+
+def np_(t, dtype=None):
+    a = t.cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def norm(st):
+    return np.where(st == 3, 1, st)  # STASHED is a placement detail: count it as INSERTED
+
+
+def disjoint_requests(pre, fresh, nreq, seed, max_size=300):
+    """Requests whose ops touch pairwise distinct keys (key 0 aside)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, max_size, nreq)
+    sizes[3] = 0  # an empty request
+    total = int(sizes.sum())
+    pool = rng.permutation(np.concatenate([rng.permutation(pre)[: total // 2], fresh[: total - total // 2]]))
+    keys = pool[:total].astype(np.uint32)
+    types = rng.choice(3, size=total, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    inv = rng.random(total) < 0.01
+    keys[inv] = 0  # invalid keys: INVALID_KEY, table untouched
+    types[rng.random(total) < 0.005] = 4  # NONE ops: NOOP
+    vals = rng.integers(0, 2**32, total, dtype=np.uint64).astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    return types, keys, vals, off
+
+
+def test_serve_results_same_alone_and_in_mix(H, O):
+    print("[case] serve: coalesced vs alone vs oracle", flush=True)
+    slots = 1 << 16
+    pre = W.distinct_keys(slots // 2, 501)
+    pre_v = W.values(pre.size, 502)
+    fresh = W.distinct_keys(20000, 501, start=pre.size + 2)
+    types, keys, vals, off = disjoint_requests(pre, fresh, 60, 503)
+    nreq = off.size - 1
+    o = O.Oracle(slots // 8)
+    o.insert(pre, pre_v)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for r in range(nreq):
+        lo, hi = int(off[r]), int(off[r + 1])
+        if hi > lo:
+            ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    keyset = np.sort(o.dump() & 0xFFFFFFFF)
+    results = {}
+    for name, cap, inflight in (("mix", 700, 2), ("mix1", 5000, 1), ("alone", 1, 1)):
+        t = H.HiveTable(slots)
+        t.insert(pre, pre_v)
+        gv, gs, lat, rep = t.serve(types, keys, vals, off, np.zeros(nreq), max_batch_ops=cap, max_inflight=inflight)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        print(f"[case] serve {name}: {rep}", flush=True)
+        assert rep["requests"] == nreq and rep["ops"] == keys.size and rep["rejected_requests"] == 0
+        assert (lat >= 0).all() and rep["p99_us"] >= rep["p50_us"] > 0
+        if name == "alone":
+            assert rep["batches"] >= nreq - 1  # the empty request may share a batch
+        else:
+            assert rep["batches"] < nreq
+        assert np.array_equal(norm(gs), norm(os_)), name
+        assert np.array_equal(gv, ov), name
+        slots_, stash, cnt = t.export_raw()
+        live = O.RawState(t.num_buckets, slots_, stash, cnt).live_kv()
+        assert np.array_equal(np.sort(live & 0xFFFFFFFF), keyset), name
+        results[name] = (gs, gv)
+        t.close()
+
+
+def test_serve_grows_table_in_place(H, O):
+    print("[case] serve: resize under inserts", flush=True)
+    slots = 1 << 14
+    pre = W.distinct_keys(int(slots * 0.85), 511)
+    pre_v = W.values(pre.size, 512)
+    fresh = W.distinct_keys(8000, 511, start=pre.size + 2)
+    sizes = np.full(40, 200)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    keys = fresh[: int(off[-1])]
+    types = np.zeros(keys.size, np.uint8)  # inserts
+    vals = W.values(keys.size, 513)
+    arrivals = np.arange(sizes.size) * 20.0
+    t = H.HiveTable(slots)
+    t.insert(pre, pre_v)
+    gv, gs, lat, rep = t.serve(types, keys, vals, off, arrivals, max_load_factor=0.9, growth=2.0)
+    print(f"[case] serve resize: {rep}", flush=True)
+    assert rep["resizes"] >= 1 and rep["final_slots"] >= 2 * slots and rep["rejected_requests"] == 0
+    assert np.isin(np_(gs), (1, 3)).all()
+    st = t.stats()
+    assert st["capacity"] == rep["final_slots"] and st["load_factor"] <= 0.9
+    allk = np.concatenate([pre, keys])
+    allv = np.concatenate([pre_v, vals])
+    v, s = t.lookup(allk)
+    assert (np_(s) == 4).all()
+    assert np.array_equal(np_(v, np.uint32), allv)
+    t.close()
+
+
+def test_serve_rejects_inserts_without_room(H, O):
+    print("[case] serve: admission rejects inserts when no resize fits", flush=True)
+    slots = 1 << 14
+    pre = W.distinct_keys(int(slots * 0.89), 521)
+    pre_v = W.values(pre.size, 522)
+    fresh = W.distinct_keys(1000, 521, start=pre.size + 2)
+    # r0 lookups, r1 500 inserts (cannot fit under LF 0.9), r2 lookups, r3 10 inserts (fit)
+    parts = [(1, pre[:100]), (0, fresh[:500]), (1, pre[100:200]), (0, fresh[500:510])]
+    types = np.concatenate([np.full(k.size, ty, np.uint8) for ty, k in parts])
+    keys = np.concatenate([k for _, k in parts]).astype(np.uint32)
+    vals = W.values(keys.size, 523)
+    off = np.concatenate([[0], np.cumsum([k.size for _, k in parts])]).astype(np.uint64)
+    t = H.HiveTable(slots)
+    t.insert(pre, pre_v)
+    gv, gs, lat, rep = t.serve(types, keys, vals, off, np.zeros(4), max_batch_ops=1,
+                               hbm_reserve_bytes=1 << 62, stash_headroom=1e-9)
+    print(f"[case] serve reject: {rep}", flush=True)
+    gs, gv = np_(gs), np_(gv, np.uint32)
+    assert rep["rejected_requests"] == 1 and rep["resizes"] == 0
+    r1 = slice(int(off[1]), int(off[2]))
+    assert (gs[r1] == 0).all() and (gv[r1] == 0).all()
+    assert (gs[: int(off[1])] == 4).all() and (gs[int(off[2]): int(off[3])] == 4).all()
+    assert np.isin(gs[int(off[3]):], (1, 3)).all()
+    _, s = t.lookup(fresh[:510])
+    s = np_(s)
+    assert (s[:500] == 5).all() and (s[500:] == 4).all()
+    t.close()
