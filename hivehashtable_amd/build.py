@@ -41,6 +41,28 @@ def _compile(src, obj):
     return obj
 
 
+CLI_SRC = os.path.join(PKG, "cli", "hive_cli.cpp")
+CLI_BIN = os.path.join(PKG, "bin")
+CLI_NAMES = {1: "hive_hash_table_benchmark", 2: "hive_hash_table_ycsb", 3: "hive_hash_table_lookup_only_workload"}
+
+
+def build_cli(verbose=True):
+    """The reference's three benchmark binaries over libhive.so (host code, g++)."""
+    os.makedirs(CLI_BIN, exist_ok=True)
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
+    for mode, name in CLI_NAMES.items():
+        out = os.path.join(CLI_BIN, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-DHIVE_CLI_MODE={mode}", f"-I{os.path.join(ROOT, 'include')}",
+               CLI_SRC, "-o", out + ".tmp", f"-L{PKG}", "-lhive", f"-L{rocm_lib}", "-Wl,-rpath,$ORIGIN/..",
+               f"-Wl,-rpath,{rocm_lib}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {name}:\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {CLI_BIN}/{{{','.join(CLI_NAMES.values())}}}")
+
+
 def build(verbose=True):
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
@@ -57,6 +79,7 @@ def build(verbose=True):
         os.remove(o)
     if verbose:
         print(f"built {LIB}")
+    build_cli(verbose)
     return LIB
 
 
