@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
     p.add_argument("--mixed-batches", type=int, default=1000)
     p.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    p.add_argument("--sweep", action="store_true", help="also run the LF x exist-ratio sweep (2 policies)")
+    p.add_argument("--growth", action="store_true", help="also run the in-place growth leg (2^22 -> 2^27 slots)")
     p.add_argument("--serve-requests", type=int, default=1000)
     p.add_argument("--serve-offered-mops", type=float, default=4000.0,
                    help="offered load of the open-loop serving runs, million ops per second")
@@ -201,8 +203,15 @@ def serve_leg(a, H, W, torch, dev):
                "writes_applied": round(float(np.isin(st[~reads], (1, 2, 3)).mean()) if (~reads).any() else 1.0, 6)}
         return res
 
-    t = H.HiveTable(1 << 27, a.policy)
-    t.insert(d_pre, d_pre_v)
+    def serving_table(slots):
+        # serving shape: inline insert slow path (latency), duplicates combined per block (Zipf skew)
+        t = H.HiveTable(slots, a.policy)
+        t.set_launch(0, 512, inline_slow_path=1, combine_duplicates=1)
+        t.insert(d_pre, d_pre_v)
+        return t
+
+    out["insert_launch"] = {"block_threads": 512, "inline_slow_path": 1, "combine_duplicates": 1}
+    t = serving_table(1 << 27)
     for mix in "ABC":
         ty, k, v, off, arr = W.serving_trace(mix, pre, a.serve_requests, a.serve_offered_mops, a.seed + 5100 + ord(mix))
         res = {"ops": int(off[-1]), "open": run(t, ty, k, v, off, arr),
@@ -215,13 +224,120 @@ def serve_leg(a, H, W, torch, dev):
     res = {"ops": int(off[-1])}
     slots_d = int(math.ceil(n_pre / 0.899 / 8)) * 8
     for name, arrivals in (("open", arr), ("saturated", np.zeros_like(arr))):
-        t = H.HiveTable(slots_d, a.policy)
-        t.insert(d_pre, d_pre_v)
+        t = serving_table(slots_d)
         res["initial_slots"] = slots_d
         res[name] = run(t, ty, k, v, off, arrivals)
         t.close()
     out["mixes"]["D"] = res
     log(f"serve D: {res}")
+    return out
+
+
+def _timed(torch, stream, fn, reps=3):
+    """Median milliseconds of fn() over reps runs (HIP events on the launch stream)."""
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return float(np.median(ms))
+
+
+def sweep_leg(a, H, W, torch, dev):
+    """cfg2: bulk insert at LF {0.65, 0.80, 0.90, 0.95, 1.0} of 2^26 slots (seed
+    42 + i), then 2^26 lookups at exist ratio {0, 25, 50, 75, 100} %, for
+    Default2Hash and MurmurCityHash.  Each time is the median of 3 runs;
+    every lookup result is checked (found exactly for placed keys)."""
+    slots = 1 << 26
+    stream = torch.cuda.current_stream()
+    tables = {p: H.HiveTable(slots, p) for p in ("Default2Hash", "MurmurCityHash")}
+    d_st = torch.empty(slots, dtype=torch.uint8, device=dev)
+    d_lv = torch.empty(slots, dtype=torch.int32, device=dev)
+    d_ls = torch.empty(slots, dtype=torch.uint8, device=dev)
+    out = {p: {} for p in tables}
+    for i, lf in enumerate(W.LF_POINTS):
+        t0 = time.time()
+        n = W.keys_for_lf(slots, lf)
+        keys = W.distinct_keys(n, 42 + i)
+        d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
+        d_v = torch.from_numpy(W.values(n, 1042 + i).view(np.int32)).to(dev)
+        sets = {}
+        for x in (0.0, 0.25, 0.5, 0.75, 1.0):
+            lk, present = W.lookup_set(keys, slots, x, 2042 + i, key_seed=42 + i)
+            sets[x] = (torch.from_numpy(lk.view(np.int32)).to(dev), lk, present)
+        log(f"sweep LF {lf}: inputs in {time.time() - t0:.1f}s")
+        for pol, t in tables.items():
+            def ins():
+                t.clear()
+                t.insert(d_k, d_v, status=d_st[:n])
+            ins_ms = _timed(torch, stream, ins)
+            st = d_st[:n].cpu().numpy()
+            placed = np.isin(st, (1, 3))
+            stats = t.stats()
+            failed = keys[~placed]
+            row = {"inserts": n, "insert_ms": round(ins_ms, 4), "insert_mops": round(n / ins_ms / 1e3, 1),
+                   "failed": int((~placed).sum()), "stash_entries": stats["stash_live"],
+                   "slow_path_frac": round(stats["slow_path_ops"] / n, 5), "lookups": {}}
+            for x, (d_lk, lk, present) in sets.items():
+                look_ms = _timed(torch, stream, lambda: t.lookup(d_lk, vals=d_lv, status=d_ls))
+                found = d_ls.cpu().numpy() == 4
+                expect = present & ~np.isin(lk, failed) if failed.size else present
+                row["lookups"][f"{x:.2f}"] = {"ms": round(look_ms, 4), "mops": round(slots / look_ms / 1e3, 1),
+                                              "verified": bool(np.array_equal(found, expect))}
+            out[pol][f"{lf:.2f}"] = row
+            log(f"sweep {pol} LF {lf}: insert {row['insert_mops']} Mops/s, lookups "
+                f"{[v['mops'] for v in row['lookups'].values()]}")
+    for t in tables.values():
+        t.close()
+    return out
+
+
+def growth_leg(a, H, W, torch, dev):
+    """cfg3: start at 2^22 slots; 64 batches of 2^20 distinct keys; before each
+    batch the table grows x2 in place when the batch would lift LF above 0.9
+    (5 doublings, ending at 2^27 slots); each batch is followed by 2^20
+    lookups uniform over the keys inserted so far.  At the end every key is
+    looked up and the exact live count is checked."""
+    stream = torch.cuda.current_stream()
+    nbatch, bsz = 64, 1 << 20
+    keys = W.distinct_keys(nbatch * bsz, 4242)
+    d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
+    rng = np.random.default_rng(4243)
+    t = H.HiveTable(1 << 22)
+    cap = 1 << 22
+    live = 0
+    d_lk = torch.empty(bsz, dtype=torch.int32, device=dev)
+    ins_ms = look_ms = 0.0
+    resizes = []
+    t_start = time.perf_counter()
+    for b in range(nbatch):
+        if live + bsz > 0.9 * cap:
+            torch.cuda.synchronize()
+            r0 = time.perf_counter()
+            t.resize(cap * 2)
+            cap *= 2
+            resizes.append({"before_batch": b, "slots": cap, "ms": round((time.perf_counter() - r0) * 1e3, 3)})
+        sl = d_k[b * bsz:(b + 1) * bsz]
+        ins_ms += _timed(torch, stream, lambda: t.insert(sl), reps=1)
+        live += bsz
+        idx = torch.from_numpy(rng.integers(0, live, bsz)).to(dev)
+        d_lk.copy_(d_k[idx])
+        look_ms += _timed(torch, stream, lambda: t.lookup(d_lk), reps=1)
+    torch.cuda.synchronize()
+    total_ms = (time.perf_counter() - t_start) * 1e3
+    _, st = t.lookup(d_k)
+    found_all = bool((st == 4).all().item())
+    tab, stash = t.count_exact()
+    t.close()
+    out = {"batches": nbatch, "batch_keys": bsz, "final_slots": cap, "resizes": resizes,
+           "total_ms": round(total_ms, 2), "insert_mops": round(nbatch * bsz / ins_ms / 1e3, 1),
+           "lookup_mops": round(nbatch * bsz / look_ms / 1e3, 1),
+           "resize_ms_total": round(sum(r["ms"] for r in resizes), 2),
+           "verified": found_all and tab + stash == nbatch * bsz}
+    log(f"growth: {out}")
     return out
 
 
@@ -387,6 +503,8 @@ def main():
     serving = None
     if not a.no_serve:
         serving = serve_leg(a, H, W, torch, dev)
+    sweep = sweep_leg(a, H, W, torch, dev) if a.sweep else None
+    growth = growth_leg(a, H, W, torch, dev) if a.growth else None
 
     cpu = None
     if not a.no_cpu:
@@ -442,6 +560,8 @@ def main():
         },
         "mixed_stream": mixed,
         "serving": serving,
+        "lf_exist_sweep": sweep,
+        "growth": growth,
         "cpu_baseline": cpu,
         "git": git_sha(),
     }

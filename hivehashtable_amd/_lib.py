@@ -47,7 +47,8 @@ class Stats(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32), ("inline_slow_path", C.c_uint32)]
+    _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32), ("inline_slow_path", C.c_uint32),
+                ("combine_duplicates", C.c_uint32)]
 
 
 class Tune(C.Structure):

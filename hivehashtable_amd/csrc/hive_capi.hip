@@ -398,6 +398,7 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (!valid_block(c.block_threads)) return fail(HIVE_EINVAL, "block_threads must be a multiple of 64 in 64..1024");
   if (c.keys_per_lane != 1) return fail(HIVE_EINVAL, "keys_per_lane must be 1");
   if (c.inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
+  if (c.combine_duplicates > 1) return fail(HIVE_EINVAL, "combine_duplicates must be 0 or 1");
   t->launch[op_type] = c;
   return HIVE_OK;
 }

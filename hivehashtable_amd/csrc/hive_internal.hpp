@@ -37,7 +37,7 @@ struct hive_table {
   int policy = 0;
   hive::TableView v{};
   size_t bytes = 0;
-  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0}, {hive::kDefaultBlock, 1, 0}, {hive::kDefaultBlock, 1, 0}};
+  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0, 0}, {hive::kDefaultBlock, 1, 0, 0}, {hive::kDefaultBlock, 1, 0, 0}};
   unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
   uint32_t stream_tile = 0;              // persistent kernel: ops per tile (0 = sized per batch)
   uint32_t stream_blocks_per_cu = 0;     // persistent kernel: resident blocks per CU (0 = default)
@@ -72,7 +72,10 @@ hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase
                             BatchClock clk = BatchClock{nullptr, nullptr}) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_INSERT].block_threads;
-  if (t->launch[OP_INSERT].inline_slow_path) {
+  if (t->launch[OP_INSERT].combine_duplicates) {
+    HIVE_POLICY_SWITCH(t->policy, (k_insert_combined<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                    phase, clk)));
+  } else if (t->launch[OP_INSERT].inline_slow_path) {
     HIVE_POLICY_SWITCH(t->policy, (k_insert<P, Src, Out, true><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                  phase, clk)));
   } else {

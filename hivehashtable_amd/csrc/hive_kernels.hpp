@@ -163,6 +163,75 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   clk.finish();
 }
 
+// Inserts with the duplicate keys of one block combined before they reach
+// memory (launch option for skewed batches, where thousands of ops update a
+// few hot keys).  Each key's first lane in a block-local LDS hash table is its
+// leader and performs the insert with its own value (inline slow path); the
+// other lanes of the key are linearized before it: if the leader placed the
+// key, the first follower reports that placement and every other member
+// UPDATED; if it updated or failed, every member reports the same.  The value
+// left in the table is the leader's, the last of the group.
+template <class P, class Src, class Out>
+__global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src src, size_t n, Out out, int phase,
+                                                               BatchClock clk = BatchClock{nullptr, nullptr}) {
+  constexpr uint32_t kTab = 2 * kMaxBlock;  // at most half full
+  __shared__ unsigned long long tab[kTab];  // key << 32 | (lane + 1), 0 = empty
+  __shared__ uint32_t followers[kMaxBlock];
+  __shared__ uint8_t res[kMaxBlock];
+  for (uint32_t j = threadIdx.x; j < kTab; j += blockDim.x) tab[j] = 0ull;
+  followers[threadIdx.x] = 0;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool mine = i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT);
+  uint32_t key = 0, val = 0;
+  if (mine) {
+    key = src.key(i);
+    val = src.val(i, key);
+  }
+  __syncthreads();
+  uint32_t leader = threadIdx.x;
+  bool first_follower = false;
+  if (mine && key != 0) {
+    const unsigned long long word = (static_cast<unsigned long long>(key) << 32) | (threadIdx.x + 1u);
+    uint32_t h = (key * 0x9E3779B1u) >> 21;  // 11 bits: [0, kTab)
+    for (;;) {
+      const unsigned long long prev = atomicCAS(&tab[h], 0ull, word);
+      if (prev == 0ull) break;
+      if (static_cast<uint32_t>(prev >> 32) == key) {
+        leader = static_cast<uint32_t>(prev & 0xFFFFFFFFull) - 1u;
+        break;
+      }
+      h = (h + 1) & (kTab - 1);
+    }
+    if (leader != threadIdx.x) first_follower = atomicAdd(&followers[leader], 1u) == 0u;
+  }
+  __syncthreads();
+  int dtab = 0, dstash = 0, dslow = 0;
+  uint8_t r = R_FAILED;
+  if (mine && leader == threadIdx.x) {
+    r = fast_insert<P>(t, key, val);
+    if (r == R_PENDING) {
+      r = slow_insert<P>(t, key, val);
+      dslow = 1;
+    }
+    dtab += (r == R_INSERTED);
+    dstash += (r == R_STASHED);
+    res[threadIdx.x] = r;
+  }
+  __syncthreads();
+  if (mine) {
+    uint8_t s;
+    if (leader == threadIdx.x) {
+      s = (followers[threadIdx.x] && (r == R_INSERTED || r == R_STASHED)) ? R_UPDATED : r;
+    } else {
+      const uint8_t rl = res[leader];
+      s = (rl == R_INSERTED || rl == R_STASHED) ? (first_follower ? rl : R_UPDATED) : rl;
+    }
+    out.put(i, s, 0);
+  }
+  wave_count(t, dtab, dstash, dslow);
+  clk.finish();
+}
+
 // ---------------------------------------------------------------- lookup
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size_t n, Out out, int phase) {

@@ -446,7 +446,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   const uint32_t blocks[] = {128, 256, 512, 1024};
   for (uint32_t inl = 0; inl < 2; ++inl)
     for (uint32_t blk : blocks) {
-      for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{blk, 1, inl};
+      for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates};
       double ms = 0;
       if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
       ++cands;
@@ -456,7 +456,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
         best.inline_slow_path = inl;
       }
     }
-  for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path};
+  for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates};
   if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms)) != HIVE_OK) return r;
   ++cands;
   const uint32_t tiles[] = {0, 256, 1024, 4096};
@@ -481,7 +481,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   best.best_mode = HIVE_STREAM_PERSISTENT;
   if (best.launches_ms < best.persistent_ms && best.launches_ms <= best.graph_ms) best.best_mode = HIVE_STREAM_LAUNCHES;
   else if (best.graph_ms < best.persistent_ms) best.best_mode = HIVE_STREAM_GRAPH;
-  for (int op = 0; op < 3; ++op) t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path};
+  for (int op = 0; op < 3; ++op) t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates};
   t->stream_tile = best.tile_ops;
   t->stream_blocks_per_cu = best.blocks_per_cu;
   t->stream_inline = best.stream_inline;

@@ -93,6 +93,9 @@ typedef struct {
   uint32_t inline_slow_path; /* inserts: 0 = stage slow-path ops in LDS behind one
                                 block barrier (throughput), 1 = each lane runs its
                                 own slow path at once (latency) */
+  uint32_t combine_duplicates; /* inserts: 1 = ops of one block on the same key are
+                                  combined into one table update before reaching
+                                  memory (skewed batches; slow path inline) */
 } hive_launch_t;
 
 /* ---- lifecycle: replaces hiveHashTableCreate / Destroy / Clear

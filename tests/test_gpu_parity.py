@@ -129,9 +129,11 @@ def test_insert_at_full_load_keeps_invariants(H, O):
     assert abs(int((~ok).sum()) - int((~placed(so)).sum())) <= max(64, 0.02 * slots)
 
 
-def test_duplicates_in_one_batch_and_invalid_keys(H, O):
-    print("[case] duplicate keys / key 0", flush=True)
+@pytest.mark.parametrize("combine", [0, 1])
+def test_duplicates_in_one_batch_and_invalid_keys(H, O, combine):
+    print(f"[case] duplicate keys / key 0, combine={combine}", flush=True)
     t = H.HiveTable(1 << 12)
+    t.set_launch(0, 256, combine_duplicates=combine)
     keys = np.array([5, 5, 5, 9, 0, 9, 0xFFFFFFFF], np.uint32)
     vals = np.array([1, 2, 3, 4, 5, 6, 7], np.uint32)
     st = np_(t.insert(keys, vals))
@@ -151,19 +153,21 @@ def test_duplicates_in_one_batch_and_invalid_keys(H, O):
     assert t.count_exact() == (3, 0)
 
 
-def test_contended_same_key_inserts_are_linearizable(H, O):
+@pytest.mark.parametrize("combine", [0, 1])
+def test_contended_same_key_inserts_are_linearizable(H, O, combine):
     # Litmus for cross-XCD coherence of the claim/update protocol: 2^20 inserts
     # over only 4096 distinct keys, shuffled so every key is written by lanes
     # of many workgroups (and so of every XCD) in the same launch.  Exactly one
     # insert per key may report INSERTED, the table must hold each key once,
     # and its value must be one of the values written for it.
-    print("[case] contended same-key inserts", flush=True)
+    print(f"[case] contended same-key inserts, combine={combine}", flush=True)
     rng = np.random.default_rng(81)
     base = W.distinct_keys(4096, 82)
     keys = np.tile(base, 256)
     rng.shuffle(keys)
     vals = (keys ^ np.uint32(0x5bd1e995)).astype(np.uint32)  # value is a function of the key
     t = H.HiveTable(1 << 16)
+    t.set_launch(0, 1024, combine_duplicates=combine)
     st = np_(t.insert(keys, vals))
     assert ((st == 1) | (st == 3)).sum() == base.size
     assert (st == 2).sum() == keys.size - base.size
@@ -173,6 +177,7 @@ def test_contended_same_key_inserts_are_linearizable(H, O):
     assert np.array_equal((kv >> 32).astype(np.uint32), (kv & 0xFFFFFFFF).astype(np.uint32) ^ np.uint32(0x5bd1e995))
     # same with a table small enough that most keys take the locked slow path
     t = H.HiveTable(1 << 9)
+    t.set_launch(0, 1024, combine_duplicates=combine)
     small = base[:600]
     keys = np.tile(small, 64)
     rng.shuffle(keys)
@@ -184,13 +189,15 @@ def test_contended_same_key_inserts_are_linearizable(H, O):
     assert ((st == 1) | (st == 3)).sum() == k_in.size
 
 
-def test_mixed_batches_match_phased_oracle(H, O):
-    print("[case] mixed batches 50/40/10", flush=True)
+@pytest.mark.parametrize("combine", [0, 1])
+def test_mixed_batches_match_phased_oracle(H, O, combine):
+    print(f"[case] mixed batches 50/40/10, combine={combine}", flush=True)
     slots = 1 << 18
     pre = W.distinct_keys(slots // 2, 31)
     o = O.Oracle(slots // 8)
     o.insert(pre)
     t = H.HiveTable(slots)
+    t.set_launch(0, 512, combine_duplicates=combine)
     t.insert(pre)
     conflicted = set()
     sizes = W.log_uniform_sizes(12, 1 << 8, 1 << 15, 32)
@@ -311,11 +318,11 @@ def test_launch_shapes_give_identical_results(H, O):
     keys = W.distinct_keys(int(slots * 0.85), 71)
     lk, _ = W.lookup_set(keys, keys.size, 0.5, 72)
     ref = None
-    for blk, inl in ((64, 0), (256, 0), (1024, 0), (64, 1), (512, 1)):
+    for blk, inl, comb in ((64, 0, 0), (256, 0, 0), (1024, 0, 0), (64, 1, 0), (512, 1, 0), (256, 0, 1), (1024, 1, 1)):
         t = H.HiveTable(slots)
         for op in (0, 1, 2):
-            t.set_launch(op, blk, inline_slow_path=inl)
-        assert t.get_launch(1) == (blk, 1, inl)
+            t.set_launch(op, blk, inline_slow_path=inl, combine_duplicates=comb)
+        assert t.get_launch(1) == (blk, 1, inl, comb)
         t.insert(keys)
         gv, gs = t.lookup(lk)
         cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
@@ -427,4 +434,4 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     assert res["stream_inline"] in (0, 1)
     assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
-    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"])
+    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0)
