@@ -580,20 +580,19 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  // the hint word (L2-resident) is requested together with H0, so a miss
-  // reaches H1 or the stash one round trip earlier
-  const uint32_t word = meta_word<P>(t, b[0]);
   {
     unsigned long long s[kSlots];
     read_bucket<P>(bucket_ptr(t, b[0]), s);
-    asm volatile("" ::"v"(word));  // keep the hint load issued ahead of the H0 test
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
   }
-  const uint32_t hint = meta_bits(word, b[0]);
+  // the hint word is read only after an H0 miss: requesting it with H0 costs
+  // every lookup one more memory request, which the request-rate-bound kernel
+  // pays for more than the round trip it saves on misses (measured)
+  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
   if (hint & kOverflowBit) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {

@@ -120,10 +120,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
   const size_t i = base + threadIdx.x;
   int dtab = 0, dstash = 0, dslow = 0;
+  // key and value are requested together with the op type (one round trip)
+  const uint32_t key = i < n ? src.key(i) : 0u;
+  const uint32_t val = i < n ? src.val(i, key) : 0u;
   if constexpr (Inline) {
     if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-      const uint32_t key = src.key(i);
-      const uint32_t val = src.val(i, key);
       uint8_t r = fast_insert<P>(t, key, val);
       if (r == R_PENDING) {
         r = slow_insert<P>(t, key, val);
@@ -139,8 +140,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-      const uint32_t key = src.key(i);
-      const uint8_t r = fast_insert<P>(t, key, src.val(i, key));
+      const uint8_t r = fast_insert<P>(t, key, val);
       if (r == R_PENDING) {
         queue[atomicAdd(&qn, 1u)] = threadIdx.x;
       } else {
@@ -181,12 +181,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src 
   for (uint32_t j = threadIdx.x; j < kTab; j += blockDim.x) tab[j] = 0ull;
   followers[threadIdx.x] = 0;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t key = i < n ? src.key(i) : 0u;  // requested together with the op type
+  const uint32_t val = i < n ? src.val(i, key) : 0u;
   const bool mine = i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT);
-  uint32_t key = 0, val = 0;
-  if (mine) {
-    key = src.key(i);
-    val = src.val(i, key);
-  }
   __syncthreads();
   uint32_t leader = threadIdx.x;
   bool first_follower = false;
@@ -237,6 +234,7 @@ template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size_t n, Out out, int phase) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const uint32_t key = src.key(i);  // requested together with the op type
   if (phase >= 0) {
     const int ty = src.type(i, OP_LOOKUP);
     if (ty != OP_LOOKUP) {
@@ -245,7 +243,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
     }
   }
   uint32_t v;
-  const uint8_t r = lookup_one<P>(t, src.key(i), v);
+  const uint8_t r = lookup_one<P>(t, key, v);
   out.put(i, r, v);
 }
 
@@ -254,8 +252,9 @@ template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_t n, Out out, int phase) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int rt = 0, rs = 0;
+  const uint32_t key = i < n ? src.key(i) : 0u;  // requested together with the op type
   if (i < n && (phase < 0 || src.type(i, OP_DELETE) == OP_DELETE)) {
-    const uint8_t r = erase_one<P>(t, src.key(i), rt, rs);
+    const uint8_t r = erase_one<P>(t, key, rt, rs);
     out.put(i, r, 0);
   }
   wave_count(t, -rt, -rs);
