@@ -76,3 +76,22 @@ def test_cpu_mixed_matches_phased_oracle(O):
     ka = dict(zip((a & 0xFFFFFFFF).tolist(), (a >> 32).tolist()))
     for kv in b[clean].tolist():
         assert ka[kv & 0xFFFFFFFF] == kv >> 32
+
+
+def test_cpu_path_is_race_free_under_tsan():
+    """The multithreaded CPU path, built with ThreadSanitizer, under contended
+    inserts and mixed batches on a small table: no race report, and the state
+    after each step passes the oracle's invariant checker."""
+    import os
+    import shutil
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    b = subprocess.run(["make", "-s", "-C", here, "tsan"], capture_output=True, text=True)
+    if b.returncode != 0:
+        pytest.skip(f"ThreadSanitizer build unavailable: {b.stderr[-300:]}")
+    r = subprocess.run([os.path.join(here, "tsan_driver")], capture_output=True, text=True, timeout=300)
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-2000:]
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "all checks passed" in r.stdout
