@@ -107,6 +107,7 @@ def lib():
         "hive_mixed_ops": [p, p, sz, p, p],
         "hive_resize": [p, u64, p],
         "hive_stash_drain": [p, p, sz, C.POINTER(sz), p],
+        "hive_dump": [p, p, sz, C.POINTER(sz), p],
         "hive_stash_reinsert": [p, C.POINTER(sz), p],
         "hive_stats": [p, C.POINTER(Stats), p],
         "hive_count_exact": [p, C.POINTER(u64), C.POINTER(u64), p],

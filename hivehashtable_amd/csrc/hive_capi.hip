@@ -267,6 +267,29 @@ hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, si
   return reset_counters(t, t->v, s, tab, 0);
 }
 
+hive_status_t hive_dump(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint64_t tab = 0, st = 0;
+  if ((r = count_exact_view(t, t->v, s, &tab, &st)) != HIVE_OK) return r;
+  const uint64_t total = tab + st;
+  if (n_out) *n_out = static_cast<size_t>(total);
+  if (total > cap) return fail(HIVE_ERANGE, "dump: output buffer too small");
+  if (total == 0) return HIVE_OK;
+  if (!out_kv) return fail(HIVE_EINVAL, "null out_kv");
+  auto* out = reinterpret_cast<unsigned long long*>(out_kv);
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 8, s));
+  k_dump_slots<<<grid_for(static_cast<size_t>(t->v.nb) * kSlots, 256), 256, 0, s>>>(t->v, out, t->scratch);
+  k_stash_compact<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, out, t->scratch);
+  HIP_CHECK_LAUNCH();
+  unsigned long long written = 0;
+  HIP_TRY(hipMemcpyAsync(&written, t->scratch, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (written != total) return fail(HIVE_EDEVICE, "dump: the table changed during the export");
+  return HIVE_OK;
+}
+
 hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* stream) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;

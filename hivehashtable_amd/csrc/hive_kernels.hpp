@@ -377,6 +377,22 @@ static __global__ void k_stash_compact(TableView t, unsigned long long* out, uns
   }
 }
 
+// live bucket entries of slots [0, nb*8), appended at *pos with one atomic
+// per wavefront
+static __global__ void k_dump_slots(TableView t, unsigned long long* out, unsigned long long* pos) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t n = static_cast<size_t>(t.nb) * kSlots;
+  const unsigned long long kv = i < n ? t.slots[i] : 0ull;
+  const unsigned long long live = __ballot(kv != 0ull);
+  if (live == 0ull) return;  // uniform across the wavefront
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int first = __ffsll(static_cast<long long>(live)) - 1;
+  unsigned long long base = 0;
+  if (lane == first) base = atomicAdd(pos, static_cast<unsigned long long>(__popcll(live)));
+  base = __shfl(base, first);
+  if (kv != 0ull) out[base + __popcll(live & ((1ull << lane) - 1ull))] = kv;
+}
+
 // ---------------------------------------------------------------- probe stats
 // Per key, following the probe order of fast_insert / lookup_one exactly:
 //   out[0] buckets an insert's fast path reads (H0, plus the other distinct

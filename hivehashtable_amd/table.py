@@ -221,6 +221,15 @@ class HiveTable:
         check(self.L.hive_stash_drain(self.h, _ptr(out), cap, C.byref(n), _stream_ptr(stream)), "hive_stash_drain")
         return out[: n.value]
 
+    def dump(self, cap=None, stream=None):
+        """Every live entry as packed kv (int64 device tensor, unordered)."""
+        torch = _torch()
+        cap = 2 * self.num_buckets * 8 if cap is None else int(cap)
+        out = torch.empty(max(cap, 1), dtype=torch.int64, device="cuda")
+        n = C.c_size_t(0)
+        check(self.L.hive_dump(self.h, _ptr(out), cap, C.byref(n), _stream_ptr(stream)), "hive_dump")
+        return out[: n.value]
+
     def stash_reinsert(self, stream=None):
         n = C.c_size_t(0)
         check(self.L.hive_stash_reinsert(self.h, C.byref(n), _stream_ptr(stream)), "hive_stash_reinsert")

@@ -242,6 +242,12 @@ hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint
  * hive_stash_reinsert: drain the stash and insert the drained entries back
  *   (with their values) through the normal insert path.  Synchronizes. */
 hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream);
+/* hive_dump: every live entry (bucket slots, then stash) as packed kv into
+ *   out_kv (device, cap entries), in no particular order -- the content
+ *   multiset, for checks and persistence (the reference has no export; its
+ *   tables live for one dispatch call).  *n_out (host) receives the count;
+ *   HIVE_ERANGE (nothing written) if it exceeds cap.  Synchronizes. */
+hive_status_t hive_dump(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream);
 hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream);
 hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* stream);
 

@@ -250,6 +250,24 @@ def test_resize_growth_sequence(H, O):
     assert t.num_buckets == slots // 8
 
 
+def test_dump_is_the_content_multiset(H, O):
+    print("[case] dump", flush=True)
+    nb = 1 << 10
+    keys = W.distinct_keys(int(nb * 8 * 0.97), 57)
+    o = O.Oracle(nb, "MurmurCityHash")
+    o.insert(keys, W.values(keys.size, 58))
+    raw = o.export()
+    t = H.HiveTable(nb * 8, "MurmurCityHash")
+    t.import_raw(raw.slots, raw.stash, raw.stash_count)
+    assert o.count_stash() > 0
+    d = np_(t.dump()).view(np.uint64)
+    assert np.array_equal(np.sort(d), np.sort(o.dump()))
+    with pytest.raises(H.HiveError):
+        t.dump(cap=10)                    # HIVE_ERANGE: nothing written
+    t.clear()
+    assert t.dump().numel() == 0
+
+
 def test_stash_drain_and_reinsert(H, O):
     print("[case] stash drain / reinsert", flush=True)
     nb = 1 << 10
@@ -332,19 +350,25 @@ def test_launch_shapes_give_identical_results(H, O):
         ref = cur
 
 
-def test_full_size_lf090_properties(H):
-    """2^26 slots, LF 0.9 (60,397,977 keys): size-independent properties."""
+@pytest.mark.parametrize("policy,lf", [("Default2Hash", 0.9), ("MurmurCityHash", 0.95), ("Default2Hash", 1.0)])
+def test_full_size_properties(H, policy, lf):
+    """2^26 slots at the sweep's load factors: size-independent properties
+    (every placed key found with its value, absent keys not found, exact
+    counts, no key stored twice, re-insert = update)."""
     import torch
-    print("[case] 2^26 slots LF 0.9 full size", flush=True)
+    print(f"[case] 2^26 slots LF {lf} {policy} full size", flush=True)
     slots = 1 << 26
-    n = W.keys_for_lf(slots, 0.9)
+    n = W.keys_for_lf(slots, lf)
     keys = W.distinct_keys(n, 44)
     vals = W.values(n, 45)
-    absent = W.distinct_keys(1 << 22, 46, exclude=keys)
-    t = H.HiveTable(slots)
+    absent = W.distinct_keys(1 << 22, 44, start=n + (n >> 6) + 64)
+    t = H.HiveTable(slots, policy)
     st = t.insert(keys, vals)
     ok = placed(np_(st))
-    assert ok.mean() > 0.999
+    assert ok.mean() > 0.9999
+    assert (np_(st)[~ok] == 0).all()  # the only other outcome: FAILED (stash full)
+    kv = np_(t.dump()).view(np.uint64)
+    assert kv.size == ok.sum() and np.unique(kv & np.uint64(0xFFFFFFFF)).size == kv.size
     dk = torch.from_numpy(keys.view(np.int32)).cuda()
     gv, gs = t.lookup(dk)
     gs = np_(gs)
