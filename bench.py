@@ -497,6 +497,18 @@ def main():
         torch.cuda.synchronize()
         cms.append(e0.elapsed_time(e1))
     cas_rate = n_ins / (min(cms) / 1e3)
+    # the same gather with 4 / 8 / 16 lanes cooperating per bucket (wavefront-per-bucket layouts)
+    lane_rates = {1: round(gather_rate, 1)}
+    for lanes in (4, 8, 16):
+        lms = []
+        for r in range(3):
+            e0.record(stream)
+            L.hive_ceiling_gather_lanes(C.c_void_p(buf.data_ptr()), nb * 64, n_g, lanes, 17 + r,
+                                        C.c_void_p(sink.data_ptr()), sptr)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            lms.append(e0.elapsed_time(e1))
+        lane_rates[lanes] = round(n_g / (min(lms) / 1e3), 1)
     del buf, sink
     look_read_rate = lreads / (look_ms / 1e3)
     ins_read_rate = cand / (ins_ms / 1e3)
@@ -561,6 +573,7 @@ def main():
             "lookup_frac_of_gather_ceiling": round(look_read_rate / gather_rate, 4),
             "insert_bucket_reads_per_s": round(ins_read_rate, 1),
             "insert_frac_of_gather_ceiling": round(ins_read_rate / gather_rate, 4),
+            "gather_64B_per_s_by_lanes_per_bucket": lane_rates,
             "footprint_bytes": nb * 64,
         },
         "mixed_stream": mixed,

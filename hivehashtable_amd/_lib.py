@@ -124,6 +124,7 @@ def lib():
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
         "hive_ceiling_gather": [p, u64, u64, u32, p, p],
         "hive_ceiling_cas": [p, u64, u64, u32, p],
+        "hive_ceiling_gather_lanes": [p, u64, u64, u32, u32, p, p],
     }
     for name, args in proto.items():
         f = getattr(L, name)

@@ -443,6 +443,25 @@ hive_status_t hive_ceiling_gather(const uint64_t* buf, uint64_t footprint_bytes,
   return HIVE_OK;
 }
 
+hive_status_t hive_ceiling_gather_lanes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
+                                        uint32_t lanes_per_bucket, uint32_t seed, uint32_t* sink, void* stream) {
+  if (lanes_per_bucket == 1) return hive_ceiling_gather(buf, footprint_bytes, n, seed, sink, stream);
+  const uint64_t nbk = footprint_bytes / 64;
+  if (!buf || nbk == 0 || nbk > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "bad ceiling buffer");
+  if (n == 0) return HIVE_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const auto* b = reinterpret_cast<const unsigned long long*>(buf);
+  const uint32_t nb32 = static_cast<uint32_t>(nbk);
+  switch (lanes_per_bucket) {
+    case 4: k_ceiling_gather_coop<4><<<grid_for(n * 4, 256), 256, 0, s>>>(b, nb32, n, seed, sink); break;
+    case 8: k_ceiling_gather_coop<8><<<grid_for(n * 8, 256), 256, 0, s>>>(b, nb32, n, seed, sink); break;
+    case 16: k_ceiling_gather_coop<16><<<grid_for(n * 16, 256), 256, 0, s>>>(b, nb32, n, seed, sink); break;
+    default: return fail(HIVE_EINVAL, "lanes_per_bucket must be 1, 4, 8 or 16");
+  }
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
 hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed, void* stream) {
   const uint64_t nw = footprint_bytes / 8;
   if (!buf || nw == 0) return fail(HIVE_EINVAL, "bad ceiling buffer");

@@ -461,6 +461,30 @@ static __global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned lo
   if (static_cast<uint32_t>(acc) == 0x9e3779b9u && sink) sink[blockIdx.x] = static_cast<uint32_t>(acc >> 32);
 }
 
+// n random 64-byte bucket reads with L lanes cooperating on each bucket (one
+// 64/L-byte piece per lane, L = 4, 8 or 16): the wavefront-per-bucket layout
+// against which the lane-per-key kernels are measured
+template <int L>
+static __global__ __launch_bounds__(256) void k_ceiling_gather_coop(const unsigned long long* buf, uint32_t nbuckets,
+                                                                    size_t n, uint32_t seed, uint32_t* sink) {
+  const size_t tid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t i = tid / L;
+  if (i >= n) return;
+  const uint32_t part = static_cast<uint32_t>(tid % L);
+  const uint32_t b = ceiling_index(i, seed) % nbuckets;
+  uint32_t acc;
+  if constexpr (L == 4) {
+    const uint4 v = reinterpret_cast<const uint4*>(buf)[static_cast<size_t>(b) * 4 + part];
+    acc = v.x ^ v.y ^ v.z ^ v.w;
+  } else if constexpr (L == 8) {
+    const uint2 v = reinterpret_cast<const uint2*>(buf)[static_cast<size_t>(b) * 8 + part];
+    acc = v.x ^ v.y;
+  } else {
+    acc = reinterpret_cast<const uint32_t*>(buf)[static_cast<size_t>(b) * 16 + part];
+  }
+  if (acc == 0x9e3779b9u && sink) sink[0] = acc;  // anti-dead-code sink: one u32
+}
+
 // n random 64-bit CAS (one per lane)
 static __global__ __launch_bounds__(256) void k_ceiling_cas(unsigned long long* buf, size_t nwords, size_t n, uint32_t seed) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -302,6 +302,12 @@ hive_status_t hive_hash_table_kernel_dispatch_ycsb(const hive_operation_t* h_pre
 hive_status_t hive_ceiling_gather(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed,
                                   uint32_t* sink, void* stream);
 hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t n, uint32_t seed, void* stream);
+/* hive_ceiling_gather_lanes: the gather with lanes_per_bucket (1, 4, 8, 16)
+ * lanes cooperating on each 64-byte bucket -- the lane-per-key against
+ * wavefront-per-bucket layout measurement.  For lanes_per_bucket > 1 the
+ * sink receives at most one u32 (sink[0]). */
+hive_status_t hive_ceiling_gather_lanes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
+                                        uint32_t lanes_per_bucket, uint32_t seed, uint32_t* sink, void* stream);
 
 const char* hive_status_string(hive_status_t s);
 const char* hive_last_error(void);
