@@ -301,48 +301,38 @@ def sweep_leg(a, H, W, torch, dev):
 
 
 def growth_leg(a, H, W, torch, dev):
-    """cfg3: start at 2^22 slots; 64 batches of 2^20 distinct keys; before each
-    batch the table grows x2 in place when the batch would lift LF above 0.9
-    (5 doublings, ending at 2^27 slots); each batch is followed by 2^20
-    lookups uniform over the keys inserted so far.  At the end every key is
-    looked up and the exact live count is checked."""
-    stream = torch.cuda.current_stream()
+    """cfg3: start at 2^22 slots; 64 batches of 2^20 distinct keys, each
+    followed by 2^20 lookups uniform over the keys inserted so far; the table
+    grows x2 in place whenever the next batch could lift LF above 0.9 (5
+    doublings, ending at 2^27 slots).  Run twice through hive_insert_sequence:
+    plain per-batch calls with host-side resize decisions, and the whole
+    sequence captured into one hipGraph.  Every key is looked up and the
+    exact count checked after each run."""
     nbatch, bsz = 64, 1 << 20
     keys = W.distinct_keys(nbatch * bsz, 4242)
     d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
     rng = np.random.default_rng(4243)
-    t = H.HiveTable(1 << 22)
-    cap = 1 << 22
-    live = 0
-    d_lk = torch.empty(bsz, dtype=torch.int32, device=dev)
-    ins_ms = look_ms = 0.0
-    resizes = []
-    t_start = time.perf_counter()
-    for b in range(nbatch):
-        if live + bsz > 0.9 * cap:
-            torch.cuda.synchronize()
-            r0 = time.perf_counter()
-            t.resize(cap * 2)
-            cap *= 2
-            resizes.append({"before_batch": b, "slots": cap, "ms": round((time.perf_counter() - r0) * 1e3, 3)})
-        sl = d_k[b * bsz:(b + 1) * bsz]
-        ins_ms += _timed(torch, stream, lambda: t.insert(sl), reps=1)
-        live += bsz
-        idx = torch.from_numpy(rng.integers(0, live, bsz)).to(dev)
-        d_lk.copy_(d_k[idx])
-        look_ms += _timed(torch, stream, lambda: t.lookup(d_lk), reps=1)
-    torch.cuda.synchronize()
-    total_ms = (time.perf_counter() - t_start) * 1e3
-    _, st = t.lookup(d_k)
-    found_all = bool((st == 4).all().item())
-    tab, stash = t.count_exact()
-    t.close()
-    out = {"batches": nbatch, "batch_keys": bsz, "final_slots": cap, "resizes": resizes,
-           "total_ms": round(total_ms, 2), "insert_mops": round(nbatch * bsz / ins_ms / 1e3, 1),
-           "lookup_mops": round(nbatch * bsz / look_ms / 1e3, 1),
-           "resize_ms_total": round(sum(r["ms"] for r in resizes), 2),
-           "verified": found_all and tab + stash == nbatch * bsz}
-    log(f"growth: {out}")
+    idx = np.concatenate([rng.integers(0, (b + 1) * bsz, bsz) for b in range(nbatch)])
+    d_lk = d_k[torch.from_numpy(idx).to(dev)].contiguous()
+    d_st = torch.empty(keys.size, dtype=torch.uint8, device=dev)
+    d_ls = torch.empty(idx.size, dtype=torch.uint8, device=dev)
+    out = {"batches": nbatch, "batch_keys": bsz, "lookups_per_batch": bsz, "start_slots": 1 << 22}
+    for name, mode in (("plain", 0), ("graph", 1)):
+        t = H.HiveTable(1 << 22)
+        rep = t.insert_sequence(d_k, bsz, lookup_keys=d_lk, lookups_per_batch=bsz, max_load_factor=0.9, growth=2.0,
+                                mode=mode, status=d_st, lookup_status=d_ls)
+        ok = bool(torch.isin(d_st, torch.tensor([1, 3], device=dev, dtype=torch.uint8)).all().item()
+                  and (d_ls == 4).all().item())
+        _, st = t.lookup(d_k)
+        tab, stash = t.count_exact()
+        ok = ok and bool((st == 4).all().item()) and tab + stash == keys.size
+        ops = 2 * keys.size
+        out[name] = {"ms": round(rep["elapsed_ms"], 3), "setup_ms": round(rep["setup_ms"], 3),
+                     "mops": round(ops / rep["elapsed_ms"] / 1e3, 1), "resizes": rep["resizes"],
+                     "final_slots": rep["final_slots"], "verified": ok}
+        t.close()
+        log(f"growth {name}: {out[name]}")
+    out["graph_speedup"] = round(out["plain"]["ms"] / out["graph"]["ms"], 3)
     return out
 
 

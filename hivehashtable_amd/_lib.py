@@ -16,6 +16,7 @@ HIVE_OK, HIVE_EINVAL, HIVE_ENOMEM, HIVE_EDEVICE, HIVE_ECAPACITY, HIVE_ERANGE = r
 R_FAILED, R_INSERTED, R_UPDATED, R_STASHED, R_FOUND, R_NOT_FOUND, R_ERASED, R_INVALID_KEY, R_NOOP = range(9)
 OP_INSERT, OP_LOOKUP, OP_DELETE, OP_ATOMIC_INC, OP_NONE = range(5)
 STREAM_LAUNCHES, STREAM_GRAPH, STREAM_PERSISTENT = range(3)
+SEQUENCE_PLAIN, SEQUENCE_GRAPH = range(2)
 POLICIES = {
     "Default2Hash": 0,
     "MurmurCityHash": 1,
@@ -70,6 +71,11 @@ class ServeReport(C.Structure):
                 ("max_us", C.c_double), ("mean_batch_ops", C.c_double), ("resize_ms", C.c_double)]
 
 
+class SequenceReport(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("setup_ms", C.c_double), ("resizes", C.c_uint32),
+                ("final_slots", C.c_uint64), ("rehash_failed", C.c_uint64)]
+
+
 class Operation(C.Structure):
     """The reference's 16-byte Operation record (include/utils.h:18-21)."""
     _fields_ = [("type", C.c_int32), ("pad", C.c_uint32), ("key", C.c_uint64)]
@@ -122,6 +128,8 @@ def lib():
         "hive_set_stream_shape": [p, u32, u32, u32],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
+        "hive_insert_sequence": [p, p, p, sz, sz, p, sz, C.c_double, C.c_double, i32, p, p,
+                                 C.POINTER(SequenceReport), p],
         "hive_ceiling_gather": [p, u64, u64, u32, p, p],
         "hive_ceiling_cas": [p, u64, u64, u32, p],
         "hive_ceiling_gather_lanes": [p, u64, u64, u32, u32, p, p],

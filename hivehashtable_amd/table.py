@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (Launch, Operation, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib, OP_INSERT,
+from ._lib import (Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib,
+                   OP_INSERT,
                    OP_LOOKUP, OP_DELETE)
 
 
@@ -184,6 +185,21 @@ class HiveTable:
                                       C.byref(cfg), lat.ctypes.data_as(C.c_void_p), C.byref(rep),
                                       _stream_ptr(stream)), "hive_serve_trace")
         return out_vals, status, lat[:nreq], {f: getattr(rep, f) for f, _ in ServeReport._fields_}
+
+    def insert_sequence(self, keys, batch_keys, vals=None, lookup_keys=None, lookups_per_batch=0,
+                        max_load_factor=0.9, growth=2.0, mode=0, status=None, lookup_status=None, stream=None):
+        """Insert keys in batches with in-place growth (hive_insert_sequence);
+        mode 0 plain per-batch calls, 1 one captured hipGraph.  Returns the report."""
+        torch = _torch()
+        k = _dev(keys, torch.int32)
+        v = _dev(vals, torch.int32)
+        lk = _dev(lookup_keys, torch.int32)
+        rep = SequenceReport()
+        check(self.L.hive_insert_sequence(self.h, _ptr(k), _ptr(v), k.numel(), int(batch_keys), _ptr(lk),
+                                          int(lookups_per_batch), float(max_load_factor), float(growth), int(mode),
+                                          _ptr(status), _ptr(lookup_status), C.byref(rep), _stream_ptr(stream)),
+              "hive_insert_sequence")
+        return {f: getattr(rep, f) for f, _ in SequenceReport._fields_}
 
     def set_stream_shape(self, tile_ops=0, blocks_per_cu=0, inline_slow_path=0):
         check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu), int(inline_slow_path)),

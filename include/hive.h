@@ -229,6 +229,35 @@ hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint
                                uint32_t* out_vals, uint8_t* status, const hive_serve_config_t* cfg,
                                double* h_latency_us, hive_serve_report_t* report, void* stream);
 
+/* ---- insert sequences with in-place growth (cfg3 shape).
+ * Inserts keys [0, n) in batches of batch_keys; after batch b, if
+ * lookups_per_batch > 0, looks up lookup_keys[b*lookups_per_batch ...]
+ * (device, nbatches * lookups_per_batch keys; lookup_status may be NULL).
+ * The table grows in place (x growth, repeated until it fits) before any
+ * batch that could lift (table + stash entries) / capacity above
+ * max_load_factor.
+ *   HIVE_SEQUENCE_PLAIN: per batch, the host reads the exact count
+ *     (hive_stats) and calls hive_resize -- the reference's DYNAMIC_RESIZE
+ *     loop (src/hive_dispatch.cu:181-211) over this library.
+ *   HIVE_SEQUENCE_GRAPH: the resize points are fixed up front from an upper
+ *     bound of the count (every insert adds an entry), the target tables are
+ *     allocated and cleared before capture, and the whole sequence -- rehash
+ *     kernels included -- is captured once into a hipGraph and launched.
+ * The report gives the wall time of the sequence (setup_ms: graph allocation,
+ * capture and instantiation, outside elapsed_ms).  Synchronizes. */
+enum { HIVE_SEQUENCE_PLAIN = 0, HIVE_SEQUENCE_GRAPH = 1 };
+typedef struct {
+  double elapsed_ms;
+  double setup_ms;
+  uint32_t resizes;
+  uint64_t final_slots;
+  uint64_t rehash_failed;   /* entries a rehash could not place (then HIVE_ECAPACITY) */
+} hive_sequence_report_t;
+hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* keys, const uint32_t* vals, size_t n,
+                                   size_t batch_keys, const uint32_t* lookup_keys, size_t lookups_per_batch,
+                                   double max_load_factor, double growth, int mode, uint8_t* status,
+                                   uint8_t* lookup_status, hive_sequence_report_t* report, void* stream);
+
 /* ---- maintenance
  * hive_resize: rehash every live entry (buckets and stash) into
  *   new_num_slots / 8 buckets; replaces hive_hash_grow / hive_hash_shrink

@@ -250,6 +250,34 @@ def test_resize_growth_sequence(H, O):
     assert t.num_buckets == slots // 8
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_insert_sequence_with_growth(H, O, mode):
+    # cfg3 shape at small scale through hive_insert_sequence: plain per-batch
+    # calls (host-side resize decisions) or one hipGraph with the rehashes inside
+    print(f"[case] insert sequence mode={mode}", flush=True)
+    nbatch, bsz = 48, 512
+    keys = W.distinct_keys(nbatch * bsz, 61)
+    vals = W.values(keys.size, 62)
+    rng = np.random.default_rng(63)
+    lk = np.concatenate([keys[rng.integers(0, (b + 1) * bsz, size=256)] for b in range(nbatch)])
+    import torch
+    st = torch.empty(keys.size, dtype=torch.uint8, device="cuda")
+    ls = torch.empty(lk.size, dtype=torch.uint8, device="cuda")
+    t = H.HiveTable(1 << 12)
+    rep = t.insert_sequence(keys, bsz, vals=vals, lookup_keys=lk, lookups_per_batch=256, max_load_factor=0.9,
+                            growth=2.0, mode=mode, status=st, lookup_status=ls)
+    print(f"[case] insert sequence report {rep}", flush=True)
+    assert rep["resizes"] == 3 and rep["final_slots"] == 1 << 15 and rep["rehash_failed"] == 0
+    assert placed(np_(st)).all() and (np_(ls) == 4).all()
+    assert t.num_buckets == (1 << 15) // 8 and t.stats()["capacity"] == 1 << 15
+    o = O.Oracle((1 << 15) // 8)
+    o.insert(keys, vals)
+    assert np.array_equal(np.sort(np_(t.dump()).view(np.uint64)), np.sort(o.dump()))
+    check_gpu_state(t, O, "Default2Hash")
+    gv, gs = t.lookup(keys)
+    assert (np_(gs) == 4).all() and np.array_equal(np_(gv, np.uint32), vals)
+
+
 def test_dump_is_the_content_multiset(H, O):
     print("[case] dump", flush=True)
     nb = 1 << 10
