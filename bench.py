@@ -228,11 +228,13 @@ def serve_leg(a, H, W, torch, dev):
                                          fresh=fresh)
     res = {"ops": int(off[-1])}
     slots_d = int(math.ceil(n_pre / 0.899 / 8)) * 8
-    for name, arrivals in (("open", arr), ("saturated", np.zeros_like(arr))):
+    for name, arrivals, cap in (("open", arr, 0), ("saturated", np.zeros_like(arr), 0),
+                                ("per_request_launches", np.zeros_like(arr), 1)):
         t = serving_table(slots_d)
         res["initial_slots"] = slots_d
-        res[name] = run(t, ty, k, v, off, arrivals)
+        res[name] = run(t, ty, k, v, off, arrivals, max_batch_ops=cap)
         t.close()
+    res["coalescing_speedup"] = round(res["saturated"]["mops"] / max(res["per_request_launches"]["mops"], 1e-9), 2)
     out["mixes"]["D"] = res
     log(f"serve D: {res}")
     return out
