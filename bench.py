@@ -92,6 +92,36 @@ def traffic_from_profiles(lf):
         return None
 
 
+def request_counts_from_profiles():
+    """Memory-side requests per launch of the headline kernels (committed PMC summary)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "request_counts_bulk_2p26.json")) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def request_roofline(counts, ins_ms, look_ms, gather_rate, cas_rate):
+    """Kernel time against the time its counted memory-side requests take at
+    the same-footprint ceilings measured in this run (reads at the gather
+    rate, atomics at the CAS rate)."""
+    if not counts:
+        return None
+    out = {"source": "profiles/request_counts_bulk_2p26.json (rocprofv3 TCC_EA0_RDREQ_sum / TCC_EA0_ATOMIC_sum)"}
+    lk = counts["lookup"]
+    t_lk = lk["ea_reads"] / gather_rate * 1e3
+    out["lookup"] = {"ea_reads": lk["ea_reads"], "l2_hit_rate": lk["l2_hit_rate"],
+                     "achieved_reads_per_s": round(lk["ea_reads"] / (look_ms / 1e3), 1),
+                     "bound_ms": round(t_lk, 4), "frac": round(t_lk / look_ms, 4)}
+    ins = counts["insert"]
+    t_r, t_a = ins["ea_reads"] / gather_rate * 1e3, ins["ea_atomics"] / cas_rate * 1e3
+    out["insert"] = {"ea_reads": ins["ea_reads"], "ea_atomics": ins["ea_atomics"], "l2_hit_rate": ins["l2_hit_rate"],
+                     "achieved_atomics_per_s": round(ins["ea_atomics"] / (ins_ms / 1e3), 1),
+                     "overlapped_bound_ms": round(max(t_r, t_a), 4), "additive_bound_ms": round(t_r + t_a, 4),
+                     "frac_overlapped": round(max(t_r, t_a) / ins_ms, 4), "frac_additive": round((t_r + t_a) / ins_ms, 4)}
+    return out
+
+
 def clean_ops(types, keys, off):
     """Ops with exactly one admissible outcome: the key occurs once in its batch
     and had no concurrent inserts in an earlier batch."""
@@ -558,6 +588,7 @@ def main():
             "algorithmic_bytes": {"insert": int(ins_bytes), "lookup": int(look_bytes)},
             "achieved_gbps": {"insert": round(ins_gbps, 2), "lookup": round(look_gbps, 2)},
         },
+        "request_roofline": request_roofline(request_counts_from_profiles(), ins_ms, look_ms, gather_rate, cas_rate),
         "ceiling": {
             "random_gather_64B_per_s": round(gather_rate, 1),
             "random_cas_u64_per_s": round(cas_rate, 1),
