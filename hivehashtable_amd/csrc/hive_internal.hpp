@@ -49,6 +49,8 @@ namespace hive {
 inline size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
 inline size_t counter_words() { return static_cast<size_t>(3) * kCounterLanes * kCounterStride; }
 inline unsigned grid_for(size_t n, uint32_t block) { return static_cast<unsigned>((n + block - 1) / block); }
+// one launch covers at most 2^31 - 1 blocks
+inline bool fits_one_launch(size_t n, uint32_t block) { return (n + block - 1) / block <= 0x7FFFFFFFull; }
 inline bool valid_block(uint32_t b) { return b >= 64 && b <= static_cast<uint32_t>(kMaxBlock) && b % 64 == 0; }
 inline unsigned fill_grid(size_t n) {
   size_t g = (n + 255) / 256;
@@ -72,6 +74,7 @@ hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase
                             BatchClock clk = BatchClock{nullptr, nullptr}) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_INSERT].block_threads;
+  if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
   if (t->launch[OP_INSERT].combine_duplicates) {
     HIVE_POLICY_SWITCH(t->policy, (k_insert_combined<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                     phase, clk)));
@@ -90,6 +93,7 @@ template <class Src, class Out>
 hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
+  if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
   HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
@@ -99,6 +103,7 @@ template <class Src, class Out>
 hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_DELETE].block_threads;
+  if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
   HIVE_POLICY_SWITCH(t->policy, (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
