@@ -106,9 +106,15 @@ int slow_insert(Table& t, uint32_t key, uint32_t value) {
       }
     }
     if (retry) continue;
-    // Stage B (a slot may have freed since the fast path)
+    // Stage B (a slot may have freed since the fast path).  An occupied slot
+    // is checked for the key too: a concurrent fast-path insert of the same
+    // key may have landed after Stage A read the bucket.
     for (int s = 0; s < S; ++s) {
-      if (ld(&cb[s]) != 0) continue;
+      const uint64_t cur = ld(&cb[s]);
+      if (cur != 0) {
+        if (kkey(cur) == key) { retry = true; break; }
+        continue;
+      }
       uint64_t seen = cas(&cb[s], 0, kv);
       if (seen == 0) { result = HO_R_INSERTED; goto done; }
       if (kkey(seen) == key) { retry = true; break; }
@@ -128,8 +134,16 @@ int slow_insert(Table& t, uint32_t key, uint32_t value) {
         if (cas(&ab[s], 0, victim) == 0) { d = s; break; }
       }
       if (d < 0) continue;
-      if (cas(&cb[vs], victim, kv) == victim) { result = HO_R_INSERTED; goto done; }
-      // the victim changed (concurrent update): withdraw the copy, retry
+      // swap kv into the source slot; a concurrent update of the victim is
+      // carried into the copy, so the claimed slot never empties again
+      for (;;) {
+        const uint64_t seen = cas(&cb[vs], victim, kv);
+        if (seen == victim) { result = HO_R_INSERTED; goto done; }
+        if (seen == 0 || kkey(seen) != kkey(victim)) break;
+        if (cas(&ab[d], victim, seen) != victim) break;
+        victim = seen;
+      }
+      // the source stopped holding the victim: withdraw the copy, retry
       for (uint64_t cur = victim;;) {
         uint64_t seen = cas(&ab[d], cur, 0);
         if (seen == cur || seen == 0 || kkey(seen) != kkey(victim)) break;
@@ -168,8 +182,15 @@ int insert_one(Table& t, uint32_t key, uint32_t value) {
   }
   if (ld32(&t.stash_count[c]) != 0) return slow_insert(t, key, value);
   uint64_t* cb = t.slots + static_cast<size_t>(c) * S;
+  // claim the lowest empty H0 slot; a slot that holds the key now (a
+  // concurrent insert of the same key landed after Stage A) sends the op to
+  // the locked path, which updates it
   for (int s = 0; s < S; ++s) {
-    if (ld(&cb[s]) != 0) continue;
+    const uint64_t cur = ld(&cb[s]);
+    if (cur != 0) {
+      if (kkey(cur) == key) return slow_insert(t, key, value);
+      continue;
+    }
     uint64_t seen = cas(&cb[s], 0, kv);
     if (seen == 0) return HO_R_INSERTED;
     if (kkey(seen) == key) return slow_insert(t, key, value);

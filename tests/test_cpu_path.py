@@ -95,3 +95,35 @@ def test_cpu_path_is_race_free_under_tsan():
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-2000:]
     assert r.returncode == 0, r.stderr[-2000:]
     assert "all checks passed" in r.stdout
+
+
+def test_cpu_duplicate_heavy_inserts_store_each_key_once(O):
+    """Many threads inserting copies of the same keys at once: every key ends
+    up stored exactly once (the claim of an empty slot re-checks the key),
+    with one of the values written for it."""
+    slots = 1 << 12
+    rng = np.random.default_rng(11)
+    for rep in range(12):
+        uniq = W.distinct_keys(300, 100 + rep)
+        keys = np.repeat(uniq, 48)
+        vals = (np.arange(keys.size, dtype=np.uint64) % 7 + 1).astype(np.uint32) + keys
+        perm = rng.permutation(keys.size)
+        keys, vals = keys[perm], vals[perm]
+        c = O.CpuTable(slots // 8, threads=8)
+        st, _ = c.insert(keys, vals)
+        assert np.isin(st, [O.R_INSERTED, O.R_STASHED, O.R_UPDATED]).all()
+        raw = c.export()
+        chk = O.Oracle(slots // 8)
+        chk.load(raw)
+        r, msg = chk.check_invariants()
+        assert r == 0, msg
+        kv = raw.live_kv()
+        assert np.array_equal(np.sort((kv & 0xFFFFFFFF).astype(np.uint32)), np.sort(uniq))
+        got = dict(zip((kv & 0xFFFFFFFF).tolist(), (kv >> 32).tolist()))
+        written = {}
+        for k, v in zip(keys.tolist(), vals.tolist()):
+            written.setdefault(k, set()).add(v)
+        assert all(got[k] in written[k] for k in got)
+        # exactly one INSERTED/STASHED per key
+        fresh = np.isin(st, [O.R_INSERTED, O.R_STASHED])
+        assert np.array_equal(np.sort(keys[fresh]), np.sort(uniq))
