@@ -49,11 +49,12 @@ class Stats(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32), ("inline_slow_path", C.c_uint32),
-                ("combine_duplicates", C.c_uint32)]
+                ("combine_duplicates", C.c_uint32), ("split_by_type", C.c_uint32)]
 
 
 class Tune(C.Structure):
-    _fields_ = [("block_threads", C.c_uint32), ("inline_slow_path", C.c_uint32), ("tile_ops", C.c_uint32),
+    _fields_ = [("block_threads", C.c_uint32), ("inline_slow_path", C.c_uint32), ("split_by_type", C.c_uint32),
+                ("tile_ops", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int),
                 ("launches_ms", C.c_double), ("graph_ms", C.c_double), ("persistent_ms", C.c_double),
                 ("candidates", C.c_uint32)]

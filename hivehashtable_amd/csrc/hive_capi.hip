@@ -130,6 +130,7 @@ hive_status_t hive_destroy(hive_table_t* t) {
   (void)hipDeviceSynchronize();
   free_view(t->v);
   if (t->scratch) (void)hipFree(t->scratch);
+  if (t->part) (void)hipFree(t->part);
   delete t;
   return HIVE_OK;
 }
@@ -173,8 +174,9 @@ hive_status_t hive_mixed(hive_table_t* t, const uint8_t* types, const uint32_t* 
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK) return r;
   if (n && (!keys || !types)) return fail(HIVE_EINVAL, "null keys or types");
-  return launch_mixed(t, SrcNative{keys, vals, types}, n, OutNative{status, out_vals},
-                      static_cast<hipStream_t>(stream));
+  if ((r = set_device(t)) != HIVE_OK || (r = ensure_partition(t, n)) != HIVE_OK) return r;
+  return launch_mixed_native(t, SrcNative{keys, vals, types}, n, OutNative{status, out_vals},
+                             static_cast<hipStream_t>(stream));
 }
 
 hive_status_t hive_insert_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results,
@@ -422,6 +424,7 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (c.keys_per_lane != 1) return fail(HIVE_EINVAL, "keys_per_lane must be 1");
   if (c.inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
   if (c.combine_duplicates > 1) return fail(HIVE_EINVAL, "combine_duplicates must be 0 or 1");
+  if (c.split_by_type > 1) return fail(HIVE_EINVAL, "split_by_type must be 0 or 1");
   t->launch[op_type] = c;
   return HIVE_OK;
 }

@@ -37,11 +37,14 @@ struct hive_table {
   int policy = 0;
   hive::TableView v{};
   size_t bytes = 0;
-  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0, 0}, {hive::kDefaultBlock, 1, 0, 0}, {hive::kDefaultBlock, 1, 0, 0}};
+  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0, 0, 1}, {hive::kDefaultBlock, 1, 0, 0, 1},
+                             {hive::kDefaultBlock, 1, 0, 0, 1}};
   unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
   uint32_t stream_tile = 0;              // persistent kernel: ops per tile (0 = sized per batch)
   uint32_t stream_blocks_per_cu = 0;     // persistent kernel: resident blocks per CU (0 = default)
   uint32_t stream_inline = 0;            // persistent kernel: insert slow path inline (1) or staged (0)
+  uint32_t* part = nullptr;              // mixed-batch type lists: 3 x (keys, values, indices) + 3 counts
+  size_t part_cap = 0;                   // ops the lists can hold
 };
 
 namespace hive {
@@ -117,6 +120,52 @@ hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_
   if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;
   if ((r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
   return launch_insert(t, src, n, out, OP_INSERT, s, clk);
+}
+
+// In mixed batches of at least kCompactMinOps ops the inserts and deletes are
+// compacted first (k_partition), so that their phase kernels run on dense
+// lists instead of every wavefront of the batch carrying a few insert lanes
+// through the whole insert chain; lookups keep the type-filtered kernel.
+constexpr size_t kCompactMinOps = size_t(1) << 18;
+
+// type-list scratch for batches of up to n ops (allocated outside any capture)
+inline hive_status_t ensure_partition(hive_table* t, size_t n) {
+  if (n < kCompactMinOps || n > 0xFFFFFFFFull || n <= t->part_cap) return HIVE_OK;
+  uint32_t* p = nullptr;
+  if (hipMalloc(&p, (9 * n + 4) * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return HIVE_OK;  // no scratch: batches run unsplit
+  }
+  if (t->part) {
+    (void)hipDeviceSynchronize();  // the old lists may still be in use by queued work
+    (void)hipFree(t->part);
+  }
+  t->part = p;
+  t->part_cap = n;
+  return HIVE_OK;
+}
+
+// a native mixed batch: split by type when it is large and scratch is there
+inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n, OutNative out, hipStream_t s,
+                                         BatchClock clk = BatchClock{nullptr, nullptr}) {
+  if (!t->launch[OP_INSERT].split_by_type || n < kCompactMinOps || n > t->part_cap || !src.types)
+    return launch_mixed(t, src, n, out, s, clk);
+  uint32_t* lkeys = t->part;
+  uint32_t* lvals = t->part + 3 * n;
+  uint32_t* lidx = t->part + 6 * n;
+  uint32_t* cnt = t->part + 9 * n;
+  k_fill_u32<<<1, 64, 0, s>>>(cnt, 3, 0u);
+  k_partition<<<grid_for(n, 1024), 1024, 0, s>>>(src.types, src.keys, src.vals, n, lkeys, lvals, lidx, cnt);
+  HIP_CHECK_LAUNCH();
+  hive_status_t r;
+  auto list = [&](int ty) {
+    const size_t o = static_cast<size_t>(ty) * n;
+    return SrcList{lkeys + o, lvals + o, cnt + ty, ty};
+  };
+  auto outs = [&](int ty) { return OutIdx{out.status, out.vals, lidx + static_cast<size_t>(ty) * n}; };
+  if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;  // also answers NONE ops
+  if ((r = launch_erase(t, list(OP_DELETE), n, outs(OP_DELETE), OP_DELETE, s)) != HIVE_OK) return r;
+  return launch_insert(t, list(OP_INSERT), n, outs(OP_INSERT), OP_INSERT, s, clk);
 }
 
 // device buffers freed together when a call returns

@@ -9,6 +9,8 @@
 // on densely packed wavefronts instead of one lane per wave.
 #pragma once
 
+#include <type_traits>
+
 #include "hive_device.hpp"
 
 namespace hive {
@@ -37,11 +39,53 @@ struct SrcKV {  // packed kv records (stash re-insert)
   __device__ __forceinline__ int type(size_t, int dflt) const { return dflt; }
 };
 
+template <class Src>
+struct requires_bound : std::false_type {};
+
+// One op type of a mixed batch, compacted by k_partition: list entry j holds
+// the key and value of batch op idx[j].  Entries at or past *cnt do not exist
+// and report another valid type, so every phase kernel skips them without
+// touching the table or the outputs.  Keys and values are stored in the list
+// (not fetched through idx), so a lane's first load is its own key, as in an
+// uncompacted batch.  own = the list's type.
+struct SrcList {
+  const uint32_t* keys;
+  const uint32_t* vals;  // NULL: value = key + 1
+  const uint32_t* cnt;
+  int own;
+  __device__ __forceinline__ size_t bound() const { return *cnt; }  // blocks at or past it have nothing to do
+  __device__ __forceinline__ uint32_t key(size_t j) const { return keys[j]; }
+  __device__ __forceinline__ uint32_t val(size_t j, uint32_t k) const { return vals ? vals[j] : k + 1u; }
+  __device__ __forceinline__ int type(size_t j, int) const { return j < *cnt ? own : (own + 1) % 3; }
+};
+
+template <>
+struct requires_bound<SrcList> : std::true_type {};
+
+// Whole blocks past a compacted list's length return at once (after their
+// batch-clock arrival); other sources have no such bound.
+template <class Src>
+__device__ __forceinline__ bool block_past_list(const Src& src) {
+  if constexpr (requires_bound<Src>::value) return static_cast<size_t>(blockIdx.x) * blockDim.x >= src.bound();
+  else return false;
+}
+
 // ---------------------------------------------------------------- outputs
 struct OutNative {
   uint8_t* status;
   uint32_t* vals;
   __device__ __forceinline__ void put(size_t i, uint8_t st, uint32_t v) const {
+    if (status) status[i] = st;
+    if (vals) vals[i] = v;
+  }
+};
+
+struct OutIdx {  // results of a compacted list, written at the batch op's index
+  uint8_t* status;
+  uint32_t* vals;
+  const uint32_t* idx;
+  __device__ __forceinline__ void put(size_t j, uint8_t st, uint32_t v) const {
+    const uint32_t i = idx[j];
     if (status) status[i] = st;
     if (vals) vals[i] = v;
   }
@@ -117,6 +161,10 @@ struct BatchClock {
 template <class P, class Src, class Out, bool Inline>
 __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size_t n, Out out, int phase,
                                                       BatchClock clk = BatchClock{nullptr, nullptr}) {
+  if (block_past_list(src)) {
+    clk.finish();
+    return;
+  }
   const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
   const size_t i = base + threadIdx.x;
   int dtab = 0, dstash = 0, dslow = 0;
@@ -174,6 +222,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src src, size_t n, Out out, int phase,
                                                                BatchClock clk = BatchClock{nullptr, nullptr}) {
+  if (block_past_list(src)) {
+    clk.finish();
+    return;
+  }
   constexpr uint32_t kTab = 2 * kMaxBlock;  // at most half full
   __shared__ unsigned long long tab[kTab];  // key << 32 | (lane + 1), 0 = empty
   __shared__ uint32_t followers[kMaxBlock];
@@ -229,6 +281,57 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src 
   clk.finish();
 }
 
+// ---------------------------------------------------------------- partition
+// Compact the inserts and deletes of a mixed batch (n < 2^32): list t
+// (OP_INSERT, OP_DELETE) occupies entries [t * n, t * n + cnt[t]) of lkeys /
+// lvals / lidx (the op's key, value and batch index); cnt is zero on entry.
+// Lookups stay in the batch (their phase kernel filters by type; they are the
+// bulk of read-heavy batches, where a copy would only add traffic).  Order
+// within a list is arbitrary: a phase's ops are concurrent.
+static __global__ __launch_bounds__(1024) void k_partition(const uint8_t* types, const uint32_t* keys,
+                                                          const uint32_t* vals, size_t n, uint32_t* lkeys,
+                                                          uint32_t* lvals, uint32_t* lidx, uint32_t* cnt) {
+  __shared__ uint32_t c[3], base[3];
+  if (threadIdx.x < 3) c[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int ty = -1;
+  if (i < n) {
+    ty = static_cast<int>(types[i]);
+    if (ty != OP_INSERT && ty != OP_DELETE) ty = -1;
+  }
+  // ranks within the wavefront by ballot; one LDS atomic per wavefront and list
+  const uint32_t lane = threadIdx.x & 63u;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mi = __ballot(ty == OP_INSERT), md = __ballot(ty == OP_DELETE);
+  uint32_t wi = 0, wd = 0;
+  if (mi) {
+    const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(mi)) - 1);
+    uint32_t x = 0;
+    if (lane == lead) x = atomicAdd(&c[OP_INSERT], static_cast<uint32_t>(__popcll(mi)));
+    wi = __shfl(x, static_cast<int>(lead));
+  }
+  if (md) {
+    const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(md)) - 1);
+    uint32_t x = 0;
+    if (lane == lead) x = atomicAdd(&c[OP_DELETE], static_cast<uint32_t>(__popcll(md)));
+    wd = __shfl(x, static_cast<int>(lead));
+  }
+  __syncthreads();
+  if ((threadIdx.x == OP_INSERT || threadIdx.x == OP_DELETE) && c[threadIdx.x])
+    base[threadIdx.x] = atomicAdd(cnt + threadIdx.x, c[threadIdx.x]);
+  __syncthreads();
+  if (ty >= 0) {
+    const bool ins = ty == OP_INSERT;
+    const size_t pos = static_cast<size_t>(ty) * n + base[ty] + (ins ? wi : wd) +
+                       static_cast<uint32_t>(__popcll((ins ? mi : md) & below));
+    const uint32_t key = keys[i];
+    lkeys[pos] = key;
+    lvals[pos] = vals ? vals[i] : key + 1u;
+    lidx[pos] = static_cast<uint32_t>(i);
+  }
+}
+
 // ---------------------------------------------------------------- lookup
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size_t n, Out out, int phase) {
@@ -250,6 +353,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
 // ---------------------------------------------------------------- erase
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_t n, Out out, int phase) {
+  if (block_past_list(src)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int rt = 0, rs = 0;
   const uint32_t key = i < n ? src.key(i) : 0u;  // requested together with the op type

@@ -123,6 +123,10 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     HIP_TRY(hipMemcpyAsync(h_ins.data(), d_cnt, nreq * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
   }
+  uint64_t max_req = 0;
+  for (size_t i = 0; i < nreq; ++i) max_req = std::max<uint64_t>(max_req, h_req_offsets[i + 1] - h_req_offsets[i]);
+  if ((r = ensure_partition(t, std::min<uint64_t>(nops, std::max<uint64_t>(c.max_batch_ops, max_req)))) != HIVE_OK)
+    return r;
   hive_stats_t st{};
   if ((r = hive_stats(t, &st, stream)) != HIVE_OK) return r;
   uint64_t capacity = st.capacity;
@@ -229,7 +233,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       const size_t o = h_req_offsets[launched], n = h_req_offsets[end] - o;
       SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
       OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
-      if ((r = launch_mixed(t, src, n, out, s)) != HIVE_OK) return r;
+      if ((r = launch_mixed_native(t, src, n, out, s)) != HIVE_OK) return r;
       hipEvent_t ev = pool.free.back();
       pool.free.pop_back();
       HIP_TRY(hipEventRecord(ev, s));

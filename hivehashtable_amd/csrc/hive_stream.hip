@@ -231,7 +231,7 @@ hive_status_t enqueue_launches(hive_table_t* t, const uint8_t* types, const uint
     }
     SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
     OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
-    hive_status_t r = launch_mixed(t, src, n, out, s, BatchClock{ctrs + b, ts + b + 1});
+    hive_status_t r = launch_mixed_native(t, src, n, out, s, BatchClock{ctrs + b, ts + b + 1});
     if (r != HIVE_OK) return r;
   }
   return HIVE_OK;
@@ -278,6 +278,9 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
   HIP_TRY(hipMemsetAsync(ts, 0, (nbatches + 1) * 8, s));
   const auto t0 = std::chrono::steady_clock::now();
   if (mode == HIVE_STREAM_LAUNCHES || mode == HIVE_STREAM_GRAPH) {
+    uint64_t maxn = 0;
+    for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
+    if ((r = ensure_partition(t, maxn)) != HIVE_OK) return r;  // before any capture
     unsigned int* ctrs = nullptr;
     HIP_TRY(ar.alloc(&ctrs, nbatches));
     HIP_TRY(hipMemsetAsync(ctrs, 0, nbatches * 4, s));
@@ -419,6 +422,9 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   if (!out || !h_offsets || nbatches == 0) return fail(HIVE_EINVAL, "autotune needs a sample stream and an output");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t nops = h_offsets[nbatches];
+  uint64_t maxn = 0;
+  for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
+  if ((r = ensure_partition(t, maxn)) != HIVE_OK) return r;  // shared with the scratch copy, never reallocated by it
   hive_table tmp = *t;  // same policy and device, scratch copy of the table state
   if ((r = alloc_view(tmp.v, t->v.nb)) != HIVE_OK) return r;
   struct ViewGuard {
@@ -444,19 +450,25 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   best.launches_ms = best.graph_ms = best.persistent_ms = 1e300;
   uint32_t cands = 0;
   const uint32_t blocks[] = {128, 256, 512, 1024};
-  for (uint32_t inl = 0; inl < 2; ++inl)
-    for (uint32_t blk : blocks) {
-      for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates};
-      double ms = 0;
-      if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
-      ++cands;
-      if (ms < best.launches_ms) {
-        best.launches_ms = ms;
-        best.block_threads = blk;
-        best.inline_slow_path = inl;
+  const bool any_big = maxn >= kCompactMinOps;  // the split only applies to big batches
+  for (uint32_t split = any_big ? 0u : 1u; split < 2; ++split)
+    for (uint32_t inl = 0; inl < 2; ++inl)
+      for (uint32_t blk : blocks) {
+        for (int op = 0; op < 3; ++op)
+          tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates, split};
+        double ms = 0;
+        if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
+        ++cands;
+        if (ms < best.launches_ms) {
+          best.launches_ms = ms;
+          best.block_threads = blk;
+          best.inline_slow_path = inl;
+          best.split_by_type = split;
+        }
       }
-    }
-  for (int op = 0; op < 3; ++op) tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates};
+  for (int op = 0; op < 3; ++op)
+    tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
+                                   best.split_by_type};
   if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms)) != HIVE_OK) return r;
   ++cands;
   const uint32_t tiles[] = {0, 256, 1024, 4096};
@@ -481,7 +493,9 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   best.best_mode = HIVE_STREAM_PERSISTENT;
   if (best.launches_ms < best.persistent_ms && best.launches_ms <= best.graph_ms) best.best_mode = HIVE_STREAM_LAUNCHES;
   else if (best.graph_ms < best.persistent_ms) best.best_mode = HIVE_STREAM_GRAPH;
-  for (int op = 0; op < 3; ++op) t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates};
+  for (int op = 0; op < 3; ++op)
+    t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
+                                  best.split_by_type};
   t->stream_tile = best.tile_ops;
   t->stream_blocks_per_cu = best.blocks_per_cu;
   t->stream_inline = best.stream_inline;

@@ -289,14 +289,15 @@ class HiveTable:
         check(self.L.hive_import_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
                                      cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_import_raw")
 
-    def set_launch(self, op_type, block_threads, keys_per_lane=1, inline_slow_path=0, combine_duplicates=0):
-        cfg = Launch(block_threads, keys_per_lane, inline_slow_path, combine_duplicates)
+    def set_launch(self, op_type, block_threads, keys_per_lane=1, inline_slow_path=0, combine_duplicates=0,
+                   split_by_type=1):
+        cfg = Launch(block_threads, keys_per_lane, inline_slow_path, combine_duplicates, split_by_type)
         check(self.L.hive_set_launch(self.h, op_type, C.byref(cfg)), "hive_set_launch")
 
     def get_launch(self, op_type):
         cfg = Launch()
         check(self.L.hive_get_launch(self.h, op_type, C.byref(cfg)), "hive_get_launch")
-        return cfg.block_threads, cfg.keys_per_lane, cfg.inline_slow_path, cfg.combine_duplicates
+        return cfg.block_threads, cfg.keys_per_lane, cfg.inline_slow_path, cfg.combine_duplicates, cfg.split_by_type
 
 
 def make_ops(types, keys):

@@ -96,6 +96,10 @@ typedef struct {
   uint32_t combine_duplicates; /* inserts: 1 = ops of one block on the same key are
                                   combined into one table update before reaching
                                   memory (skewed batches; slow path inline) */
+  uint32_t split_by_type;      /* mixed batches (read from the insert entry): 1 = in
+                                  batches of >= 2^18 ops, inserts and deletes are
+                                  compacted into dense lists before their phase
+                                  kernels; 0 = every phase filters the whole batch */
 } hive_launch_t;
 
 /* ---- lifecycle: replaces hiveHashTableCreate / Destroy / Clear
@@ -164,13 +168,15 @@ hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t
 /* Autotuner for the stream forms (no reference counterpart): times the
  * sample stream on a scratch copy of the table -- so at the table's current
  * load factor, leaving the table unchanged -- for per-batch launches at
- * 128..1024 threads per block x {staged, inline} insert slow path, the
+ * 128..1024 threads per block x {staged, inline} insert slow path (x split
+ * by type or not, when the sample has batches of >= 2^18 ops), the
  * captured graph, and the persistent kernel over tile sizes {auto, 256, 1024,
  * 4096} x {2, 4, 8} blocks per CU x {staged, inline}, then stores the fastest
  * launch shapes in the handle.  Synchronizes. */
 typedef struct {
   uint32_t block_threads;  /* chosen threads per block of the plain kernels */
   uint32_t inline_slow_path; /* chosen insert slow-path placement of the plain kernels */
+  uint32_t split_by_type;  /* chosen: big batches' inserts and deletes compacted (1) or not (0) */
   uint32_t tile_ops;       /* chosen persistent tile (0 = sized per batch) */
   uint32_t blocks_per_cu;  /* chosen persistent residency */
   uint32_t stream_inline;  /* chosen insert slow-path placement of the persistent kernel */

@@ -222,6 +222,45 @@ def test_mixed_batches_match_phased_oracle(H, O, combine):
             assert ka[kv & 0xFFFFFFFF] == kv >> 32
 
 
+def test_big_mixed_batches_split_by_type(H, O):
+    # batches of >= 2^18 ops are split by op type before their phase kernels
+    print("[case] big mixed batches (type-split path)", flush=True)
+    slots = 1 << 20
+    pre = W.distinct_keys(slots // 2, 35)
+    sizes = np.array([300_000, 1 << 18, 5_000, 400_000])
+    types, keys, vals, off = W.mixed_stream(pre, 1 << 22, sizes, seed=36)
+    rng = np.random.default_rng(37)
+    types[rng.random(types.size) < 0.01] = 4  # NONE ops: NOOP, value 0
+    clean = _clean_ops(types, keys, off)
+    o = O.Oracle(slots // 8)
+    o.insert(pre)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for b in range(len(sizes)):
+        lo, hi = int(off[b]), int(off[b + 1])
+        ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    norm = lambda s: np.where(s == 3, 1, s)
+    keyset = np.sort(o.dump() & 0xFFFFFFFF)
+    for how, split in (("mixed", 1), ("launches", 1), ("graph", 1), ("mixed", 0)):
+        t = H.HiveTable(slots)
+        t.set_launch(0, 256, split_by_type=split)
+        t.insert(pre)
+        if how == "mixed":
+            outs = [t.mixed(types[int(off[b]):int(off[b + 1])], keys[int(off[b]):int(off[b + 1])],
+                            vals[int(off[b]):int(off[b + 1])]) for b in range(len(sizes))]
+            gv = np.concatenate([np_(v, np.uint32) for v, _ in outs])
+            gs = np.concatenate([np_(s_) for _, s_ in outs])
+        else:
+            gv, gs, done, _ = t.mixed_stream(types, keys, vals, off, 0 if how == "launches" else 1)
+            gv, gs = np_(gv, np.uint32), np_(gs)
+        assert (gs[types == 4] == 8).all() and (gv[types == 4] == 0).all(), how
+        assert np.array_equal(norm(gs)[clean], norm(os_)[clean]), how
+        assert np.array_equal(gv[clean], ov[clean]), how
+        live = np.sort(np_(t.dump()).view(np.uint64) & np.uint64(0xFFFFFFFF))
+        assert np.array_equal(live, keyset), how
+        t.close()
+
+
 def test_resize_growth_sequence(H, O):
     # cfg3 shape at small scale: grow x2 whenever the next batch would exceed LF 0.9
     print("[case] resize sequence", flush=True)
@@ -368,7 +407,7 @@ def test_launch_shapes_give_identical_results(H, O):
         t = H.HiveTable(slots)
         for op in (0, 1, 2):
             t.set_launch(op, blk, inline_slow_path=inl, combine_duplicates=comb)
-        assert t.get_launch(1) == (blk, 1, inl, comb)
+        assert t.get_launch(1) == (blk, 1, inl, comb, 1)
         t.insert(keys)
         gv, gs = t.lookup(lk)
         cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
@@ -486,4 +525,4 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     assert res["stream_inline"] in (0, 1)
     assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
-    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0)
+    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"])
