@@ -237,11 +237,12 @@ def serve_leg(a, H, W, torch, dev):
     def serving_table(slots):
         # serving shape: inline insert slow path (latency), duplicates combined per block (Zipf skew)
         t = H.HiveTable(slots, a.policy)
-        t.set_launch(0, 1024, inline_slow_path=1, combine_duplicates=1)
+        t.set_launch(0, 1024, inline_slow_path=1, combine_duplicates=1, split_by_type=1)
         t.insert(d_pre, d_pre_v)
         return t
 
-    out["insert_launch"] = {"block_threads": 1024, "inline_slow_path": 1, "combine_duplicates": 1}
+    out["insert_launch"] = {"block_threads": 1024, "inline_slow_path": 1, "combine_duplicates": 1,
+                            "split_by_type": 1}
     t = serving_table(1 << 27)
     for mix in "ABC":
         ty, k, v, off, arr = W.serving_trace(mix, pre, a.serve_requests, a.serve_offered_mops, a.seed + 5100 + ord(mix))
