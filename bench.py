@@ -146,8 +146,9 @@ def clean_ops(types, keys, off):
 def mixed_leg(a, H, W, torch, dev):
     """2^26 slots preloaded to LF 0.5; a stream of uneven mixed batches
     (log-uniform 2^10..2^20 ops, 50/40/10 insert/lookup/delete over a 2^27 key
-    space) run as per-batch launches, one captured hipGraph, and the
-    persistent work-queue kernel (shapes from the autotuner)."""
+    space) run as per-batch launches, one captured hipGraph, the persistent
+    work-queue kernel, and windows of consecutive batches (shapes and window
+    size from the autotuner)."""
     slots = 1 << a.log2_slots
     t0 = time.time()
     pre = W.distinct_keys(slots // 2, a.seed + 3000, bits=27)  # preload from the same 2^27 key space
@@ -171,13 +172,17 @@ def mixed_leg(a, H, W, torch, dev):
            "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}, "modes": {}}
     ref = None
     agree = True
-    names = {0: "launches", 1: "graph", 2: "persistent"}
-    for mode in (0, 1, 2):
+    names = {0: "launches", 1: "graph", 2: "persistent", 3: "windowed"}
+    for mode in (0, 1, 2, 3):
         t.clear()
         t.insert(d_pre)
         torch.cuda.synchronize()
         gv, gs, done, setup = t.mixed_stream(d_ty, d_k, d_v, off, mode)
-        lat = np.diff(np.concatenate([[0.0], done])) / 1e3  # per-batch service time, us
+        # per-batch service time, us: from the previous completion (of another
+        # batch or window) to this batch's; batches of one window complete together
+        uniq = np.unique(done)
+        pos = np.searchsorted(uniq, done)
+        lat = (done - np.where(pos > 0, uniq[np.maximum(pos - 1, 0)], 0.0)) / 1e3
         total_ms = done[-1] / 1e6
         st = torch.where(gs == 3, torch.ones_like(gs), gs)
         if ref is None:

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "hive.h")
 HIVE_OK, HIVE_EINVAL, HIVE_ENOMEM, HIVE_EDEVICE, HIVE_ECAPACITY, HIVE_ERANGE = range(6)
 R_FAILED, R_INSERTED, R_UPDATED, R_STASHED, R_FOUND, R_NOT_FOUND, R_ERASED, R_INVALID_KEY, R_NOOP = range(9)
 OP_INSERT, OP_LOOKUP, OP_DELETE, OP_ATOMIC_INC, OP_NONE = range(5)
-STREAM_LAUNCHES, STREAM_GRAPH, STREAM_PERSISTENT = range(3)
+STREAM_LAUNCHES, STREAM_GRAPH, STREAM_PERSISTENT, STREAM_WINDOWED = range(4)
 SEQUENCE_PLAIN, SEQUENCE_GRAPH = range(2)
 POLICIES = {
     "Default2Hash": 0,
@@ -57,7 +57,8 @@ class Tune(C.Structure):
                 ("tile_ops", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int),
                 ("launches_ms", C.c_double), ("graph_ms", C.c_double), ("persistent_ms", C.c_double),
-                ("candidates", C.c_uint32)]
+                ("candidates", C.c_uint32), ("window_ops", C.c_uint64), ("window_batch_ops", C.c_uint64),
+                ("windowed_ms", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -127,6 +128,7 @@ def lib():
         "hive_hash_table_kernel_dispatch_ycsb": [p, p, sz, sz, sz, sz, sz, C.POINTER(C.c_double), p, i32, i32],
         "hive_mixed_stream": [p, p, p, p, p, sz, p, p, i32, p, C.POINTER(C.c_double), p],
         "hive_set_stream_shape": [p, u32, u32, u32],
+        "hive_set_stream_window": [p, C.c_uint64, C.c_uint64],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
         "hive_insert_sequence": [p, p, p, sz, sz, p, sz, C.c_double, C.c_double, i32, p, p,

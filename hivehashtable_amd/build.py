@@ -16,7 +16,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhive.so")
-SOURCES = ["hive_capi.hip", "hive_compat.hip", "hive_stream.hip", "hive_serve.hip", "hive_sequence.hip"]
+SOURCES = ["hive_capi.hip", "hive_compat.hip", "hive_stream.hip", "hive_serve.hip", "hive_sequence.hip",
+           "hive_window.hip"]
 ARCH = os.environ.get("HIVE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -131,6 +131,7 @@ hive_status_t hive_destroy(hive_table_t* t) {
   free_view(t->v);
   if (t->scratch) (void)hipFree(t->scratch);
   if (t->part) (void)hipFree(t->part);
+  (void)free_window(t);
   delete t;
   return HIVE_OK;
 }

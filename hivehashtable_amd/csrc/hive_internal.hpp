@@ -45,6 +45,13 @@ struct hive_table {
   uint32_t stream_inline = 0;            // persistent kernel: insert slow path inline (1) or staged (0)
   uint32_t* part = nullptr;              // mixed-batch type lists: 3 x (keys, values, indices) + 3 counts
   size_t part_cap = 0;                   // ops the lists can hold
+  uint64_t stream_window_ops = 0;        // windowed streams: most ops per window (0 = default)
+  uint64_t stream_window_batch = 0;      // windowed streams: batches below this size are coalesced (0 = default)
+  unsigned long long* win_set = nullptr; // windowed streams: key set of the current window, then round words
+  uint32_t* win_buf = nullptr;           // windowed streams: per-op and pending arrays, control block
+  size_t win_cap = 0;                    // key-set entries (a power of two; windows of up to win_cap / 2 ops)
+  uint32_t win_epoch = 0;                // epoch of the last window (entries of other epochs are free)
+  uint64_t win_seq = 0;                  // sequence number of the last window (tags its round words)
 };
 
 namespace hive {
@@ -167,6 +174,25 @@ inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n,
   if ((r = launch_erase(t, list(OP_DELETE), n, outs(OP_DELETE), OP_DELETE, s)) != HIVE_OK) return r;
   return launch_insert(t, list(OP_INSERT), n, outs(OP_INSERT), OP_INSERT, s, clk);
 }
+
+// ---- windowed streams (hive_window.hip)
+constexpr uint64_t kDefaultWindowOps = uint64_t(1) << 21;
+constexpr uint64_t kDefaultWindowBatchOps = uint64_t(1) << 19;
+constexpr size_t kMaxWindowBatches = 4096;
+struct WindowPlan {
+  size_t b0, b1;   // batches [b0, b1)
+  uint32_t t0, t1; // tile descriptors [t0, t1) (multi-batch windows only)
+};
+hive_status_t ensure_window(hive_table* t, size_t ops);
+hive_status_t free_window(hive_table* t);
+hive_status_t plan_windows(const uint64_t* off, size_t nb, uint64_t window_ops, uint64_t batch_ops,
+                           std::vector<WindowPlan>& plan, std::vector<uint4>& desc);
+hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                              const uint64_t* off, uint32_t* out_vals, uint8_t* status, unsigned int* ctrs,
+                              unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
+                              int ncu, unsigned long long wait_ticks, hipStream_t s);
+// the sticky error word of the window control block (nonzero: a round barrier gave up)
+const unsigned int* window_error_word(const hive_table* t);
 
 // device buffers freed together when a call returns
 struct DevArena {

@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
   if (phase >= 0) {
     const int ty = src.type(i, OP_LOOKUP);
     if (ty != OP_LOOKUP) {
-      if (ty != OP_INSERT && ty != OP_DELETE) out.put(i, R_NOOP, 0);
+      if (ty != OP_INSERT && ty != OP_DELETE && ty != OP_DEFERRED) out.put(i, R_NOOP, 0);
       return;
     }
   }

@@ -255,6 +255,17 @@ hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t
   return HIVE_OK;
 }
 
+hive_status_t hive_set_stream_window(hive_table_t* t, uint64_t window_ops, uint64_t batch_ops) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (window_ops && (window_ops < (1u << 10) || window_ops > (1u << 24)))
+    return fail(HIVE_EINVAL, "window_ops must be 0 or 2^10..2^24");
+  if (batch_ops > (1u << 24)) return fail(HIVE_EINVAL, "batch_ops must be 0..2^24");
+  t->stream_window_ops = window_ops;
+  t->stream_window_batch = batch_ops;
+  return HIVE_OK;
+}
+
 hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                 const uint64_t* h_offsets, size_t nbatches, uint32_t* out_vals, uint8_t* status,
                                 int mode, double* batch_done_ns, double* setup_ms, void* stream) {
@@ -267,7 +278,7 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
     if (h_offsets[b + 1] < h_offsets[b]) return fail(HIVE_EINVAL, "offsets must be non-decreasing");
   if (h_offsets[0] != 0) return fail(HIVE_EINVAL, "offsets[0] must be 0");
   if (h_offsets[nbatches] && (!keys || !types)) return fail(HIVE_EINVAL, "null keys or types");
-  if (mode < HIVE_STREAM_LAUNCHES || mode > HIVE_STREAM_PERSISTENT) return fail(HIVE_EINVAL, "unknown stream mode");
+  if (mode < HIVE_STREAM_LAUNCHES || mode > HIVE_STREAM_WINDOWED) return fail(HIVE_EINVAL, "unknown stream mode");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int dev = t->device, khz = 0, ncu = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -277,7 +288,38 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
   HIP_TRY(ar.alloc(&ts, nbatches + 1));
   HIP_TRY(hipMemsetAsync(ts, 0, (nbatches + 1) * 8, s));
   const auto t0 = std::chrono::steady_clock::now();
-  if (mode == HIVE_STREAM_LAUNCHES || mode == HIVE_STREAM_GRAPH) {
+  if (mode == HIVE_STREAM_WINDOWED) {
+    if (h_offsets[nbatches] > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "windowed stream: more than 2^32-1 ops");
+    const uint64_t wops = t->stream_window_ops ? t->stream_window_ops : kDefaultWindowOps;
+    const uint64_t wbatch = t->stream_window_batch ? t->stream_window_batch : kDefaultWindowBatchOps;
+    uint64_t maxn = wops;
+    for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
+    if ((r = ensure_partition(t, maxn)) != HIVE_OK) return r;
+    if ((r = ensure_window(t, wops)) != HIVE_OK) return r;
+    std::vector<WindowPlan> plan;
+    std::vector<uint4> desc;
+    if ((r = plan_windows(h_offsets, nbatches, wops, wbatch, plan, desc)) != HIVE_OK) return r;
+    uint4* d_desc = nullptr;
+    unsigned int* ctrs = nullptr;
+    HIP_TRY(ar.alloc(&d_desc, desc.size()));
+    HIP_TRY(ar.alloc(&ctrs, nbatches));
+    if (!desc.empty())
+      HIP_TRY(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(ctrs, 0, nbatches * 4, s));
+    HIP_TRY(hipMemsetAsync(const_cast<unsigned int*>(window_error_word(t)), 0, 4, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (setup_ms)
+      *setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    k_stamp<<<1, 64, 0, s>>>(ts);
+    HIP_CHECK_LAUNCH();
+    if ((r = enqueue_windows(t, types, keys, vals, h_offsets, out_vals, status, ctrs, ts, plan, d_desc, ncu,
+                             static_cast<unsigned long long>(kWaitLimitSeconds * 1e3 * khz), s)) != HIVE_OK)
+      return r;
+    unsigned int err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, window_error_word(t), 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (err) return fail(HIVE_EDEVICE, "windowed stream: a round barrier did not complete");
+  } else if (mode == HIVE_STREAM_LAUNCHES || mode == HIVE_STREAM_GRAPH) {
     uint64_t maxn = 0;
     for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
     if ((r = ensure_partition(t, maxn)) != HIVE_OK) return r;  // before any capture
@@ -424,7 +466,11 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   const size_t nops = h_offsets[nbatches];
   uint64_t maxn = 0;
   for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
-  if ((r = ensure_partition(t, maxn)) != HIVE_OK) return r;  // shared with the scratch copy, never reallocated by it
+  const uint64_t windows[] = {uint64_t(1) << 20, uint64_t(1) << 21, uint64_t(1) << 22};
+  const uint64_t coalesce[] = {uint64_t(1) << 17, uint64_t(1) << 18, uint64_t(1) << 19};
+  // scratch shared with the copy below, so the copy never reallocates it
+  if ((r = ensure_partition(t, std::max<uint64_t>(maxn, windows[2]))) != HIVE_OK) return r;
+  if ((r = ensure_window(t, windows[2])) != HIVE_OK) return r;
   hive_table tmp = *t;  // same policy and device, scratch copy of the table state
   if ((r = alloc_view(tmp.v, t->v.nb)) != HIVE_OK) return r;
   struct ViewGuard {
@@ -447,7 +493,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
     return HIVE_OK;
   };
   hive_tune_t best{};
-  best.launches_ms = best.graph_ms = best.persistent_ms = 1e300;
+  best.launches_ms = best.graph_ms = best.persistent_ms = best.windowed_ms = 1e300;
   uint32_t cands = 0;
   const uint32_t blocks[] = {128, 256, 512, 1024};
   const bool any_big = maxn >= kCompactMinOps;  // the split only applies to big batches
@@ -489,16 +535,50 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
           best.stream_inline = inl;
         }
       }
+  uint32_t win_split = best.split_by_type;
+  {  // first touch of the window scratch, untimed
+    double ms = 0;
+    tmp.stream_window_ops = windows[2];
+    tmp.stream_window_batch = coalesce[2];
+    r = run(HIVE_STREAM_WINDOWED, &ms);
+    t->win_epoch = tmp.win_epoch;
+    t->win_seq = tmp.win_seq;
+    if (r != HIVE_OK) return r;
+  }
+  for (uint32_t split = 0; split < 2; ++split)
+    for (uint64_t w : windows)
+      for (uint64_t cb : coalesce) {
+        if (cb > w) continue;
+        for (int op = 0; op < 3; ++op)
+          tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path,
+                                         t->launch[op].combine_duplicates, split};
+        tmp.stream_window_ops = w;
+        tmp.stream_window_batch = cb;
+        double ms = 0;
+        r = run(HIVE_STREAM_WINDOWED, &ms);
+        t->win_epoch = tmp.win_epoch;  // the key set is shared: later windows continue its epochs
+        t->win_seq = tmp.win_seq;
+        if (r != HIVE_OK) return r;
+        ++cands;
+        if (ms < best.windowed_ms) {
+          best.windowed_ms = ms;
+          best.window_ops = w;
+          best.window_batch_ops = cb;
+          win_split = split;
+        }
+      }
   best.candidates = cands;
-  best.best_mode = HIVE_STREAM_PERSISTENT;
-  if (best.launches_ms < best.persistent_ms && best.launches_ms <= best.graph_ms) best.best_mode = HIVE_STREAM_LAUNCHES;
-  else if (best.graph_ms < best.persistent_ms) best.best_mode = HIVE_STREAM_GRAPH;
+  const double modes[4] = {best.launches_ms, best.graph_ms, best.persistent_ms, best.windowed_ms};
+  best.best_mode = static_cast<int>(std::min_element(modes, modes + 4) - modes);
+  if (best.best_mode == HIVE_STREAM_WINDOWED) best.split_by_type = win_split;
   for (int op = 0; op < 3; ++op)
     t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
                                   best.split_by_type};
   t->stream_tile = best.tile_ops;
   t->stream_blocks_per_cu = best.blocks_per_cu;
   t->stream_inline = best.stream_inline;
+  t->stream_window_ops = best.window_ops;
+  t->stream_window_batch = best.window_batch_ops;
   *out = best;
   return HIVE_OK;
 }

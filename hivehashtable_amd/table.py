@@ -126,7 +126,8 @@ class HiveTable:
     def mixed_stream(self, types, keys, vals, offsets, mode=2, out_vals=None, status=None, stream=None):
         """Run consecutive mixed batches (batch b = ops[offsets[b]:offsets[b+1]]).
 
-        mode: 0 per-batch launches, 1 captured hipGraph, 2 persistent kernel.
+        mode: 0 per-batch launches, 1 captured hipGraph, 2 persistent kernel,
+        3 windows of consecutive batches (set_stream_window).
         Returns (out_vals, status, batch_done_ns numpy array, setup_ms)."""
         torch = _torch()
         ty = _dev(types, torch.uint8)
@@ -204,6 +205,11 @@ class HiveTable:
     def set_stream_shape(self, tile_ops=0, blocks_per_cu=0, inline_slow_path=0):
         check(self.L.hive_set_stream_shape(self.h, int(tile_ops), int(blocks_per_cu), int(inline_slow_path)),
               "hive_set_stream_shape")
+
+    def set_stream_window(self, window_ops=0, batch_ops=0):
+        """Windowed stream form: batches below batch_ops ops are coalesced into
+        windows of at most window_ops ops (0 = defaults)."""
+        check(self.L.hive_set_stream_window(self.h, int(window_ops), int(batch_ops)), "hive_set_stream_window")
 
     # --------------------------------------------- reference record ops
     def _ops(self, fn, ops, results, stream):

@@ -152,10 +152,17 @@ hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_
  *                          and replayed (capture + instantiate in *setup_ms)
  *   HIVE_STREAM_PERSISTENT one resident kernel that claims tiles of
  *                          (batch, phase) from a device work queue
+ *   HIVE_STREAM_WINDOWED   consecutive batches of up to window_ops ops in
+ *                          all run as one window: each key's ops up to its
+ *                          first written batch in the window run as one mixed
+ *                          batch, its later ops in batch order after it (see
+ *                          hive_set_stream_window); other batches run alone
+ *                          as in LAUNCHES
  * h_offsets: host array of nbatches + 1 offsets.  batch_done_ns (host, may be
  * NULL) receives each batch's completion time in ns after the stream start
- * (device wall clock).  Synchronizes the stream. */
-enum { HIVE_STREAM_LAUNCHES = 0, HIVE_STREAM_GRAPH = 1, HIVE_STREAM_PERSISTENT = 2 };
+ * (device wall clock); in WINDOWED mode, every batch of a window completes
+ * with the window.  Synchronizes the stream. */
+enum { HIVE_STREAM_LAUNCHES = 0, HIVE_STREAM_GRAPH = 1, HIVE_STREAM_PERSISTENT = 2, HIVE_STREAM_WINDOWED = 3 };
 hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                 const uint64_t* h_offsets, size_t nbatches, uint32_t* out_vals, uint8_t* status,
                                 int mode, double* batch_done_ns, double* setup_ms, void* stream);
@@ -164,15 +171,23 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
  * and whether insert slow paths run inline (1) or staged per tile (0). */
 hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t blocks_per_cu,
                                     uint32_t inline_slow_path);
+/* Windows of HIVE_STREAM_WINDOWED: runs of consecutive batches of fewer
+ * than batch_ops ops each (0 = 2^19, at most 2^24) are coalesced into windows
+ * of at most window_ops ops (2^10..2^24, 0 = 2^21); every other batch runs
+ * alone.  The handle keeps 120-240 bytes of device scratch per window op once
+ * a windowed stream has run. */
+hive_status_t hive_set_stream_window(hive_table_t* t, uint64_t window_ops, uint64_t batch_ops);
 
 /* Autotuner for the stream forms (no reference counterpart): times the
  * sample stream on a scratch copy of the table -- so at the table's current
  * load factor, leaving the table unchanged -- for per-batch launches at
  * 128..1024 threads per block x {staged, inline} insert slow path (x split
  * by type or not, when the sample has batches of >= 2^18 ops), the
- * captured graph, and the persistent kernel over tile sizes {auto, 256, 1024,
- * 4096} x {2, 4, 8} blocks per CU x {staged, inline}, then stores the fastest
- * launch shapes in the handle.  Synchronizes. */
+ * captured graph, the persistent kernel over tile sizes {auto, 256, 1024,
+ * 4096} x {2, 4, 8} blocks per CU x {staged, inline}, and windows of
+ * {2^20, 2^21, 2^22} ops coalescing batches below {2^17, 2^18, 2^19} ops x
+ * {split by type or not}, then stores the fastest launch shapes and window in
+ * the handle.  Synchronizes. */
 typedef struct {
   uint32_t block_threads;  /* chosen threads per block of the plain kernels */
   uint32_t inline_slow_path; /* chosen insert slow-path placement of the plain kernels */
@@ -185,6 +200,9 @@ typedef struct {
   double graph_ms;
   double persistent_ms;
   uint32_t candidates;     /* configurations timed */
+  uint64_t window_ops;     /* chosen window size of the windowed form */
+  uint64_t window_batch_ops; /* chosen: batches below this size are coalesced */
+  double windowed_ms;
 } hive_tune_t;
 hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                    const uint64_t* h_offsets, size_t nbatches, hive_tune_t* out, void* stream);

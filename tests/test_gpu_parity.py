@@ -487,15 +487,18 @@ def test_stream_modes_agree_with_phased_oracle(H, O):
     norm = lambda s: np.where(s == 3, 1, s)
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
     prev = None
-    # (mode, plain-kernel slow path inline, persistent tile ops, blocks per CU, persistent inline)
-    shapes = [(0, 0, 0, 0, 0), (0, 1, 0, 0, 0), (1, 0, 0, 0, 0), (2, 0, 0, 0, 0), (2, 0, 256, 8, 1),
-              (2, 0, 1024, 2, 0), (2, 0, 0, 4, 1)]
-    for mode, inl, tile, bpc, sinl in shapes:
-        print(f"[case] stream shape mode={mode} inline={inl} tile={tile} bpc={bpc} stream_inline={sinl}", flush=True)
+    # (mode, plain-kernel slow path inline, persistent tile ops, blocks per CU, persistent inline, window ops)
+    shapes = [(0, 0, 0, 0, 0, 0), (0, 1, 0, 0, 0, 0), (1, 0, 0, 0, 0, 0), (2, 0, 0, 0, 0, 0), (2, 0, 256, 8, 1, 0),
+              (2, 0, 1024, 2, 0, 0), (2, 0, 0, 4, 1, 0), (3, 0, 0, 0, 0, (1 << 10, 0)), (3, 1, 0, 0, 0, (1 << 13, 0)),
+              (3, 0, 0, 0, 0, (1 << 16, 0)), (3, 0, 0, 0, 0, (1 << 13, 1 << 9))]
+    for mode, inl, tile, bpc, sinl, win in shapes:
+        print(f"[case] stream shape mode={mode} inline={inl} tile={tile} bpc={bpc} stream_inline={sinl} "
+              f"window={win}", flush=True)
         t = H.HiveTable(slots)
         for op in (0, 1, 2):
             t.set_launch(op, 256, inline_slow_path=inl)
         t.set_stream_shape(tile, bpc, sinl)
+        t.set_stream_window(*(win or (0, 0)))
         t.insert(pre)
         gv, gs, done, setup = t.mixed_stream(types, keys, vals, off, mode)
         gv, gs = np_(gv, np.uint32), np_(gs)
@@ -519,10 +522,58 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     t.insert(pre)
     before = gpu_raw(t, O).live_kv()
     res = t.autotune_stream(types, keys, vals, off)
-    assert res["candidates"] == 33
+    assert res["candidates"] == 51
+    assert res["window_ops"] in (1 << 20, 1 << 21, 1 << 22) and res["windowed_ms"] > 0
+    assert res["window_batch_ops"] in (1 << 17, 1 << 18, 1 << 19)
+    assert res["best_mode"] in (0, 1, 2, 3)
     assert res["block_threads"] in (128, 256, 512, 1024) and res["inline_slow_path"] in (0, 1)
     assert res["tile_ops"] in (0, 256, 1024, 4096) and res["blocks_per_cu"] in (2, 4, 8)
     assert res["stream_inline"] in (0, 1)
     assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
     assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"])
+
+
+def test_windowed_stream_orders_each_key_by_batch(H, O):
+    """Windows of many small batches in which the same keys recur in every
+    batch (a hot set in all of them, cold keys from a small space): each
+    key's ops must see exactly the batches before theirs, one round per
+    batch, against the oracle run batch by batch.  No key repeats inside a
+    batch, so every op has one admissible outcome."""
+    print("[case] windowed stream: recurring keys", flush=True)
+    slots = 1 << 14
+    rng = np.random.default_rng(7)
+    pre = W.distinct_keys(slots // 4, 71, bits=13)
+    hot = W.distinct_keys(48, 72, bits=13)
+    parts, sizes = [], []
+    for b in range(64):
+        k = np.unique(np.concatenate([hot, rng.integers(1, 1 << 13, size=220, dtype=np.uint64).astype(np.uint32)]))
+        k = k[rng.permutation(k.size)]
+        parts.append(k)
+        sizes.append(k.size)
+    keys = np.concatenate(parts).astype(np.uint32)
+    types = rng.choice(3, size=keys.size, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    vals = rng.integers(0, 2**32, size=keys.size, dtype=np.uint64).astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    o = O.Oracle(slots // 8)
+    o.insert(pre)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for b in range(len(sizes)):
+        lo, hi = int(off[b]), int(off[b + 1])
+        ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    norm = lambda s: np.where(s == 3, 1, s)
+    keyset = np.sort(o.dump() & 0xFFFFFFFF)
+    for win, split in ((1 << 10, 1), (1 << 12, 0), (1 << 14, 1)):
+        print(f"[case] windowed stream window={win} split={split}", flush=True)
+        t = H.HiveTable(slots)
+        t.set_launch(0, 256, split_by_type=split)
+        t.set_stream_window(win, 1 << 10)
+        t.insert(pre)
+        gv, gs, done, _ = t.mixed_stream(types, keys, vals, off, 3)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        assert (np.diff(done) >= 0).all() and done[-1] > 0
+        assert np.array_equal(norm(gs), norm(os_)), win
+        assert np.array_equal(gv, ov), win
+        raw, _ = check_gpu_state(t, O, "Default2Hash")
+        assert np.array_equal(np.sort(raw.live_kv() & 0xFFFFFFFF), keyset), win
