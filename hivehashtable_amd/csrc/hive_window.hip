@@ -41,6 +41,7 @@ using namespace hive;
 namespace {
 
 constexpr uint32_t kWinBlock = 256;       // ops per tile of the mark/level kernels
+constexpr uint32_t kRoundBlock = 1024;    // threads per block of the rounds kernel
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 constexpr uint32_t kDoneBatch = 0xFFFFFFFFu;
 constexpr uint32_t kNoWrite = 0xFFFFu;    // first-write field of a key only looked up
@@ -160,19 +161,21 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, cons
   }
 }
 
-// Grid barrier over every block of the launch (all resident: the grid is at
-// most one block per CU).  Each thread's memory operations are performed
-// before its block arrives.  A wait longer than the limit, or an error raised
-// elsewhere, sets the error flag and returns false in every block that sees it.
-__device__ bool grid_sync(const WinCtl& ctl, unsigned long long wait_ticks) {
+// Barrier over the nblk active blocks of the launch (all resident: the grid
+// is at most one block per CU); a lone block needs only its own barrier.
+// Each thread's memory operations are performed before its block arrives.  A
+// wait longer than the limit, or an error raised elsewhere, sets the error
+// flag and returns false in every block that sees it.
+__device__ bool grid_sync(const WinCtl& ctl, uint32_t nblk, unsigned long long wait_ticks) {
   __shared__ uint32_t s_ok;
   wait_vmem();
   __syncthreads();
+  if (nblk == 1u) return true;
   if (threadIdx.x == 0) {
     uint32_t ok = 1;
     const uint32_t g = ld_sync32(ctl.gen());
     const uint32_t arrived = __hip_atomic_fetch_add(ctl.arrive(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1u) {
+    if (arrived == nblk - 1u) {
       __hip_atomic_store(ctl.arrive(), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       wait_vmem();  // the reset is performed before the generation moves on
       __hip_atomic_fetch_add(ctl.gen(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -226,8 +229,10 @@ __device__ void stamp_window(const RoundArgs& a) {
 // are only taken when a later pass has work.  Table reads are device-coherent:
 // passes are separated by barriers, not kernel boundaries.  Block 0 stamps
 // the window's batches complete at the end.
+// Only the blocks the pending list needs (one op per thread where it fits)
+// take part; the others return at once.
 template <class P0>
-__global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs a) {
+__global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundArgs a) {
   using P = CoherentReads<P0>;
   __shared__ uint32_t s_cnt[3];
   const uint32_t npend = ld_sync32(a.ctl.pending(0));
@@ -235,8 +240,10 @@ __global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs
     if (blockIdx.x == 0) stamp_window(a);
     return;
   }
+  const uint32_t nblk = min(gridDim.x, (npend + blockDim.x - 1u) / blockDim.x);
+  if (blockIdx.x >= nblk) return;
   int dtab = 0, dstash = 0, dslow = 0;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = nblk * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t r = 0;; ++r) {
     const unsigned long long* cur = a.rw + (r & 1u) * a.cap;
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs
       if (s_cnt[1]) atomicAdd(a.ctl.count(OP_DELETE, cs), s_cnt[1]);
       if (s_cnt[2]) atomicAdd(a.ctl.count(OP_INSERT, cs), s_cnt[2]);
     }
-    if (!grid_sync(a.ctl, a.wait_ticks)) break;
+    if (!grid_sync(a.ctl, nblk, a.wait_ticks)) break;
     // final after the barrier, and the same in every block
     const bool more = ld_sync32(a.ctl.pending((r + 1u) % 3u)) != 0u;
     const bool any_ins = ld_sync32(a.ctl.count(OP_INSERT, cs)) != 0u;
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs
         dstash -= rs;
         a.pbid[j] = kDoneBatch;
       }
-      if ((any_ins || more) && !grid_sync(a.ctl, a.wait_ticks)) break;
+      if ((any_ins || more) && !grid_sync(a.ctl, nblk, a.wait_ticks)) break;
     }
     if (any_ins) {
       for (uint32_t j = tid; j < npend; j += stride) {  // inserts of this round
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs
         if (a.out_vals) a.out_vals[i] = 0;
         a.pbid[j] = kDoneBatch;
       }
-      if (more && !grid_sync(a.ctl, a.wait_ticks)) break;
+      if (more && !grid_sync(a.ctl, nblk, a.wait_ticks)) break;
     }
     if (!more) break;
   }
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_rounds(TableView t, RoundArgs
   wait_vmem();
   __syncthreads();
   if (threadIdx.x == 0)
-    s_last = __hip_atomic_fetch_add(a.ctl.finished(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+    s_last = __hip_atomic_fetch_add(a.ctl.finished(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nblk - 1u;
   __syncthreads();
   if (s_last) stamp_window(a);
   wave_count(t, dtab, dstash, dslow);
@@ -446,7 +453,7 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
   unsigned long long* set = t->win_set;
   const uint32_t mask = static_cast<uint32_t>(cap - 1);
   // a small grid, every block resident: the rounds are separated by grid barriers
-  const unsigned round_grid = static_cast<unsigned>(std::max(1, std::min(ncu, 128)));
+  const unsigned round_grid = static_cast<unsigned>(std::max(1, std::min(ncu, 64)));
   Pipeline pl;
   HIP_TRY(hipStreamCreateWithFlags(&pl.ms, hipStreamNonBlocking));
   for (hipEvent_t* e : {&pl.start, &pl.done, &pl.ready[0], &pl.ready[1], &pl.free[0], &pl.free[1]})
@@ -495,7 +502,7 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
     if (r != HIVE_OK) return r;
     RoundArgs ra{types, keys, vals, out_vals, status, wb.pidx, wb.pbid, wb.pe, wb.rw, static_cast<uint32_t>(cap), seq,
                  wb.ctl, wait_ticks, ts + w.b0 + 1, static_cast<uint32_t>(w.b1 - w.b0)};
-    HIVE_POLICY_SWITCH(t->policy, (k_win_rounds<P><<<round_grid, kWinBlock, 0, s>>>(t->v, ra)));
+    HIVE_POLICY_SWITCH(t->policy, (k_win_rounds<P><<<round_grid, kRoundBlock, 0, s>>>(t->v, ra)));
     HIP_CHECK_LAUNCH();
     HIP_TRY(hipEventRecord(pl.free[p], s));
     used[p] = true;
