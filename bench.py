@@ -49,6 +49,13 @@ def parse():
     p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
     p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
     p.add_argument("--mixed-batches", type=int, default=1000)
+    p.add_argument("--stream-modes", default="launches,graph,persistent,windowed",
+                   help="stream forms the mixed leg runs (comma list)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="mixed leg: fixed shapes (--stream-shape) instead of the stream autotuner (profiling runs)")
+    p.add_argument("--stream-shape", default="512:1:0:2097152:524288",
+                   help="with --no-autotune: threads per block:inline slow path:split by type:window ops:"
+                        "coalesce below ops")
     p.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
     p.add_argument("--sweep", action="store_true", help="also run the LF x exist-ratio sweep (2 policies)")
     p.add_argument("--growth", action="store_true", help="also run the in-place growth leg (2^22 -> 2^27 slots)")
@@ -165,15 +172,27 @@ def mixed_leg(a, H, W, torch, dev):
     t = H.HiveTable(slots, a.policy)
     t.insert(d_pre)
     ns = min(100, len(sizes))
-    tune = t.autotune_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1])
-    log(f"autotune: {tune}")
+    tune = {}
+    if a.no_autotune:
+        blk, inl, split, wops, wbatch = (int(x) for x in a.stream_shape.split(":"))
+        for op in (0, 1, 2):
+            t.set_launch(op, blk, inline_slow_path=inl, split_by_type=split)
+        t.set_stream_window(wops, wbatch)
+        tune = {"fixed_shape": a.stream_shape}
+        # untimed: the first windowed stream allocates and first-touches its scratch
+        t.mixed_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1], 3)
+        torch.cuda.synchronize()
+    else:
+        tune = t.autotune_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1])
+        log(f"autotune: {tune}")
     out = {"batches": len(sizes), "ops": n_ops, "preload_keys": int(pre.size), "key_space": 1 << 27,
            "batch_ops_min_max": [int(sizes.min()), int(sizes.max())], "mix_insert_lookup_delete": [0.5, 0.4, 0.1],
            "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}, "modes": {}}
     ref = None
     agree = True
     names = {0: "launches", 1: "graph", 2: "persistent", 3: "windowed"}
-    for mode in (0, 1, 2, 3):
+    wanted = [m for m, nm in names.items() if nm in a.stream_modes.split(",")]
+    for mode in wanted:
         t.clear()
         t.insert(d_pre)
         torch.cuda.synchronize()
