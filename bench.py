@@ -349,9 +349,17 @@ def sweep_leg(a, H, W, torch, dev):
                 expect = present & ~np.isin(lk, failed) if failed.size else present
                 row["lookups"][f"{x:.2f}"] = {"ms": round(look_ms, 4), "mops": round(slots / look_ms / 1e3, 1),
                                               "verified": bool(np.array_equal(found, expect))}
+            # lookup layout at this load factor: the bulk autotuner's lane-per-key
+            # vs 4 / 8 lanes per key on the 50 % exist-ratio set (then reset)
+            bt = t.autotune_bulk(None, None, sets[0.5][0])
+            row["lookup_layout"] = {"chosen_lanes_per_key": bt["lookup_lanes_per_key"],
+                                    "chosen_block_threads": bt["lookup_block_threads"],
+                                    "ms_by_lanes_per_key": {str(k): round(v, 4)
+                                                            for k, v in bt["lookup_ms_by_lanes"].items()}}
+            t.set_launch(1, 256)
             out[pol][f"{lf:.2f}"] = row
             log(f"sweep {pol} LF {lf}: insert {row['insert_mops']} Mops/s, lookups "
-                f"{[v['mops'] for v in row['lookups'].values()]}")
+                f"{[v['mops'] for v in row['lookups'].values()]}, layout {row['lookup_layout']}")
     for t in tables.values():
         t.close()
     return out

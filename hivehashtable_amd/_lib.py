@@ -49,7 +49,7 @@ class Stats(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("keys_per_lane", C.c_uint32), ("inline_slow_path", C.c_uint32),
-                ("combine_duplicates", C.c_uint32), ("split_by_type", C.c_uint32)]
+                ("combine_duplicates", C.c_uint32), ("split_by_type", C.c_uint32), ("lanes_per_key", C.c_uint32)]
 
 
 class Tune(C.Structure):
@@ -59,6 +59,13 @@ class Tune(C.Structure):
                 ("launches_ms", C.c_double), ("graph_ms", C.c_double), ("persistent_ms", C.c_double),
                 ("candidates", C.c_uint32), ("window_ops", C.c_uint64), ("window_batch_ops", C.c_uint64),
                 ("windowed_ms", C.c_double)]
+
+
+class BulkTune(C.Structure):
+    _fields_ = [("insert_block_threads", C.c_uint32), ("insert_inline_slow_path", C.c_uint32),
+                ("lookup_block_threads", C.c_uint32), ("lookup_lanes_per_key", C.c_uint32),
+                ("insert_ms", C.c_double), ("lookup_ms", C.c_double), ("lookup_ms_by_lanes", C.c_double * 3),
+                ("load_factor", C.c_double), ("candidates", C.c_uint32)]
 
 
 class ServeConfig(C.Structure):
@@ -130,6 +137,7 @@ def lib():
         "hive_set_stream_shape": [p, u32, u32, u32],
         "hive_set_stream_window": [p, C.c_uint64, C.c_uint64],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
+        "hive_autotune_bulk": [p, p, p, sz, p, sz, C.POINTER(BulkTune), p],
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
         "hive_insert_sequence": [p, p, p, sz, sz, p, sz, C.c_double, C.c_double, i32, p, p,
                                  C.POINTER(SequenceReport), p],

@@ -426,6 +426,10 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (c.inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
   if (c.combine_duplicates > 1) return fail(HIVE_EINVAL, "combine_duplicates must be 0 or 1");
   if (c.split_by_type > 1) return fail(HIVE_EINVAL, "split_by_type must be 0 or 1");
+  if (c.lanes_per_key == 0) c.lanes_per_key = 1;
+  if (c.lanes_per_key != 1 && c.lanes_per_key != 4 && c.lanes_per_key != 8)
+    return fail(HIVE_EINVAL, "lanes_per_key must be 1, 4 or 8");
+  if (c.lanes_per_key != 1 && op_type != OP_LOOKUP) return fail(HIVE_EINVAL, "lanes_per_key > 1 is for lookups");
   t->launch[op_type] = c;
   return HIVE_OK;
 }

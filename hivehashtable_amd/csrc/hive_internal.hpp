@@ -37,8 +37,8 @@ struct hive_table {
   int policy = 0;
   hive::TableView v{};
   size_t bytes = 0;
-  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0, 0, 1}, {hive::kDefaultBlock, 1, 0, 0, 1},
-                             {hive::kDefaultBlock, 1, 0, 0, 1}};
+  hive_launch_t launch[3] = {{hive::kDefaultBlock, 1, 0, 0, 1, 1}, {hive::kDefaultBlock, 1, 0, 0, 1, 1},
+                             {hive::kDefaultBlock, 1, 0, 0, 1, 1}};
   unsigned long long* scratch = nullptr;  // 8 u64 of device scratch
   uint32_t stream_tile = 0;              // persistent kernel: ops per tile (0 = sized per batch)
   uint32_t stream_blocks_per_cu = 0;     // persistent kernel: resident blocks per CU (0 = default)
@@ -103,8 +103,17 @@ template <class Src, class Out>
 hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
-  if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
-  HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  const uint32_t lanes = t->launch[OP_LOOKUP].lanes_per_key > 1 ? t->launch[OP_LOOKUP].lanes_per_key : 1;
+  if (!fits_one_launch(n * lanes, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
+  if (lanes == 8) {
+    HIVE_POLICY_SWITCH(t->policy, (k_lookup_coop<P, Src, Out, 8><<<grid_for(n * 8, blk), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                   phase)));
+  } else if (lanes == 4) {
+    HIVE_POLICY_SWITCH(t->policy, (k_lookup_coop<P, Src, Out, 4><<<grid_for(n * 4, blk), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                   phase)));
+  } else {
+    HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  }
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }

@@ -350,6 +350,101 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
   out.put(i, r, v);
 }
 
+// ------------------------------------------------------ cooperative lookup
+// L lanes per key (L = 4 or 8, a group of consecutive lanes of one
+// wavefront): lane r of a group loads slots [r*8/L, (r+1)*8/L) of each probed
+// 64-byte bucket, and one wavefront ballot gives every group the lowest
+// matching slot.  The probe sequence and result are those of lookup_one: H0,
+// the other candidates if H0's overflow bit is set, then stash[H0] if its
+// stash bit is set (stash entries below its count, one per lane of an
+// 8-lane group, two per lane of a 4-lane group).
+template <int L>
+__device__ __forceinline__ uint32_t group_bits(bool pred) {
+  const unsigned long long b = __ballot(pred);
+  const uint32_t base = (threadIdx.x & 63u) & ~static_cast<uint32_t>(L - 1);
+  return static_cast<uint32_t>(b >> base) & ((1u << L) - 1u);
+}
+
+// this lane's part of a probed region: kSlots / L consecutive slots from slot0
+template <int L>
+__device__ __forceinline__ void load_part(const unsigned long long* p, unsigned long long (&v)[kSlots / L]) {
+  if constexpr (L == 8) {
+    v[0] = p[0];
+  } else {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    v[0] = (static_cast<unsigned long long>(q.y) << 32) | q.x;
+    v[1] = (static_cast<unsigned long long>(q.w) << 32) | q.z;
+  }
+}
+
+// lowest slot of the group's region holding key: its value to every lane
+template <int L>
+__device__ __forceinline__ bool group_match(const unsigned long long (&v)[kSlots / L], uint32_t key, uint32_t& val) {
+  constexpr int W = kSlots / L;
+  int mine = -1;
+#pragma unroll
+  for (int j = W - 1; j >= 0; --j)
+    if (v[j] != 0ull && kv_key(v[j]) == key) mine = j;
+  const uint32_t bits = group_bits<L>(mine >= 0);
+  if (!bits) return false;
+  const uint32_t src = __builtin_ctz(bits);
+  uint32_t my_val = 0;
+#pragma unroll
+  for (int j = 0; j < W; ++j)
+    if (j == mine) my_val = kv_val(v[j]);
+  const int lane0 = static_cast<int>((threadIdx.x & 63u) & ~static_cast<uint32_t>(L - 1));
+  val = static_cast<uint32_t>(__shfl(static_cast<int>(my_val), lane0 + static_cast<int>(src)));
+  return true;
+}
+
+template <class P, class Src, class Out, int L>
+__global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src, size_t n, Out out, int phase) {
+  constexpr int W = kSlots / L;
+  const size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / L;
+  const uint32_t r = threadIdx.x & (L - 1);
+  if (i >= n) return;  // whole groups: n ops cover whole groups of lanes
+  const uint32_t key = src.key(i);
+  if (phase >= 0) {
+    const int ty = src.type(i, OP_LOOKUP);
+    if (ty != OP_LOOKUP) {
+      if (r == 0 && ty != OP_INSERT && ty != OP_DELETE && ty != OP_DEFERRED) out.put(i, R_NOOP, 0);
+      return;
+    }
+  }
+  uint32_t val = 0;
+  uint8_t res = R_NOT_FOUND;
+  if (key != 0) {
+    uint32_t b[P::NH];
+#pragma unroll
+    for (int h = 0; h < P::NH; ++h) b[h] = bucket_of<P>(h, key, t.nb);
+    unsigned long long v[W];
+    load_part<L>(bucket_ptr(t, b[0]) + r * W, v);
+    if (group_match<L>(v, key, val)) {
+      res = R_FOUND;
+    } else {
+      const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+      if (hint & kOverflowBit) {
+#pragma unroll
+        for (int h = 1; h < P::NH; ++h) {
+          if (res == R_FOUND || is_dup<P>(b, h)) continue;
+          load_part<L>(bucket_ptr(t, b[h]) + r * W, v);
+          if (group_match<L>(v, key, val)) res = R_FOUND;
+        }
+      }
+      if (res != R_FOUND && (hint & kStashBit)) {
+        uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
+        if (cnt > kStashSlots) cnt = kStashSlots;
+        load_part<L>(stash_ptr(t, b[0]) + r * W, v);
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          if (r * W + j >= cnt) v[j] = 0ull;  // past the count: not an entry
+        if (group_match<L>(v, key, val)) res = R_FOUND;
+      }
+    }
+  }
+  if (r == 0) out.put(i, res, res == R_FOUND ? val : 0u);
+}
+
 // ---------------------------------------------------------------- erase
 template <class P, class Src, class Out>
 __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_t n, Out out, int phase) {

@@ -501,7 +501,8 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
     for (uint32_t inl = 0; inl < 2; ++inl)
       for (uint32_t blk : blocks) {
         for (int op = 0; op < 3; ++op)
-          tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates, split};
+          tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates, split,
+                                         t->launch[op].lanes_per_key};
         double ms = 0;
         if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
         ++cands;
@@ -514,7 +515,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
       }
   for (int op = 0; op < 3; ++op)
     tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
-                                   best.split_by_type};
+                                   best.split_by_type, t->launch[op].lanes_per_key};
   if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms)) != HIVE_OK) return r;
   ++cands;
   const uint32_t tiles[] = {0, 256, 1024, 4096};
@@ -551,7 +552,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
         if (cb > w) continue;
         for (int op = 0; op < 3; ++op)
           tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path,
-                                         t->launch[op].combine_duplicates, split};
+                                         t->launch[op].combine_duplicates, split, t->launch[op].lanes_per_key};
         tmp.stream_window_ops = w;
         tmp.stream_window_batch = cb;
         double ms = 0;
@@ -573,12 +574,121 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   if (best.best_mode == HIVE_STREAM_WINDOWED) best.split_by_type = win_split;
   for (int op = 0; op < 3; ++op)
     t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
-                                  best.split_by_type};
+                                  best.split_by_type, t->launch[op].lanes_per_key};
   t->stream_tile = best.tile_ops;
   t->stream_blocks_per_cu = best.blocks_per_cu;
   t->stream_inline = best.stream_inline;
   t->stream_window_ops = best.window_ops;
   t->stream_window_batch = best.window_batch_ops;
+  *out = best;
+  return HIVE_OK;
+}
+
+extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
+                                            size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup,
+                                            hive_bulk_tune_t* out, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (!out || (n_insert && !insert_keys) || (n_lookup && !lookup_keys)) return fail(HIVE_EINVAL, "bad sample or output");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hive_bulk_tune_t best{};
+  best.insert_ms = best.lookup_ms = 1e300;
+  for (double& v : best.lookup_ms_by_lanes) v = 1e300;
+  hive_stats_t st{};
+  if ((r = hive_stats(t, &st, stream)) != HIVE_OK) return r;
+  best.load_factor = st.load_factor;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {
+    hipEvent_t* a;
+    hipEvent_t* b;
+    ~Events() {
+      if (*a) (void)hipEventDestroy(*a);
+      if (*b) (void)hipEventDestroy(*b);
+    }
+  } ev_guard{&e0, &e1};
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto timed = [&](auto&& launch, double* ms) -> hive_status_t {
+    HIP_TRY(hipEventRecord(e0, s));
+    hive_status_t rr = launch();
+    if (rr != HIVE_OK) return rr;
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    return HIVE_OK;
+  };
+  const uint32_t blocks[] = {256, 512, 1024};
+  uint32_t cands = 0;
+  DevArena ar;
+  if (n_insert) {
+    hive_table tmp = *t;  // scratch copy of the table state, same policy and device
+    if ((r = alloc_view(tmp.v, t->v.nb)) != HIVE_OK) return r;
+    struct ViewGuard {
+      TableView* v;
+      ~ViewGuard() { free_view(*v); }
+    } guard{&tmp.v};
+    uint8_t* st8 = nullptr;
+    HIP_TRY(ar.alloc(&st8, n_insert));
+    for (uint32_t inl = 0; inl < 2; ++inl)
+      for (uint32_t blk : blocks) {
+        tmp.launch[OP_INSERT] = hive_launch_t{blk, 1, inl, 0, t->launch[OP_INSERT].split_by_type, 1};
+        if ((r = copy_view(tmp.v, t->v, s)) != HIVE_OK) return r;
+        double ms = 0;
+        r = timed([&] { return launch_insert(&tmp, SrcNative{insert_keys, insert_vals, nullptr}, n_insert,
+                                             OutNative{st8, nullptr}, -1, s); },
+                  &ms);
+        if (r != HIVE_OK) return r;
+        ++cands;
+        if (ms < best.insert_ms) {
+          best.insert_ms = ms;
+          best.insert_block_threads = blk;
+          best.insert_inline_slow_path = inl;
+        }
+      }
+  }
+  if (n_lookup) {
+    uint8_t* st8 = nullptr;
+    uint32_t* v32 = nullptr;
+    HIP_TRY(ar.alloc(&st8, n_lookup));
+    HIP_TRY(ar.alloc(&v32, n_lookup));
+    const hive_launch_t saved = t->launch[OP_LOOKUP];
+    const uint32_t lanes[] = {1, 4, 8};
+    for (int li = 0; li < 3; ++li)
+      for (uint32_t blk : blocks) {
+        t->launch[OP_LOOKUP] = hive_launch_t{blk, 1, 0, 0, saved.split_by_type, lanes[li]};
+        double ms = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {  // read-only: repeat and keep the fastest
+          double m = 0;
+          r = timed([&] { return launch_lookup(t, SrcNative{lookup_keys, nullptr, nullptr}, n_lookup,
+                                               OutNative{st8, v32}, -1, s); },
+                    &m);
+          if (r != HIVE_OK) {
+            t->launch[OP_LOOKUP] = saved;
+            return r;
+          }
+          ms = std::min(ms, m);
+        }
+        ++cands;
+        best.lookup_ms_by_lanes[li] = std::min(best.lookup_ms_by_lanes[li], ms);
+        if (ms < best.lookup_ms) {
+          best.lookup_ms = ms;
+          best.lookup_block_threads = blk;
+          best.lookup_lanes_per_key = lanes[li];
+        }
+      }
+    t->launch[OP_LOOKUP] = saved;
+  }
+  best.candidates = cands;
+  if (n_insert) {
+    t->launch[OP_INSERT].block_threads = best.insert_block_threads;
+    t->launch[OP_INSERT].inline_slow_path = best.insert_inline_slow_path;
+  }
+  if (n_lookup) {
+    t->launch[OP_LOOKUP].block_threads = best.lookup_block_threads;
+    t->launch[OP_LOOKUP].lanes_per_key = best.lookup_lanes_per_key;
+  }
   *out = best;
   return HIVE_OK;
 }

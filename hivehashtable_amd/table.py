@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib,
+from ._lib import (BulkTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib,
                    OP_INSERT,
                    OP_LOOKUP, OP_DELETE)
 
@@ -296,14 +296,32 @@ class HiveTable:
                                      cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_import_raw")
 
     def set_launch(self, op_type, block_threads, keys_per_lane=1, inline_slow_path=0, combine_duplicates=0,
-                   split_by_type=1):
-        cfg = Launch(block_threads, keys_per_lane, inline_slow_path, combine_duplicates, split_by_type)
+                   split_by_type=1, lanes_per_key=1):
+        cfg = Launch(block_threads, keys_per_lane, inline_slow_path, combine_duplicates, split_by_type, lanes_per_key)
         check(self.L.hive_set_launch(self.h, op_type, C.byref(cfg)), "hive_set_launch")
 
     def get_launch(self, op_type):
         cfg = Launch()
         check(self.L.hive_get_launch(self.h, op_type, C.byref(cfg)), "hive_get_launch")
-        return cfg.block_threads, cfg.keys_per_lane, cfg.inline_slow_path, cfg.combine_duplicates, cfg.split_by_type
+        return (cfg.block_threads, cfg.keys_per_lane, cfg.inline_slow_path, cfg.combine_duplicates, cfg.split_by_type,
+                cfg.lanes_per_key)
+
+    def autotune_bulk(self, insert_keys=None, insert_vals=None, lookup_keys=None, stream=None):
+        """Time bulk insert and lookup shapes at the table's current load
+        factor (inserts on a scratch copy); keep the fastest."""
+        torch = _torch()
+        ik = _dev(insert_keys, torch.int32) if insert_keys is not None else None
+        iv = _dev(insert_vals, torch.int32) if insert_vals is not None else None
+        lk = _dev(lookup_keys, torch.int32) if lookup_keys is not None else None
+        res = BulkTune()
+        check(self.L.hive_autotune_bulk(self.h, _ptr(ik) if ik is not None else None,
+                                        _ptr(iv) if iv is not None else None, ik.numel() if ik is not None else 0,
+                                        _ptr(lk) if lk is not None else None, lk.numel() if lk is not None else 0,
+                                        C.byref(res), _stream_ptr(stream)), "hive_autotune_bulk")
+        out = {f: getattr(res, f) for f, _ in BulkTune._fields_}
+        out["lookup_ms_by_lanes"] = {1: res.lookup_ms_by_lanes[0], 4: res.lookup_ms_by_lanes[1],
+                                     8: res.lookup_ms_by_lanes[2]}
+        return out
 
 
 def make_ops(types, keys):

@@ -100,6 +100,10 @@ typedef struct {
                                   batches of >= 2^18 ops, inserts and deletes are
                                   compacted into dense lists before their phase
                                   kernels; 0 = every phase filters the whole batch */
+  uint32_t lanes_per_key;      /* lookups: 0 or 1 = one lane per key reads whole
+                                  buckets; 4 or 8 = that many lanes of a wavefront
+                                  share a key, each reading part of every probed
+                                  bucket (same results) */
 } hive_launch_t;
 
 /* ---- lifecycle: replaces hiveHashTableCreate / Destroy / Clear
@@ -206,6 +210,29 @@ typedef struct {
 } hive_tune_t;
 hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                    const uint64_t* h_offsets, size_t nbatches, hive_tune_t* out, void* stream);
+
+/* Autotuner for the bulk kernels (no reference counterpart), at the table's
+ * current load factor: times the insert of the sample keys (device arrays,
+ * vals may be NULL) on a scratch copy of the table at 256/512/1024 threads
+ * per block x {staged, inline} slow path, and the lookup of the sample lookup
+ * keys on the table itself at 256/512/1024 threads x {1, 4, 8} lanes per key,
+ * then stores the fastest insert and lookup shapes in the handle.  The table
+ * is left unchanged.  Either sample may be empty (n = 0: that op is not
+ * tuned).  Synchronizes. */
+typedef struct {
+  uint32_t insert_block_threads;
+  uint32_t insert_inline_slow_path;
+  uint32_t lookup_block_threads;
+  uint32_t lookup_lanes_per_key;
+  double insert_ms;               /* best sample times (device clock) */
+  double lookup_ms;
+  double lookup_ms_by_lanes[3];   /* best lookup time with 1, 4 and 8 lanes per key */
+  double load_factor;             /* of the table while tuned */
+  uint32_t candidates;
+} hive_bulk_tune_t;
+hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
+                                 size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup, hive_bulk_tune_t* out,
+                                 void* stream);
 
 /* ---- serving: continuous batching over one resident table
  * Replaces the reference's YCSB driver (src/hashTableBenchmark.cc:434-570,

@@ -94,9 +94,13 @@ def test_lookup_and_erase_on_oracle_built_table(H, O, policy):
     t = H.HiveTable(nb * 8, policy)
     t.import_raw(raw.slots, raw.stash, raw.stash_count)
     probe = np.concatenate([keys, W.distinct_keys(5000, 13, exclude=keys), np.array([0], np.uint32)])
-    gv, gs = t.lookup(probe)
     ov, os_ = o.lookup(probe)
-    assert np.array_equal(np_(gs), os_) and np.array_equal(np_(gv, np.uint32), ov)
+    # lane-per-key and cooperative (4 or 8 lanes per key) lookups, two block sizes
+    for lanes, blk in ((1, 256), (4, 256), (8, 256), (4, 1024), (8, 64)):
+        t.set_launch(1, blk, lanes_per_key=lanes)
+        gv, gs = t.lookup(probe)
+        assert np.array_equal(np_(gs), os_) and np.array_equal(np_(gv, np.uint32), ov), (lanes, blk)
+    t.set_launch(1, 256)
     rng = np.random.default_rng(14)
     dk = np.concatenate([rng.choice(keys, 4000, replace=False), W.distinct_keys(500, 15, exclude=keys),
                          np.array([0], np.uint32)])
@@ -407,7 +411,7 @@ def test_launch_shapes_give_identical_results(H, O):
         t = H.HiveTable(slots)
         for op in (0, 1, 2):
             t.set_launch(op, blk, inline_slow_path=inl, combine_duplicates=comb)
-        assert t.get_launch(1) == (blk, 1, inl, comb, 1)
+        assert t.get_launch(1) == (blk, 1, inl, comb, 1, 1)
         t.insert(keys)
         gv, gs = t.lookup(lk)
         cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
@@ -531,16 +535,18 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     assert res["stream_inline"] in (0, 1)
     assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
-    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"])
+    assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"], 1)
 
 
-def test_windowed_stream_orders_each_key_by_batch(H, O):
+@pytest.mark.parametrize("policy", ["Default2Hash", "TripleHash"])
+def test_windowed_stream_orders_each_key_by_batch(H, O, policy):
     """Windows of many small batches in which the same keys recur in every
     batch (a hot set in all of them, cold keys from a small space): each
     key's ops must see exactly the batches before theirs, one round per
     batch, against the oracle run batch by batch.  No key repeats inside a
-    batch, so every op has one admissible outcome."""
-    print("[case] windowed stream: recurring keys", flush=True)
+    batch, so every op has one admissible outcome.  A few ops are NONE /
+    ATOMIC_INC or carry key 0, which a window must answer like a batch."""
+    print(f"[case] windowed stream: recurring keys, {policy}", flush=True)
     slots = 1 << 14
     rng = np.random.default_rng(7)
     pre = W.distinct_keys(slots // 4, 71, bits=13)
@@ -554,8 +560,11 @@ def test_windowed_stream_orders_each_key_by_batch(H, O):
     keys = np.concatenate(parts).astype(np.uint32)
     types = rng.choice(3, size=keys.size, p=[0.4, 0.4, 0.2]).astype(np.uint8)
     vals = rng.integers(0, 2**32, size=keys.size, dtype=np.uint64).astype(np.uint32)
+    odd = rng.random(keys.size)
+    types[odd < 0.02] = rng.choice([3, 4], size=int((odd < 0.02).sum())).astype(np.uint8)
+    keys[(odd >= 0.02) & (odd < 0.03)] = 0
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
-    o = O.Oracle(slots // 8)
+    o = O.Oracle(slots // 8, policy)
     o.insert(pre)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
@@ -566,7 +575,7 @@ def test_windowed_stream_orders_each_key_by_batch(H, O):
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
     for win, split in ((1 << 10, 1), (1 << 12, 0), (1 << 14, 1)):
         print(f"[case] windowed stream window={win} split={split}", flush=True)
-        t = H.HiveTable(slots)
+        t = H.HiveTable(slots, policy)
         t.set_launch(0, 256, split_by_type=split)
         t.set_stream_window(win, 1 << 10)
         t.insert(pre)
@@ -575,5 +584,31 @@ def test_windowed_stream_orders_each_key_by_batch(H, O):
         assert (np.diff(done) >= 0).all() and done[-1] > 0
         assert np.array_equal(norm(gs), norm(os_)), win
         assert np.array_equal(gv, ov), win
-        raw, _ = check_gpu_state(t, O, "Default2Hash")
+        raw, _ = check_gpu_state(t, O, policy)
         assert np.array_equal(np.sort(raw.live_kv() & 0xFFFFFFFF), keyset), win
+
+
+def test_bulk_autotune_leaves_table_unchanged(H, O):
+    """The bulk autotuner times insert shapes on a scratch copy and lookup
+    layouts on the table itself; the table's state is unchanged and the
+    chosen shapes are stored and give the same lookup results."""
+    print("[case] bulk autotune", flush=True)
+    slots = 1 << 18
+    keys = W.distinct_keys(int(slots * 0.9), 81)
+    t = H.HiveTable(slots)
+    t.insert(keys[: keys.size // 2])
+    before = gpu_raw(t, O).live_kv()
+    lk, _ = W.lookup_set(keys, keys.size // 2, 0.5, 82)
+    ref_v, ref_s = (np_(x) for x in t.lookup(lk))
+    res = t.autotune_bulk(keys[keys.size // 2:], None, lk)
+    assert res["candidates"] == 15
+    assert res["insert_block_threads"] in (256, 512, 1024) and res["insert_inline_slow_path"] in (0, 1)
+    assert res["lookup_block_threads"] in (256, 512, 1024) and res["lookup_lanes_per_key"] in (1, 4, 8)
+    assert res["load_factor"] == pytest.approx(0.45, abs=0.005)
+    assert min(res["lookup_ms_by_lanes"].values()) == pytest.approx(res["lookup_ms"])
+    assert np.array_equal(gpu_raw(t, O).live_kv(), before)
+    lp = t.get_launch(1)
+    assert (lp[0], lp[5]) == (res["lookup_block_threads"], res["lookup_lanes_per_key"])
+    assert t.get_launch(0)[0] == res["insert_block_threads"]
+    v, st = (np_(x) for x in t.lookup(lk))
+    assert np.array_equal(v, ref_v) and np.array_equal(st, ref_s)
