@@ -201,7 +201,7 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
                               unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
                               int ncu, unsigned long long wait_ticks, hipStream_t s);
 // the sticky error word of the window control block (nonzero: a round barrier gave up)
-const unsigned int* window_error_word(const hive_table* t);
+unsigned int* window_error_word(const hive_table* t);
 
 // device buffers freed together when a call returns
 struct DevArena {

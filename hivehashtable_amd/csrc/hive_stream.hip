@@ -306,7 +306,7 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
     if (!desc.empty())
       HIP_TRY(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(ctrs, 0, nbatches * 4, s));
-    HIP_TRY(hipMemsetAsync(const_cast<unsigned int*>(window_error_word(t)), 0, 4, s));
+    HIP_TRY(hipMemsetAsync(window_error_word(t), 0, 4, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (setup_ms)
       *setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

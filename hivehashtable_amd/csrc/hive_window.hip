@@ -215,10 +215,13 @@ struct RoundArgs {
   uint32_t nts;
 };
 
+// one clock reading for every batch of the window (called by a whole block)
 __device__ void stamp_window(const RoundArgs& a) {
-  const unsigned long long now = wall_clock64();
+  __shared__ unsigned long long s_now;
+  if (threadIdx.x == 0) s_now = wall_clock64();
+  __syncthreads();
   for (uint32_t j = threadIdx.x; j < a.nts; j += blockDim.x)
-    __hip_atomic_store(a.ts + j, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Deferred ops, round by round.  Round r runs the pending ops whose batch is
@@ -343,9 +346,11 @@ __global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundAr
 }
 
 __global__ void k_stamp_range(unsigned long long* ts, uint32_t n) {
-  const unsigned long long now = wall_clock64();
+  __shared__ unsigned long long s_now;
+  if (threadIdx.x == 0) s_now = wall_clock64();
+  __syncthreads();
   for (uint32_t j = threadIdx.x; j < n; j += blockDim.x)
-    __hip_atomic_store(ts + j, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -385,7 +390,7 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
 }
 }  // namespace
 
-const unsigned int* window_error_word(const hive_table* t) {
+unsigned int* window_error_word(const hive_table* t) {
   return t->win_buf ? t->win_buf + 2 * buffer_words(t->win_cap) : nullptr;
 }
 

@@ -612,3 +612,32 @@ def test_bulk_autotune_leaves_table_unchanged(H, O):
     assert t.get_launch(0)[0] == res["insert_block_threads"]
     v, st = (np_(x) for x in t.lookup(lk))
     assert np.array_equal(v, ref_v) and np.array_equal(st, ref_s)
+
+
+def test_windowed_stream_thousands_of_tiny_batches(H, O):
+    """5000 batches of 2 ops over 4 keys: windows reach their 4096-batch cap,
+    and each key has a pending op in most batches of its window, so the
+    rounds kernel runs about two thousand rounds; results equal the oracle
+    run batch by batch (no key repeats inside a batch)."""
+    print("[case] windowed stream: 5000 two-op batches over 4 keys", flush=True)
+    rng = np.random.default_rng(17)
+    nb = 5000
+    keys = np.stack([rng.choice(np.arange(1, 5, dtype=np.uint32), 2, replace=False) for _ in range(nb)]).reshape(-1)
+    types = rng.choice(3, size=keys.size, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    vals = rng.integers(0, 2**32, size=keys.size, dtype=np.uint64).astype(np.uint32)
+    off = (np.arange(nb + 1) * 2).astype(np.uint64)
+    o = O.Oracle(1 << 7)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for b in range(nb):
+        ov[2 * b:2 * b + 2], os_[2 * b:2 * b + 2] = o.mixed(types[2 * b:2 * b + 2], keys[2 * b:2 * b + 2],
+                                                            vals[2 * b:2 * b + 2])
+    t = H.HiveTable(1 << 10)
+    t.set_stream_window(1 << 16, 1 << 10)
+    gv, gs, done, _ = t.mixed_stream(types, keys, vals, off, 3)
+    gv, gs = np_(gv, np.uint32), np_(gs)
+    norm = lambda s: np.where(s == 3, 1, s)
+    assert np.array_equal(norm(gs), norm(os_))
+    assert np.array_equal(gv, ov)
+    assert np.unique(done).size == 2  # two windows: 4096 batches, then 904
+    assert np.array_equal(np.sort(gpu_raw(t, O).live_kv() & 0xFFFFFFFF), np.sort(o.dump() & 0xFFFFFFFF))
