@@ -34,9 +34,23 @@ constexpr int OP_INSERT = 0, OP_LOOKUP = 1, OP_DELETE = 2;
 // internal op type of a windowed stream: an op held back for a later round;
 // every phase kernel skips it and writes nothing for it
 constexpr int OP_DEFERRED = 0xFE;
-// per-bucket hint bits in TableView::meta
-constexpr uint32_t kOverflowBit = 1u;  // a key whose H0 is this bucket may live in another candidate bucket
-constexpr uint32_t kStashBit = 2u;     // this bucket's stash holds pushed entries (count > 0)
+// per-bucket hint bits in TableView::meta: kHintBits per bucket, of which the
+// low kHintBits - 1 are overflow fingerprint classes and the top one is the
+// stash bit.  Overflow class bit f of bucket c: a key whose H0 is c and whose
+// fingerprint class is f may live in another candidate bucket.  2 bits (one
+// class) is the default: at 2^26 slots, 4 and 8 bits skip more H1 reads of
+// absent keys but make the hint array 2-4x larger, and measured slower
+// (insert to LF 0.9: 5.12 ms at 2 bits, 5.34 at 4, 5.31 at 8; lookups equal
+// or slower).
+#ifndef HIVE_HINT_BITS
+#define HIVE_HINT_BITS 2
+#endif
+constexpr uint32_t kHintBits = HIVE_HINT_BITS;
+static_assert(kHintBits == 2 || kHintBits == 4 || kHintBits == 8, "hint bits per bucket: 2, 4 or 8");
+constexpr uint32_t kHintsPerWord = 32 / kHintBits;
+constexpr uint32_t kOverflowClasses = kHintBits - 1;
+constexpr uint32_t kOverflowMask = (1u << kOverflowClasses) - 1u;  // every overflow class
+constexpr uint32_t kStashBit = 1u << kOverflowClasses;            // this bucket's stash holds pushed entries
 
 struct Op16 {  // the reference's Operation record (include/utils.h:18-21)
   int32_t type;
@@ -48,7 +62,7 @@ struct TableView {
   unsigned long long* slots;        // nb * 8
   unsigned long long* stash;        // nb * 8
   uint32_t* stash_count;            // nb
-  uint32_t* meta;                   // (nb + 15) / 16 words, 2 bits per bucket: kOverflowBit, kStashBit
+  uint32_t* meta;                   // kHintBits per bucket, kHintsPerWord buckets per word
   uint32_t* locks;                  // nb
   unsigned long long* counters;     // [3][kCounterLanes][kCounterStride]: table, stash, slow-path ops
   uint32_t nb;
@@ -304,21 +318,36 @@ __device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[kSl
 }
 
 // ------------------------------------------------------------- hint bits
-// meta packs 2 bits per bucket, 16 buckets per 32-bit word (2 MiB for 2^23
-// buckets, small enough to stay L2-resident under the random bucket stream).
-//  kOverflowBit of bucket c is set before any key whose H0 is c is moved out of
-//  c by a cuckoo kick.  A key leaves H0 only that way (Stage B places into H0;
-//  the kick that moves a victim to a non-primary bucket sets the bit of the
-//  victim's H0), so while the bit is clear no key with H0 == c lives in another
-//  candidate bucket and probes of H1/H2 can be skipped.  The bit is
-//  conservative: deletes never clear it; clear() and resize() rebuild it.
+// meta packs kHintBits per bucket (2 MiB for 2^23 buckets at 2 bits, small
+// enough to stay L2-resident under the random bucket stream).
+//  The overflow class bit ovf_bit(k) of bucket c is set before any key k
+//  whose H0 is c is moved out of c by a cuckoo kick.  A key leaves H0 only
+//  that way (Stage B places into H0; the kick that moves a victim to a
+//  non-primary bucket sets the victim's class bit in its H0), so while key k's
+//  class bit is clear no key with k's H0 and class lives in another candidate
+//  bucket, and k's probes of H1/H2 can be skipped.  With kHintBits - 1 classes
+//  a bucket whose displaced keys are all of other classes no longer costs
+//  absent keys the H1 read.  The bits are conservative: deletes never clear
+//  them; clear() and resize() rebuild them.
 //  kStashBit of c is set when stash[c] receives its first entry (the
 //  reference's freeMask bit 31, include/hive_hash_insert.cuh:512-514).
-__device__ __forceinline__ uint32_t meta_bits(uint32_t word, uint32_t b) { return (word >> ((b & 15) * 2)) & 3u; }
+__device__ __forceinline__ uint32_t meta_bits(uint32_t word, uint32_t b) {
+  return (word >> ((b % kHintsPerWord) * kHintBits)) & ((1u << kHintBits) - 1u);
+}
 template <class P>
-__device__ __forceinline__ uint32_t meta_word(const TableView& t, uint32_t b) { return read_u32<P>(t.meta + (b >> 4)); }
-__device__ __forceinline__ void meta_set(const TableView& t, uint32_t b, uint32_t bit) {
-  __hip_atomic_fetch_or(t.meta + (b >> 4), bit << ((b & 15) * 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t meta_word(const TableView& t, uint32_t b) {
+  return read_u32<P>(t.meta + b / kHintsPerWord);
+}
+__device__ __forceinline__ void meta_set(const TableView& t, uint32_t b, uint32_t bits) {
+  __hip_atomic_fetch_or(t.meta + b / kHintsPerWord, bits << ((b % kHintsPerWord) * kHintBits), __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_AGENT);
+}
+// a key's overflow class bit: a multiply-shift of a mix independent of the
+// bucket hashes onto [0, kOverflowClasses)
+__device__ __forceinline__ uint32_t ovf_bit(uint32_t key) {
+  if constexpr (kOverflowClasses == 1) return 1u;
+  const uint32_t x = fmix32(key ^ 0x2545F491u);
+  return 1u << static_cast<uint32_t>((static_cast<unsigned long long>(x) * kOverflowClasses) >> 32);
 }
 
 __device__ __forceinline__ unsigned long long* bucket_ptr(const TableView& t, uint32_t b) {
@@ -360,7 +389,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
       return update_slot(bucket_ptr(t, b[0]) + j, s0[j], kv) ? R_UPDATED : R_PENDING;
     }
   }
-  if (hint & kOverflowBit) {
+  if (hint & ovf_bit(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
@@ -451,7 +480,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     // stash[c], so the hint read here is exact for this key.
     unsigned long long cs[kSlots];
     load_bucket_sync(cb, cs);
-    const uint32_t hint = meta_bits(ld_sync32(t.meta + (c >> 4)), c);
+    const uint32_t hint = meta_bits(ld_sync32(t.meta + c / kHintsPerWord), c);
     if (attempt == 0 && pre_alt != ~0u) {
       read_bucket<P>(bucket_ptr(t, pre_alt), pre);
       have_pre = true;
@@ -464,7 +493,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
         continue;
       }
     }
-    if (hint & kOverflowBit) {
+    if (hint & ovf_bit(key)) {
       uint32_t b[P::NH];
 #pragma unroll
       for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
@@ -519,7 +548,8 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
       // before the victim can leave its H0 (already set when vh0 == c and the
       // coherent hint read above saw it)
-      if (a != vh0 && !(vh0 == c && (hint & kOverflowBit))) meta_set(t, vh0, kOverflowBit);
+      const uint32_t vbit = ovf_bit(vkey);
+      if (a != vh0 && !(vh0 == c && (hint & vbit))) meta_set(t, vh0, vbit);
       int d = -1;
       while (ae) {
         const int j = __builtin_ctz(ae);
@@ -596,7 +626,7 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   // every lookup one more memory request, which the request-rate-bound kernel
   // pays for more than the round trip it saves on misses (measured)
   const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
-  if (hint & kOverflowBit) {
+  if (hint & ovf_bit(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
@@ -671,7 +701,7 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
   const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
   removed_tab += erase_bucket<P>(bucket_ptr(t, b[0]), key);
-  if (hint & kOverflowBit) {
+  if (hint & ovf_bit(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) removed_tab += erase_bucket<P>(bucket_ptr(t, b[i]), key);
   }

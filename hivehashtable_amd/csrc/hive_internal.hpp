@@ -56,7 +56,7 @@ struct hive_table {
 
 namespace hive {
 
-inline size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + 15) / 16; }
+inline size_t meta_words(uint32_t nb) { return (static_cast<size_t>(nb) + kHintsPerWord - 1) / kHintsPerWord; }
 inline size_t counter_words() { return static_cast<size_t>(3) * kCounterLanes * kCounterStride; }
 inline unsigned grid_for(size_t n, uint32_t block) { return static_cast<unsigned>((n + block - 1) / block); }
 // one launch covers at most 2^31 - 1 blocks

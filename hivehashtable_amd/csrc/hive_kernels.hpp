@@ -423,7 +423,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
       res = R_FOUND;
     } else {
       const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
-      if (hint & kOverflowBit) {
+      if (hint & ovf_bit(key)) {
 #pragma unroll
         for (int h = 1; h < P::NH; ++h) {
           if (res == R_FOUND || is_dup<P>(b, h)) continue;
@@ -541,15 +541,17 @@ __global__ void k_rebuild_meta(TableView t) {
   const unsigned long long kv = t.slots[i];
   if (kv != 0ull) {
     const uint32_t h0 = bucket_of<P>(0, kv_key(kv), t.nb);
-    if (h0 != b) meta_set(t, h0, kOverflowBit);
+    if (h0 != b) meta_set(t, h0, ovf_bit(kv_key(kv)));
   }
   if (i % kSlots == 0 && t.stash_count[b] > 0) meta_set(t, b, kStashBit);
 }
 
 // drop every stash bit, keep the overflow bits (stash drain)
 static __global__ void k_clear_stash_bits(TableView t) {
+  uint32_t keep = 0;
+  for (uint32_t j = 0; j < kHintsPerWord; ++j) keep |= kOverflowMask << (j * kHintBits);
   const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w < (static_cast<size_t>(t.nb) + 15) / 16) t.meta[w] &= 0x55555555u;
+  if (w < (static_cast<size_t>(t.nb) + kHintsPerWord - 1) / kHintsPerWord) t.meta[w] &= keep;
 }
 
 // stash drain (include/hive_stash_table.cuh:33-64; holes are skipped)
@@ -622,7 +624,7 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
     if (!found) v[3] = 1;
 #pragma unroll
     for (int j = 1; j < P::NH; ++j) {
-      if (!(hint & kOverflowBit) || is_dup<P>(b, j)) continue;
+      if (!(hint & ovf_bit(key)) || is_dup<P>(b, j)) continue;
       ++v[0];
       if (found) continue;
       ++v[1];
