@@ -89,8 +89,11 @@ constexpr uint32_t kCtlWords = (kCtlCounters + 1) * kLine;  // + the shared erro
 
 // One block per tile of 256 ops of one batch: desc = {first op, end op,
 // batch index in the window, -}.  Block 0 also resets the control block.
-// Slot reads are plain (a stale view only makes a CAS fail and return the
-// current entry) except right after the set was cleared (coherent = 1).
+// Only writes (inserts, deletes) enter the set: a key's first written batch
+// is all that decides deferral, and lookups find their key's entry, if any,
+// by a read-only probe in k_win_level.  Slot reads are plain (a stale view
+// only makes a CAS fail and return the current entry) except right after the
+// set was cleared (coherent = 1).
 __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const uint8_t* types, const uint32_t* keys,
                                                         uint32_t wlo, unsigned long long* set, uint32_t mask,
                                                         uint32_t epoch, uint32_t coherent, uint32_t* eidx, WinCtl ctl) {
@@ -102,8 +105,8 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const
   const int ty = types[i];
   const uint32_t key = keys[i];
   uint32_t e = kNoEntry;
-  if (key != 0u && ty <= OP_DELETE) {
-    const uint32_t fw = ty == OP_LOOKUP ? kNoWrite : d.z, cap = mask + 1u;
+  if (key != 0u && (ty == OP_INSERT || ty == OP_DELETE)) {
+    const uint32_t fw = d.z, cap = mask + 1u;
     uint32_t h = win_hash(key) & mask;
     for (uint32_t probe = 0; probe < cap && e == kNoEntry; ++probe, h = (h + 1u) & mask) {
       unsigned long long cur = coherent ? ld_sync(set + h) : set[h];
@@ -130,18 +133,33 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const
 // (wavefront-aggregated), and their batch folded into round 0's word.  Ops up
 // to and including that batch keep their type: before it the key is only
 // looked up, so they see the table as it was before the window.
-__global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, const uint8_t* types, uint32_t wlo,
-                                                         const unsigned long long* set, const uint32_t* eidx,
-                                                         uint8_t* wtype, unsigned long long* rw, unsigned long long seq,
-                                                         uint32_t* pidx, uint32_t* pbid, uint32_t* pe, WinCtl ctl) {
+__global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, const uint8_t* types, const uint32_t* keys,
+                                                         uint32_t wlo, const unsigned long long* set, uint32_t mask,
+                                                         uint32_t epoch, const uint32_t* eidx, uint8_t* wtype,
+                                                         unsigned long long* rw, unsigned long long seq, uint32_t* pidx,
+                                                         uint32_t* pbid, uint32_t* pe, WinCtl ctl) {
   const uint4 d = desc[blockIdx.x];
   const uint32_t i = d.x + threadIdx.x;
   bool pend = false;
   uint32_t e = kNoEntry;
   if (i < d.y) {
     int ty = types[i];
-    e = eidx[i - wlo];
-    if (e != kNoEntry) pend = d.z > went_fw(set[e]);
+    if (ty == OP_LOOKUP) {
+      const uint32_t key = keys[i];
+      // the key's entry exists only if the window writes the key
+      for (uint32_t h = win_hash(key) & mask, probe = 0; key != 0u && probe <= mask; ++probe, h = (h + 1u) & mask) {
+        const unsigned long long en = set[h];
+        if (went_epoch(en) != epoch) break;  // a free slot ends the probe: no write of the key
+        if (static_cast<uint32_t>(en) == key) {
+          e = h;
+          pend = d.z > went_fw(en);
+          break;
+        }
+      }
+    } else {
+      e = eidx[i - wlo];
+      if (e != kNoEntry) pend = d.z > went_fw(set[e]);
+    }
     if (ty == OP_DEFERRED) ty = HIVE_OP_NONE;  // not an input type
     wtype[i - wlo] = pend ? static_cast<uint8_t>(OP_DEFERRED) : static_cast<uint8_t>(ty);
   }
@@ -497,8 +515,9 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
     k_win_mark<<<ntiles, kWinBlock, 0, pl.ms>>>(d_desc + w.t0, types, keys, static_cast<uint32_t>(lo), set, mask,
                                                 t->win_epoch, coherent, wb.eidx, wb.ctl);
     HIP_CHECK_LAUNCH();
-    k_win_level<<<ntiles, kWinBlock, 0, pl.ms>>>(d_desc + w.t0, types, static_cast<uint32_t>(lo), set, wb.eidx,
-                                                 wb.wtype, wb.rw, seq, wb.pidx, wb.pbid, wb.pe, wb.ctl);
+    k_win_level<<<ntiles, kWinBlock, 0, pl.ms>>>(d_desc + w.t0, types, keys, static_cast<uint32_t>(lo), set, mask,
+                                                 t->win_epoch, wb.eidx, wb.wtype, wb.rw, seq, wb.pidx, wb.pbid, wb.pe,
+                                                 wb.ctl);
     HIP_CHECK_LAUNCH();
     HIP_TRY(hipEventRecord(pl.ready[p], pl.ms));
     HIP_TRY(hipStreamWaitEvent(s, pl.ready[p], 0));
