@@ -425,6 +425,24 @@ def cpu_baseline(keys, vals, lk, threads, gpu_found, gpu_vals, policy, nb):
     }
 
 
+def mixed_summary(mixed):
+    """The metric's second half: the fastest stream form on the uneven mixed
+    batches, its per-batch p50/p99 latency, and its speedup over per-batch
+    launches (same results: modes_agree)."""
+    modes = (mixed or {}).get("modes") or {}
+    if not modes:
+        return None
+    best = min(modes, key=lambda m: modes[m]["total_ms"])
+    out = {"form": best, "total_ms": modes[best]["total_ms"], "mops": modes[best]["mops"],
+           "p50_batch_us": modes[best]["p50_batch_us"], "p99_batch_us": modes[best]["p99_batch_us"],
+           "results_agree": mixed.get("modes_agree")}
+    if "launches" in modes:
+        out["launches_p50_batch_us"] = modes["launches"]["p50_batch_us"]
+        out["launches_p99_batch_us"] = modes["launches"]["p99_batch_us"]
+        out["speedup_vs_launches"] = round(modes["launches"]["total_ms"] / modes[best]["total_ms"], 3)
+    return out
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -632,6 +650,7 @@ def main():
             "gather_64B_per_s_by_lanes_per_bucket": lane_rates,
             "footprint_bytes": nb * 64,
         },
+        "mixed_latency": mixed_summary(mixed),
         "mixed_stream": mixed,
         "serving": serving,
         "lf_exist_sweep": sweep,
