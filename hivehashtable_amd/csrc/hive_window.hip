@@ -57,12 +57,17 @@ __device__ __forceinline__ uint32_t went_epoch(unsigned long long e) { return st
 __device__ __forceinline__ uint32_t went_fw(unsigned long long e) { return static_cast<uint32_t>(e >> 32) & 0xFFFFu; }
 __device__ __forceinline__ uint32_t win_hash(uint32_t key) { return fmix32(key * 0x9E3779B1u + 0x7F4A7C15u); }
 
-// Round word of a key: the earliest batch among its pending ops, tagged with
-// the window and round it belongs to: (window seq << 12 | round) << 16 |
+// Round word of a key: the earliest batch among its pending writes, tagged
+// with the window and round it belongs to: (window seq << 12 | round) << 16 |
 // (0xFFFF - batch).  atomicMax keeps the newest tag and, within it, the
-// earliest batch, so the words never need clearing.
+// earliest batch, so the words never need clearing.  A word without the
+// round's tag means the key has no pending write.
 __device__ __forceinline__ unsigned long long round_word(unsigned long long seq, uint32_t r, uint32_t b) {
   return (((seq << 12) | r) << 16) | (0xFFFFu - b);
+}
+// the earliest pending written batch of a key in round r (kNoWrite: none)
+__device__ __forceinline__ uint32_t round_write_batch(unsigned long long w, unsigned long long seq, uint32_t r) {
+  return (w >> 16) == ((seq << 12) | r) ? 0xFFFFu - static_cast<uint32_t>(w & 0xFFFFu) : kNoWrite;
 }
 
 // control block of one window buffer, one 128-byte line per counter, all
@@ -129,10 +134,10 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const
 }
 
 // Ops in a batch after their key's first write in the window are deferred:
-// typed OP_DEFERRED for the phase kernels, appended to the pending list
-// (wavefront-aggregated), and their batch folded into round 0's word.  Ops up
-// to and including that batch keep their type: before it the key is only
-// looked up, so they see the table as it was before the window.
+// typed OP_DEFERRED for the phase kernels and appended to the pending list
+// (wavefront-aggregated); deferred writes fold their batch into round 0's
+// word.  Ops up to and including that batch keep their type: before it the
+// key is only looked up, so they see the table as it was before the window.
 __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, const uint8_t* types, const uint32_t* keys,
                                                          uint32_t wlo, const unsigned long long* set, uint32_t mask,
                                                          uint32_t epoch, const uint32_t* eidx, uint8_t* wtype,
@@ -175,7 +180,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, cons
     pidx[pos] = i;
     pbid[pos] = d.z;
     pe[pos] = e;
-    atomicMax(rw + e, round_word(seq, 0, d.z));
+    if (types[i] != OP_LOOKUP) atomicMax(rw + e, round_word(seq, 0, d.z));
   }
 }
 
@@ -242,14 +247,17 @@ __device__ void stamp_window(const RoundArgs& a) {
     __hip_atomic_store(a.ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deferred ops, round by round.  Round r runs the pending ops whose batch is
-// their key's earliest pending one (round word of slot r%2, complete since
-// round r-1 or k_win_level), and folds the batches of the ops that stay into
-// slot (r+1)%2 under round r+1's tag.  The lookup pass also counts the
-// round's deletes and inserts and the ops that stay, so the barriers after it
-// are only taken when a later pass has work.  Table reads are device-coherent:
-// passes are separated by barriers, not kernel boundaries.  Block 0 stamps
-// the window's batches complete at the end.
+// Deferred ops, round by round.  With m the key's earliest pending written
+// batch (round word of slot r%2, complete since round r-1 or k_win_level; no
+// word: no pending write), round r runs the key's pending lookups of batches
+// up to m -- lookups between two writes commute, so they all see the writes
+// before them -- then the deletes and inserts of batch m.  Writes that stay
+// fold their batch into slot (r+1)%2 under round r+1's tag.  The rounds a key
+// needs are its written batches, not every batch that touches it.  The lookup
+// pass also counts the round's deletes and inserts and the ops that stay, so
+// the barriers after it are only taken when a later pass has work.  Table
+// reads are device-coherent: passes are separated by barriers, not kernel
+// boundaries.  The last block stamps the window's batches complete.
 // Only the blocks the pending list needs (one op per thread where it fits)
 // take part; the others return at once.
 template <class P0>
@@ -283,16 +291,21 @@ __global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundAr
       const uint32_t b = a.pbid[j];
       if (b == kDoneBatch) continue;
       const uint32_t e = a.pe[j];
-      if (ld_sync(cur + e) != round_word(a.seq, r, b)) {
-        atomicMax(nxt + e, round_word(a.seq, r + 1u, b));
-        ++stay;
-        continue;
-      }
+      const uint32_t m = round_write_batch(ld_sync(cur + e), a.seq, r);
       const uint32_t i = a.pidx[j];
       const int ty = a.types[i];
       if (ty != OP_LOOKUP) {
+        if (b != m) {  // a later written batch: a later round
+          atomicMax(nxt + e, round_word(a.seq, r + 1u, b));
+          ++stay;
+          continue;
+        }
         ndel += ty == OP_DELETE;
         nins += ty == OP_INSERT;
+        continue;
+      }
+      if (b > m) {  // after this round's write of the key
+        ++stay;
         continue;
       }
       uint32_t v;
@@ -317,7 +330,7 @@ __global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundAr
     if (ld_sync32(a.ctl.count(OP_DELETE, cs)) != 0u) {
       for (uint32_t j = tid; j < npend; j += stride) {  // deletes of this round
         const uint32_t b = a.pbid[j];
-        if (b == kDoneBatch || ld_sync(cur + a.pe[j]) != round_word(a.seq, r, b)) continue;
+        if (b == kDoneBatch || round_write_batch(ld_sync(cur + a.pe[j]), a.seq, r) != b) continue;
         const uint32_t i = a.pidx[j];
         if (a.types[i] != OP_DELETE) continue;
         int rt, rs;
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundAr
     if (any_ins) {
       for (uint32_t j = tid; j < npend; j += stride) {  // inserts of this round
         const uint32_t b = a.pbid[j];
-        if (b == kDoneBatch || ld_sync(cur + a.pe[j]) != round_word(a.seq, r, b)) continue;
+        if (b == kDoneBatch || round_write_batch(ld_sync(cur + a.pe[j]), a.seq, r) != b) continue;
         const uint32_t i = a.pidx[j];
         const uint32_t key = a.keys[i];
         const uint32_t val = a.vals ? a.vals[i] : key + 1u;
