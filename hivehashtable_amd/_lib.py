@@ -70,7 +70,8 @@ class BulkTune(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("max_batch_ops", C.c_uint64), ("max_inflight", C.c_uint32), ("max_load_factor", C.c_double),
-                ("growth", C.c_double), ("hbm_reserve_bytes", C.c_uint64), ("stash_headroom", C.c_double)]
+                ("growth", C.c_double), ("hbm_reserve_bytes", C.c_uint64), ("stash_headroom", C.c_double),
+                ("ordered", C.c_uint32)]
 
 
 class ServeReport(C.Structure):

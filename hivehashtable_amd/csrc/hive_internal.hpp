@@ -200,6 +200,10 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
                               const uint64_t* off, uint32_t* out_vals, uint8_t* status, unsigned int* ctrs,
                               unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
                               int ncu, unsigned long long wait_ticks, hipStream_t s);
+hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                             uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
+                             uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
+                             unsigned long long wait_ticks, hipStream_t ks, hipStream_t s, hipEvent_t ready);
 // the sticky error word of the window control block (nonzero: a round barrier gave up)
 unsigned int* window_error_word(const hive_table* t);
 

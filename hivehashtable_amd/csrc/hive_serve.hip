@@ -94,7 +94,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   if (c.hbm_reserve_bytes == 0) c.hbm_reserve_bytes = 1ull << 30;
   if (c.stash_headroom == 0.0) c.stash_headroom = 0.5;
   if (c.max_inflight > 8 || !(c.max_load_factor > 0.0 && c.max_load_factor <= 1.0) || !(c.growth > 1.0) ||
-      !(c.stash_headroom > 0.0 && c.stash_headroom <= 1.0))
+      !(c.stash_headroom > 0.0 && c.stash_headroom <= 1.0) || c.ordered > 1)
     return fail(HIVE_EINVAL, "serve config out of range");
   hive_serve_report_t rep{};
   rep.requests = nreq;
@@ -127,6 +127,39 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   for (size_t i = 0; i < nreq; ++i) max_req = std::max<uint64_t>(max_req, h_req_offsets[i + 1] - h_req_offsets[i]);
   if ((r = ensure_partition(t, std::min<uint64_t>(nops, std::max<uint64_t>(c.max_batch_ops, max_req)))) != HIVE_OK)
     return r;
+  // ordered: tiles of 256 ops of one request each (numbered by request) for
+  // every request of the trace, and window scratch for the largest batch
+  DevArena ord_ar;
+  std::vector<uint32_t> tile0;  // first tile of request r
+  uint4* d_desc = nullptr;
+  int ncu = 0;
+  unsigned long long wait_ticks = 0;
+  if (c.ordered) {
+    if (nops > 0xFFFFFFFFull || nreq > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "ordered serving: more than 2^32-1 ops");
+    std::vector<uint4> desc;
+    tile0.resize(nreq + 1);
+    for (size_t q = 0; q < nreq; ++q) {
+      tile0[q] = static_cast<uint32_t>(desc.size());
+      for (uint64_t i = h_req_offsets[q]; i < h_req_offsets[q + 1]; i += 256) {
+        if (desc.size() >= 0x7FFFFFFFu) return fail(HIVE_EINVAL, "ordered serving: too many tiles");
+        desc.push_back(make_uint4(static_cast<uint32_t>(i),
+                                  static_cast<uint32_t>(std::min<uint64_t>(i + 256, h_req_offsets[q + 1])),
+                                  static_cast<uint32_t>(q), 0u));
+      }
+    }
+    tile0[nreq] = static_cast<uint32_t>(desc.size());
+    HIP_TRY(ord_ar.alloc(&d_desc, desc.size()));
+    if (!desc.empty())
+      HIP_TRY(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+    if ((r = ensure_window(t, std::min<uint64_t>(nops, std::max<uint64_t>(c.max_batch_ops, max_req)))) != HIVE_OK)
+      return r;
+    HIP_TRY(hipMemsetAsync(window_error_word(t), 0, 4, s));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, t->device));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
+    wait_ticks = static_cast<unsigned long long>(10.0 * 1e3 * khz);  // a round barrier gives up after 10 s
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   hive_stats_t st{};
   if ((r = hive_stats(t, &st, stream)) != HIVE_OK) return r;
   uint64_t capacity = st.capacity;
@@ -182,7 +215,10 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       // coalesce every arrived request, up to max_batch_ops (at least one request)
       size_t end = launched + 1;
       uint64_t ins = h_ins[launched];
-      while (end < arrived && h_req_offsets[end + 1] - h_req_offsets[launched] <= c.max_batch_ops) ins += h_ins[end++];
+      const size_t max_req_per_batch = c.ordered ? kMaxWindowBatches : ~size_t(0);
+      while (end < arrived && end - launched < max_req_per_batch &&
+             h_req_offsets[end + 1] - h_req_offsets[launched] <= c.max_batch_ops)
+        ins += h_ins[end++];
       if (ins && !fits(ins)) {
         if ((r = retire(true)) != HIVE_OK || (r = refresh()) != HIVE_OK) return r;
         if (!fits(ins)) {
@@ -231,9 +267,16 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
         }
       }
       const size_t o = h_req_offsets[launched], n = h_req_offsets[end] - o;
-      SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
-      OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
-      if ((r = launch_mixed_native(t, src, n, out, s)) != HIVE_OK) return r;
+      if (c.ordered && end - launched > 1) {  // each request a batch of one window
+        if (n && (r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
+                                     tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, 0,
+                                     ncu, wait_ticks, s, s, nullptr)) != HIVE_OK)
+          return r;
+      } else {
+        SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
+        OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
+        if ((r = launch_mixed_native(t, src, n, out, s)) != HIVE_OK) return r;
+      }
       hipEvent_t ev = pool.free.back();
       pool.free.pop_back();
       HIP_TRY(hipEventRecord(ev, s));
@@ -245,6 +288,11 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     }
   }
   HIP_TRY(hipStreamSynchronize(s));
+  if (c.ordered) {
+    unsigned int err = 0;
+    HIP_TRY(hipMemcpy(&err, window_error_word(t), 4, hipMemcpyDeviceToHost));
+    if (err) return fail(HIVE_EDEVICE, "ordered serving: a round barrier did not complete");
+  }
   if ((r = refresh()) != HIVE_OK) return r;
   std::vector<double> served;
   served.reserve(nreq);

@@ -93,15 +93,17 @@ constexpr uint32_t kCtlCounters = 12;
 constexpr uint32_t kCtlWords = (kCtlCounters + 1) * kLine;  // + the shared error word's line
 
 // One block per tile of 256 ops of one batch: desc = {first op, end op,
-// batch index in the window, -}.  Block 0 also resets the control block.
+// batch number, -}; the batch's index in the window is its number - base.
+// Block 0 also resets the control block.
 // Only writes (inserts, deletes) enter the set: a key's first written batch
 // is all that decides deferral, and lookups find their key's entry, if any,
 // by a read-only probe in k_win_level.  Slot reads are plain (a stale view
 // only makes a CAS fail and return the current entry) except right after the
 // set was cleared (coherent = 1).
-__global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const uint8_t* types, const uint32_t* keys,
-                                                        uint32_t wlo, unsigned long long* set, uint32_t mask,
-                                                        uint32_t epoch, uint32_t coherent, uint32_t* eidx, WinCtl ctl) {
+__global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint32_t base, const uint8_t* types,
+                                                        const uint32_t* keys, uint32_t wlo, unsigned long long* set,
+                                                        uint32_t mask, uint32_t epoch, uint32_t coherent, uint32_t* eidx,
+                                                        WinCtl ctl) {
   if (blockIdx.x == 0 && threadIdx.x < kCtlCounters)
     __hip_atomic_store(ctl.c + threadIdx.x * kLine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint4 d = desc[blockIdx.x];
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const
   const uint32_t key = keys[i];
   uint32_t e = kNoEntry;
   if (key != 0u && (ty == OP_INSERT || ty == OP_DELETE)) {
-    const uint32_t fw = d.z, cap = mask + 1u;
+    const uint32_t fw = d.z - base, cap = mask + 1u;
     uint32_t h = win_hash(key) & mask;
     for (uint32_t probe = 0; probe < cap && e == kNoEntry; ++probe, h = (h + 1u) & mask) {
       unsigned long long cur = coherent ? ld_sync(set + h) : set[h];
@@ -138,13 +140,15 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, const
 // (wavefront-aggregated); deferred writes fold their batch into round 0's
 // word.  Ops up to and including that batch keep their type: before it the
 // key is only looked up, so they see the table as it was before the window.
-__global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, const uint8_t* types, const uint32_t* keys,
-                                                         uint32_t wlo, const unsigned long long* set, uint32_t mask,
-                                                         uint32_t epoch, const uint32_t* eidx, uint8_t* wtype,
-                                                         unsigned long long* rw, unsigned long long seq, uint32_t* pidx,
-                                                         uint32_t* pbid, uint32_t* pe, WinCtl ctl) {
+__global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint32_t base, const uint8_t* types,
+                                                         const uint32_t* keys, uint32_t wlo,
+                                                         const unsigned long long* set, uint32_t mask, uint32_t epoch,
+                                                         const uint32_t* eidx, uint8_t* wtype, unsigned long long* rw,
+                                                         unsigned long long seq, uint32_t* pidx, uint32_t* pbid,
+                                                         uint32_t* pe, WinCtl ctl) {
   const uint4 d = desc[blockIdx.x];
   const uint32_t i = d.x + threadIdx.x;
+  const uint32_t bw = d.z - base;  // batch index in the window
   bool pend = false;
   uint32_t e = kNoEntry;
   if (i < d.y) {
@@ -157,13 +161,13 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, cons
         if (went_epoch(en) != epoch) break;  // a free slot ends the probe: no write of the key
         if (static_cast<uint32_t>(en) == key) {
           e = h;
-          pend = d.z > went_fw(en);
+          pend = bw > went_fw(en);
           break;
         }
       }
     } else {
       e = eidx[i - wlo];
-      if (e != kNoEntry) pend = d.z > went_fw(set[e]);
+      if (e != kNoEntry) pend = bw > went_fw(set[e]);
     }
     if (ty == OP_DEFERRED) ty = HIVE_OP_NONE;  // not an input type
     wtype[i - wlo] = pend ? static_cast<uint8_t>(OP_DEFERRED) : static_cast<uint8_t>(ty);
@@ -172,15 +176,15 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, cons
   if (m == 0ull) return;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(m));
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(ctl.pending(0), static_cast<uint32_t>(__popcll(m)));
-  base = __shfl(base, static_cast<int>(leader));
+  uint32_t first = 0;
+  if (lane == leader) first = atomicAdd(ctl.pending(0), static_cast<uint32_t>(__popcll(m)));
+  first = __shfl(first, static_cast<int>(leader));
   if (pend) {
-    const uint32_t pos = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+    const uint32_t pos = first + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
     pidx[pos] = i;
-    pbid[pos] = d.z;
+    pbid[pos] = bw;
     pe[pos] = e;
-    if (types[i] != OP_LOOKUP) atomicMax(rw + e, round_word(seq, 0, d.z));
+    if (types[i] != OP_LOOKUP) atomicMax(rw + e, round_word(seq, 0, bw));
   }
 }
 
@@ -476,6 +480,52 @@ struct Pipeline {
 };
 }  // namespace
 
+// One multi-batch window: ops [lo, lo + n), described by ntiles tiles from
+// desc (batch numbers base .. base + nbatches - 1), in window buffer p.  The
+// key-set passes run on ks, the table passes on s; when they differ, s waits
+// for the key-set passes through `ready`, and the caller orders ks after the
+// buffer's last use.  ts/nts: completion stamps of the window's batches (nts
+// may be 0).
+hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                             uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
+                             uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
+                             unsigned long long wait_ticks, hipStream_t ks, hipStream_t s, hipEvent_t ready) {
+  const size_t cap = t->win_cap;
+  unsigned long long* set = t->win_set;
+  const uint32_t mask = static_cast<uint32_t>(cap - 1);
+  const WindowBuffer wb = window_buffer(t, p);
+  uint32_t coherent = 0;
+  if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry
+    k_fill_u64<<<fill_grid(cap), 256, 0, ks>>>(set, cap, 0ull);
+    HIP_CHECK_LAUNCH();
+    t->win_epoch = 1;
+    coherent = 1;
+  }
+  const unsigned long long seq = ++t->win_seq;
+  if (seq >= (1ull << 36)) return fail(HIVE_EINVAL, "windowed batches: window sequence exhausted");
+  k_win_mark<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
+                                           t->win_epoch, coherent, wb.eidx, wb.ctl);
+  HIP_CHECK_LAUNCH();
+  k_win_level<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
+                                            t->win_epoch, wb.eidx, wb.wtype, wb.rw, seq, wb.pidx, wb.pbid, wb.pe,
+                                            wb.ctl);
+  HIP_CHECK_LAUNCH();
+  if (ks != s) {
+    HIP_TRY(hipEventRecord(ready, ks));
+    HIP_TRY(hipStreamWaitEvent(s, ready, 0));
+  }
+  hive_status_t r = launch_mixed_native(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, wb.wtype}, n,
+                                        OutNative{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr}, s);
+  if (r != HIVE_OK) return r;
+  RoundArgs ra{types, keys, vals, out_vals, status, wb.pidx, wb.pbid, wb.pe, wb.rw, static_cast<uint32_t>(cap), seq,
+               wb.ctl, wait_ticks, ts, nts};
+  // a small grid, every block resident: the rounds are separated by grid barriers
+  const unsigned round_grid = static_cast<unsigned>(std::max(1, std::min(ncu, 64)));
+  HIVE_POLICY_SWITCH(t->policy, (k_win_rounds<P><<<round_grid, kRoundBlock, 0, s>>>(t->v, ra)));
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
 // Multi-batch windows alternate between two buffers.  The key-set passes
 // (k_win_mark, k_win_level) touch only the inputs and the window buffer, so
 // they run on a side stream one window ahead, overlapping the table work of
@@ -485,11 +535,6 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
                               const uint64_t* off, uint32_t* out_vals, uint8_t* status, unsigned int* ctrs,
                               unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
                               int ncu, unsigned long long wait_ticks, hipStream_t s) {
-  const size_t cap = t->win_cap;
-  unsigned long long* set = t->win_set;
-  const uint32_t mask = static_cast<uint32_t>(cap - 1);
-  // a small grid, every block resident: the rounds are separated by grid barriers
-  const unsigned round_grid = static_cast<unsigned>(std::max(1, std::min(ncu, 64)));
   Pipeline pl;
   HIP_TRY(hipStreamCreateWithFlags(&pl.ms, hipStreamNonBlocking));
   for (hipEvent_t* e : {&pl.start, &pl.done, &pl.ready[0], &pl.ready[1], &pl.free[0], &pl.free[1]})
@@ -514,33 +559,11 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
       continue;
     }
     const int p = static_cast<int>(k++ & 1u);
-    const WindowBuffer wb = window_buffer(t, p);
     if (used[p]) HIP_TRY(hipStreamWaitEvent(pl.ms, pl.free[p], 0));
-    uint32_t coherent = 0;
-    if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry
-      k_fill_u64<<<fill_grid(cap), 256, 0, pl.ms>>>(set, cap, 0ull);
-      HIP_CHECK_LAUNCH();
-      t->win_epoch = 1;
-      coherent = 1;
-    }
-    const unsigned long long seq = ++t->win_seq;
-    if (seq >= (1ull << 36)) return fail(HIVE_EINVAL, "windowed stream: window sequence exhausted");
-    k_win_mark<<<ntiles, kWinBlock, 0, pl.ms>>>(d_desc + w.t0, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                                t->win_epoch, coherent, wb.eidx, wb.ctl);
-    HIP_CHECK_LAUNCH();
-    k_win_level<<<ntiles, kWinBlock, 0, pl.ms>>>(d_desc + w.t0, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                                 t->win_epoch, wb.eidx, wb.wtype, wb.rw, seq, wb.pidx, wb.pbid, wb.pe,
-                                                 wb.ctl);
-    HIP_CHECK_LAUNCH();
-    HIP_TRY(hipEventRecord(pl.ready[p], pl.ms));
-    HIP_TRY(hipStreamWaitEvent(s, pl.ready[p], 0));
-    hive_status_t r = launch_mixed_native(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, wb.wtype}, n,
-                                          OutNative{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr}, s);
+    hive_status_t r = enqueue_window(t, types, keys, vals, lo, n, out_vals, status, d_desc + w.t0, ntiles,
+                                     static_cast<uint32_t>(w.b0), ts + w.b0 + 1, static_cast<uint32_t>(w.b1 - w.b0),
+                                     p, ncu, wait_ticks, pl.ms, s, pl.ready[p]);
     if (r != HIVE_OK) return r;
-    RoundArgs ra{types, keys, vals, out_vals, status, wb.pidx, wb.pbid, wb.pe, wb.rw, static_cast<uint32_t>(cap), seq,
-                 wb.ctl, wait_ticks, ts + w.b0 + 1, static_cast<uint32_t>(w.b1 - w.b0)};
-    HIVE_POLICY_SWITCH(t->policy, (k_win_rounds<P><<<round_grid, kRoundBlock, 0, s>>>(t->v, ra)));
-    HIP_CHECK_LAUNCH();
     HIP_TRY(hipEventRecord(pl.free[p], s));
     used[p] = true;
   }
@@ -552,7 +575,8 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
 // Windows: runs of consecutive batches of fewer than batch_ops ops each, while
 // their ops fit in window_ops (at most kMaxWindowBatches batches); any other
 // batch is a window alone and runs as plain phase kernels.  Tiles of 256 ops
-// (one batch each) describe the ops of multi-batch windows.
+// (one batch each, with its batch number) describe the ops of multi-batch
+// windows.
 hive_status_t plan_windows(const uint64_t* off, size_t nb, uint64_t window_ops, uint64_t batch_ops,
                            std::vector<WindowPlan>& plan, std::vector<uint4>& desc) {
   plan.clear();
@@ -573,7 +597,7 @@ hive_status_t plan_windows(const uint64_t* off, size_t nb, uint64_t window_ops, 
           if (desc.size() >= 0x7FFFFFFFu) return fail(HIVE_EINVAL, "windowed stream: too many tiles");
           desc.push_back(make_uint4(static_cast<uint32_t>(i),
                                     static_cast<uint32_t>(std::min<uint64_t>(i + kWinBlock, off[c + 1])),
-                                    static_cast<uint32_t>(c - b), 0u));
+                                    static_cast<uint32_t>(c), 0u));
         }
     w.t1 = static_cast<uint32_t>(desc.size());
     plan.push_back(w);

@@ -159,11 +159,14 @@ class HiveTable:
         return {f: getattr(res, f) for f, _ in Tune._fields_}
 
     def serve(self, types, keys, vals, req_offsets, arrival_us, max_batch_ops=0, max_inflight=0, max_load_factor=0.0,
-              growth=0.0, hbm_reserve_bytes=0, stash_headroom=0.0, out_vals=None, status=None, stream=None):
+              growth=0.0, hbm_reserve_bytes=0, stash_headroom=0.0, ordered=0, out_vals=None, status=None,
+              stream=None):
         """Replay a request trace with continuous batching (hive_serve_trace).
 
         Request r = ops[req_offsets[r]:req_offsets[r+1]], arriving arrival_us[r]
-        after the call starts.  Returns (out_vals, status, latency_us, report)."""
+        after the call starts; ordered=1: every request sees the writes of the
+        requests before it, also inside one batch.  Returns (out_vals, status,
+        latency_us, report)."""
         torch = _torch()
         ty = _dev(types, torch.uint8)
         k = _dev(keys, torch.int32)
@@ -178,7 +181,7 @@ class HiveTable:
         if status is None:
             status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
         cfg = ServeConfig(int(max_batch_ops), int(max_inflight), float(max_load_factor), float(growth),
-                          int(hbm_reserve_bytes), float(stash_headroom))
+                          int(hbm_reserve_bytes), float(stash_headroom), int(ordered))
         lat = np.zeros(max(nreq, 1), np.float64)
         rep = ServeReport()
         check(self.L.hive_serve_trace(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),

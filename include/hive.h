@@ -246,7 +246,11 @@ hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, c
  * hive_mixed semantics (lookups, then deletes, then inserts).  Requests
  * whose keys do not meet other requests of the same batch get exactly the
  * results they would get alone; ops of different requests on one key in one
- * batch follow the batch's phase order.
+ * batch follow the batch's phase order.  With ordered = 1, a batch of several
+ * requests runs as a window in which each request is a batch of its own
+ * (HIVE_STREAM_WINDOWED), so every request gets the results it would get
+ * alone, run after the requests that arrived before it (at most 4096
+ * requests are coalesced into one batch).
  * Admission: before a batch whose inserts could lift the load factor above
  * max_load_factor, the table is resized in place by `growth` if free HBM
  * keeps hbm_reserve_bytes after the new table; if not, requests that insert
@@ -261,6 +265,7 @@ typedef struct {
   double growth;              /* capacity multiplier of one resize, > 1 (0 = 2.0) */
   uint64_t hbm_reserve_bytes; /* free HBM a resize must leave (0 = 1 GiB) */
   double stash_headroom;      /* admit inserts without a resize while stash use < this (0 = 0.5) */
+  uint32_t ordered;           /* 1 = each request sees every earlier request's writes, also inside a batch */
 } hive_serve_config_t;
 typedef struct {
   uint64_t requests;

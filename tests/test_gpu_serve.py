@@ -144,3 +144,50 @@ def test_serve_rejects_inserts_without_room(H, O):
     s = np_(s)
     assert (s[:500] == 5).all() and (s[500:] == 4).all()
     t.close()
+
+
+def test_serve_ordered_results_same_alone_even_when_requests_conflict(H, O):
+    """ordered=1: requests whose keys collide (a 2^11 key space, no key twice
+    inside one request) are coalesced into big batches, yet each gets exactly
+    the results it gets served alone after the requests before it -- and the
+    oracle's request-by-request execution."""
+    print("[case] serve ordered: conflicting requests", flush=True)
+    slots = 1 << 14
+    rng = np.random.default_rng(511)
+    pre = W.distinct_keys(1 << 10, 512, bits=11)
+    pre_v = W.values(pre.size, 513)
+    parts, sizes = [], []
+    for r in range(80):
+        n = int(rng.integers(1, 200)) if r != 5 else 0  # one empty request
+        parts.append(rng.choice(np.arange(1, 1 << 11, dtype=np.uint32), n, replace=False))
+        sizes.append(n)
+    keys = np.concatenate(parts).astype(np.uint32)
+    types = rng.choice(3, size=keys.size, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    odd = rng.random(keys.size)
+    types[odd < 0.01] = 4
+    keys[(odd >= 0.01) & (odd < 0.02)] = 0
+    vals = rng.integers(0, 2**32, keys.size, dtype=np.uint64).astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    nreq = off.size - 1
+    o = O.Oracle(slots // 8)
+    o.insert(pre, pre_v)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for r in range(nreq):
+        lo, hi = int(off[r]), int(off[r + 1])
+        if hi > lo:
+            ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    keyset = np.sort(o.dump() & 0xFFFFFFFF)
+    for name, cap in (("ordered mix", 4000), ("ordered one batch", 1 << 20), ("alone", 1)):
+        t = H.HiveTable(slots)
+        t.insert(pre, pre_v)
+        gv, gs, lat, rep = t.serve(types, keys, vals, off, np.zeros(nreq), max_batch_ops=cap, max_inflight=2,
+                                   ordered=1)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        print(f"[case] serve {name}: batches {rep['batches']}", flush=True)
+        if name != "alone":
+            assert rep["batches"] < nreq // 2
+        assert np.array_equal(norm(gs), norm(os_)), name
+        assert np.array_equal(gv, ov), name
+        assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), keyset), name
+        t.close()
