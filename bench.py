@@ -239,14 +239,14 @@ def serve_leg(a, H, W, torch, dev):
     out = {"preload_keys": n_pre, "requests": a.serve_requests, "request_ops": [256, 262144], "zipf_theta": 0.99,
            "offered_mops": a.serve_offered_mops, "mixes": {}}
 
-    def run(t, ty, k, v, off, arr, max_batch_ops=0):
+    def run(t, ty, k, v, off, arr, max_batch_ops=0, ordered=0):
         log(f"serve run: {off.size - 1} requests, {int(off[-1])} ops, last arrival {arr[-1] / 1e3:.2f} ms"
-            + (" (one request per batch)" if max_batch_ops == 1 else ""))
+            + (" (one request per batch)" if max_batch_ops == 1 else "") + (" (request order)" if ordered else ""))
         d_ty = torch.from_numpy(ty).to(dev)
         d_k = torch.from_numpy(k.view(np.int32)).to(dev)
         d_v = torch.from_numpy(v.view(np.int32)).to(dev)
         torch.cuda.synchronize()
-        gv, gs, lat, rep = t.serve(d_ty, d_k, d_v, off, arr, max_batch_ops=max_batch_ops)
+        gv, gs, lat, rep = t.serve(d_ty, d_k, d_v, off, arr, max_batch_ops=max_batch_ops, ordered=ordered)
         st = gs.cpu().numpy()
         reads = ty == 1
         res = {"p50_us": round(rep["p50_us"], 2), "p99_us": round(rep["p99_us"], 2), "max_us": round(rep["max_us"], 2),
@@ -272,8 +272,11 @@ def serve_leg(a, H, W, torch, dev):
         ty, k, v, off, arr = W.serving_trace(mix, pre, a.serve_requests, a.serve_offered_mops, a.seed + 5100 + ord(mix))
         # "per_request_launches": the same saturated trace with every request its
         # own mixed batch (plain per-batch launches, no coalescing)
+        # "saturated_ordered": every request sees the earlier requests' writes
+        # inside a batch too (windowed batches, one request per window batch)
         res = {"ops": int(off[-1]), "open": run(t, ty, k, v, off, arr),
                "saturated": run(t, ty, k, v, off, np.zeros_like(arr)),
+               "saturated_ordered": run(t, ty, k, v, off, np.zeros_like(arr), ordered=1),
                "per_request_launches": run(t, ty, k, v, off, np.zeros_like(arr), max_batch_ops=1)}
         res["coalescing_speedup"] = round(res["saturated"]["mops"] / max(res["per_request_launches"]["mops"], 1e-9), 2)
         out["mixes"][mix] = res
@@ -283,11 +286,12 @@ def serve_leg(a, H, W, torch, dev):
                                          fresh=fresh)
     res = {"ops": int(off[-1])}
     slots_d = int(math.ceil(n_pre / 0.899 / 8)) * 8
-    for name, arrivals, cap in (("open", arr, 0), ("saturated", np.zeros_like(arr), 0),
-                                ("per_request_launches", np.zeros_like(arr), 1)):
+    for name, arrivals, cap, ordered in (("open", arr, 0, 0), ("saturated", np.zeros_like(arr), 0, 0),
+                                         ("saturated_ordered", np.zeros_like(arr), 0, 1),
+                                         ("per_request_launches", np.zeros_like(arr), 1, 0)):
         t = serving_table(slots_d)
         res["initial_slots"] = slots_d
-        res[name] = run(t, ty, k, v, off, arrivals, max_batch_ops=cap)
+        res[name] = run(t, ty, k, v, off, arrivals, max_batch_ops=cap, ordered=ordered)
         t.close()
     res["coalescing_speedup"] = round(res["saturated"]["mops"] / max(res["per_request_launches"]["mops"], 1e-9), 2)
     out["mixes"]["D"] = res
