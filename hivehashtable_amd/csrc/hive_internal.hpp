@@ -47,11 +47,10 @@ struct hive_table {
   size_t part_cap = 0;                   // ops the lists can hold
   uint64_t stream_window_ops = 0;        // windowed streams: most ops per window (0 = default)
   uint64_t stream_window_batch = 0;      // windowed streams: batches below this size are coalesced (0 = default)
-  unsigned long long* win_set = nullptr; // windowed streams: key set of the current window, then round words
-  uint32_t* win_buf = nullptr;           // windowed streams: per-op and pending arrays, control block
+  unsigned long long* win_set = nullptr; // windowed streams: key set of the current window, then chain words
+  uint32_t* win_buf = nullptr;           // windowed streams: per-op and pending arrays, control blocks, chains
   size_t win_cap = 0;                    // key-set entries (a power of two; windows of up to win_cap / 2 ops)
   uint32_t win_epoch = 0;                // epoch of the last window (entries of other epochs are free)
-  uint64_t win_seq = 0;                  // sequence number of the last window (tags its round words)
 };
 
 namespace hive {
@@ -199,13 +198,11 @@ hive_status_t plan_windows(const uint64_t* off, size_t nb, uint64_t window_ops, 
 hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                               const uint64_t* off, uint32_t* out_vals, uint8_t* status, unsigned int* ctrs,
                               unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
-                              int ncu, unsigned long long wait_ticks, hipStream_t s);
+                              int ncu, hipStream_t s);
 hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                              uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
                              uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
-                             unsigned long long wait_ticks, hipStream_t ks, hipStream_t s, hipEvent_t ready);
-// the sticky error word of the window control block (nonzero: a round barrier gave up)
-unsigned int* window_error_word(const hive_table* t);
+                             hipStream_t ks, hipStream_t s, hipEvent_t ready);
 
 // device buffers freed together when a call returns
 struct DevArena {

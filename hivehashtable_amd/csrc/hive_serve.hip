@@ -133,7 +133,6 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   std::vector<uint32_t> tile0;  // first tile of request r
   uint4* d_desc = nullptr;
   int ncu = 0;
-  unsigned long long wait_ticks = 0;
   if (c.ordered) {
     if (nops > 0xFFFFFFFFull || nreq > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "ordered serving: more than 2^32-1 ops");
     std::vector<uint4> desc;
@@ -153,11 +152,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       HIP_TRY(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
     if ((r = ensure_window(t, std::min<uint64_t>(nops, std::max<uint64_t>(c.max_batch_ops, max_req)))) != HIVE_OK)
       return r;
-    HIP_TRY(hipMemsetAsync(window_error_word(t), 0, 4, s));
-    int khz = 0;
-    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, t->device));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
-    wait_ticks = static_cast<unsigned long long>(10.0 * 1e3 * khz);  // a round barrier gives up after 10 s
     HIP_TRY(hipStreamSynchronize(s));
   }
   hive_stats_t st{};
@@ -270,7 +265,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       if (c.ordered && end - launched > 1) {  // each request a batch of one window
         if (n && (r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
                                      tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, 0,
-                                     ncu, wait_ticks, s, s, nullptr)) != HIVE_OK)
+                                     ncu, s, s, nullptr)) != HIVE_OK)
           return r;
       } else {
         SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
@@ -288,11 +283,6 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     }
   }
   HIP_TRY(hipStreamSynchronize(s));
-  if (c.ordered) {
-    unsigned int err = 0;
-    HIP_TRY(hipMemcpy(&err, window_error_word(t), 4, hipMemcpyDeviceToHost));
-    if (err) return fail(HIVE_EDEVICE, "ordered serving: a round barrier did not complete");
-  }
   if ((r = refresh()) != HIVE_OK) return r;
   std::vector<double> served;
   served.reserve(nreq);

@@ -306,19 +306,14 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
     if (!desc.empty())
       HIP_TRY(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(ctrs, 0, nbatches * 4, s));
-    HIP_TRY(hipMemsetAsync(window_error_word(t), 0, 4, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (setup_ms)
       *setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     k_stamp<<<1, 64, 0, s>>>(ts);
     HIP_CHECK_LAUNCH();
-    if ((r = enqueue_windows(t, types, keys, vals, h_offsets, out_vals, status, ctrs, ts, plan, d_desc, ncu,
-                             static_cast<unsigned long long>(kWaitLimitSeconds * 1e3 * khz), s)) != HIVE_OK)
+    if ((r = enqueue_windows(t, types, keys, vals, h_offsets, out_vals, status, ctrs, ts, plan, d_desc, ncu, s)) !=
+        HIVE_OK)
       return r;
-    unsigned int err = 0;
-    HIP_TRY(hipMemcpyAsync(&err, window_error_word(t), 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (err) return fail(HIVE_EDEVICE, "windowed stream: a round barrier did not complete");
   } else if (mode == HIVE_STREAM_LAUNCHES || mode == HIVE_STREAM_GRAPH) {
     uint64_t maxn = 0;
     for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
@@ -543,7 +538,6 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
     tmp.stream_window_batch = coalesce[2];
     r = run(HIVE_STREAM_WINDOWED, &ms);
     t->win_epoch = tmp.win_epoch;
-    t->win_seq = tmp.win_seq;
     if (r != HIVE_OK) return r;
   }
   for (uint32_t split = 0; split < 2; ++split)
@@ -558,7 +552,6 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
         double ms = 0;
         r = run(HIVE_STREAM_WINDOWED, &ms);
         t->win_epoch = tmp.win_epoch;  // the key set is shared: later windows continue its epochs
-        t->win_seq = tmp.win_seq;
         if (r != HIVE_OK) return r;
         ++cands;
         if (ms < best.windowed_ms) {

@@ -1,5 +1,6 @@
-// hive_window.hip -- windowed streams: consecutive small batches of a stream
-// run as one window, with the per-batch results of running them one by one.
+// hive_window.hip -- windowed batches: consecutive small batches of a stream
+// (or the requests of one serving batch) run as one window, with the results
+// of running them one by one.
 //
 // No reference counterpart: the reference's harness launches and synchronizes
 // one hive_mixed_kernel per batch (src/hive_dispatch.cu:119-164), and so do
@@ -10,21 +11,22 @@
 // Why the results are those of batch-by-batch execution: the outcome of an op
 // depends only on the ops on the same key before it (the table's capacity
 // aside, see below), and the phases of a batch are ordered lookups, deletes,
-// inserts.  So it suffices that, for every key, its ops run in (batch, phase)
-// order -- except that lookups with no write of the key between them may run
-// in any order.  Per window:
-//   k_win_mark   every op records its key in a window key set, keeping the
-//                first batch of the window that writes (inserts or deletes)
-//                the key;
-//   k_win_level  ops up to that batch keep their type (before it the key is
-//                only looked up); later ones are deferred (OP_DEFERRED) and
-//                appended to a pending list;
-//   phase kernels the window's lookups, deletes, inserts, as one mixed batch:
-//                for each key, its lookups before its first write, then the
-//                ops of that batch in phase order;
-//   k_win_rounds round r runs, for every key with pending ops, the ops of its
-//                earliest remaining batch, again lookups, deletes, inserts,
-//                separated by grid barriers, until no op is pending.
+// inserts.  So it suffices that every op sees, for its key, exactly the
+// writes of the batches before its own (and, for an insert, its own batch's
+// deletes).  Per window:
+//   k_win_mark      every write records its key in a window key set, keeping
+//                   the first batch of the window that writes the key (fw);
+//   k_win_level     ops of batches up to fw keep their type (before fw the
+//                   key is only looked up); later ones are deferred
+//                   (OP_DEFERRED) and appended to a pending list;
+//   phase kernels   the window's lookups, deletes, inserts, as one mixed
+//                   batch: for each key, its lookups before fw, then fw's ops
+//                   in phase order;
+//   chain kernels   the deferred ops of a key form its chain.  Its state
+//                   after fw is looked up once; every deferred op's result
+//                   follows from that state and the latest deferred write
+//                   batch before its own; only the chain's last write batch
+//                   is applied to the table.  No op waits for another.
 // Ops on different keys run concurrently, as inside one batch.  What the
 // table's capacity decides -- whether an insert lands in a bucket or the stash,
 // and an insert failing on a full stash -- can differ, as it can between two
@@ -41,15 +43,14 @@ using namespace hive;
 namespace {
 
 constexpr uint32_t kWinBlock = 256;       // ops per tile of the mark/level kernels
-constexpr uint32_t kRoundBlock = 1024;    // threads per block of the rounds kernel
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
-constexpr uint32_t kDoneBatch = 0xFFFFFFFFu;
-constexpr uint32_t kNoWrite = 0xFFFFu;    // first-write field of a key only looked up
+constexpr uint32_t kNoWrite = 0xFFFFu;    // no written batch
+constexpr uint32_t kListed = 4;           // written batches kept per chain for the chain lookups
 constexpr uint32_t kLine = 32;            // u32 words per control counter (own 128-byte line)
 
 // Window key-set entry: epoch (16) | first batch of the window that writes
-// the key, kNoWrite if none (16) | key (32).  Entries of other epochs are
-// free, so the set is never cleared between windows.
+// the key (16) | key (32).  Entries of other epochs are free, so the set is
+// never cleared between windows.
 __device__ __forceinline__ unsigned long long went(uint32_t epoch, uint32_t fw, uint32_t key) {
   return (static_cast<unsigned long long>((epoch << 16) | fw) << 32) | key;
 }
@@ -57,40 +58,18 @@ __device__ __forceinline__ uint32_t went_epoch(unsigned long long e) { return st
 __device__ __forceinline__ uint32_t went_fw(unsigned long long e) { return static_cast<uint32_t>(e >> 32) & 0xFFFFu; }
 __device__ __forceinline__ uint32_t win_hash(uint32_t key) { return fmix32(key * 0x9E3779B1u + 0x7F4A7C15u); }
 
-// Round word of a key: the earliest batch among its pending writes, tagged
-// with the window and round it belongs to: (window seq << 12 | round) << 16 |
-// (0xFFFF - batch).  atomicMax keeps the newest tag and, within it, the
-// earliest batch, so the words never need clearing.  A word without the
-// round's tag means the key has no pending write.
-__device__ __forceinline__ unsigned long long round_word(unsigned long long seq, uint32_t r, uint32_t b) {
-  return (((seq << 12) | r) << 16) | (0xFFFFu - b);
-}
-// the earliest pending written batch of a key in round r (kNoWrite: none)
-__device__ __forceinline__ uint32_t round_write_batch(unsigned long long w, unsigned long long seq, uint32_t r) {
-  return (w >> 16) == ((seq << 12) | r) ? 0xFFFFu - static_cast<uint32_t>(w & 0xFFFFu) : kNoWrite;
-}
-
-// control block of one window buffer, one 128-byte line per counter, all
-// reset by k_win_mark: [0..2] ops pending for a round, [3] barrier arrivals,
-// [4] barrier generation, [5..7] deletes and [8..10] inserts of a round,
-// [11] blocks finished.  Per-round counters rotate over 3 slots (round r uses
-// slot r%3 and clears slot (r+1)%3 or (r+2)%3 for a later round), so a slot
-// is cleared only after every block has passed a barrier since reading it.
-// The error word is shared by both buffers and sticky for the whole stream.
+// control block of one window buffer, one 128-byte line per word, reset by
+// k_win_mark: [0] nonzero if any op is deferred, [1] blocks of the last chain
+// kernel finished
 struct WinCtl {
   unsigned int* c;
-  unsigned int* err;
-  __device__ unsigned int* pending(uint32_t slot) const { return c + slot * kLine; }
-  __device__ unsigned int* arrive() const { return c + 3 * kLine; }
-  __device__ unsigned int* gen() const { return c + 4 * kLine; }
-  __device__ unsigned int* count(int op, uint32_t slot) const {
-    return c + ((op == OP_DELETE ? 5u : 8u) + slot) * kLine;
-  }
-  __device__ unsigned int* finished() const { return c + 11 * kLine; }
-  __device__ unsigned int* error() const { return err; }
+  __device__ unsigned int* pending() const { return c; }
+  __device__ unsigned int* finished() const { return c + 1 * kLine; }
 };
-constexpr uint32_t kCtlCounters = 12;
-constexpr uint32_t kCtlWords = (kCtlCounters + 1) * kLine;  // + the shared error word's line
+constexpr uint32_t kCtlCounters = 2;
+// internal types of deferred ops whose write carries their chain's final
+// state (set by k_chain_resolve, read by the apply passes)
+constexpr uint8_t kApplyDelete = 0xFC, kApplyInsert = 0xFD;
 
 // One block per tile of 256 ops of one batch: desc = {first op, end op,
 // batch number, -}; the batch's index in the window is its number - base.
@@ -136,23 +115,23 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint3
 }
 
 // Ops in a batch after their key's first write in the window are deferred:
-// typed OP_DEFERRED for the phase kernels and appended to the pending list
-// (wavefront-aggregated); deferred writes fold their batch into round 0's
-// word.  Ops up to and including that batch keep their type: before it the
-// key is only looked up, so they see the table as it was before the window.
+// typed OP_DEFERRED in place, with their key entry and fw << 16 | batch at
+// their window position (no list, so no shared counter), and the window's
+// pending flag raised.  Ops up to and including that batch keep their type:
+// before it the key is only looked up, so they see the table as it was
+// before the window.
 __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint32_t base, const uint8_t* types,
                                                          const uint32_t* keys, uint32_t wlo,
                                                          const unsigned long long* set, uint32_t mask, uint32_t epoch,
-                                                         const uint32_t* eidx, uint8_t* wtype, unsigned long long* rw,
-                                                         unsigned long long seq, uint32_t* pidx, uint32_t* pbid,
+                                                         const uint32_t* eidx, uint8_t* wtype, uint32_t* pbid,
                                                          uint32_t* pe, WinCtl ctl) {
   const uint4 d = desc[blockIdx.x];
   const uint32_t i = d.x + threadIdx.x;
   const uint32_t bw = d.z - base;  // batch index in the window
   bool pend = false;
-  uint32_t e = kNoEntry;
   if (i < d.y) {
     int ty = types[i];
+    uint32_t e = kNoEntry, fw = 0;
     if (ty == OP_LOOKUP) {
       const uint32_t key = keys[i];
       // the key's entry exists only if the window writes the key
@@ -161,89 +140,204 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
         if (went_epoch(en) != epoch) break;  // a free slot ends the probe: no write of the key
         if (static_cast<uint32_t>(en) == key) {
           e = h;
-          pend = bw > went_fw(en);
+          fw = went_fw(en);
+          pend = bw > fw;
           break;
         }
       }
     } else {
       e = eidx[i - wlo];
-      if (e != kNoEntry) pend = bw > went_fw(set[e]);
-    }
-    if (ty == OP_DEFERRED) ty = HIVE_OP_NONE;  // not an input type
-    wtype[i - wlo] = pend ? static_cast<uint8_t>(OP_DEFERRED) : static_cast<uint8_t>(ty);
-  }
-  const unsigned long long m = __ballot(pend);
-  if (m == 0ull) return;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(m));
-  uint32_t first = 0;
-  if (lane == leader) first = atomicAdd(ctl.pending(0), static_cast<uint32_t>(__popcll(m)));
-  first = __shfl(first, static_cast<int>(leader));
-  if (pend) {
-    const uint32_t pos = first + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
-    pidx[pos] = i;
-    pbid[pos] = bw;
-    pe[pos] = e;
-    if (types[i] != OP_LOOKUP) atomicMax(rw + e, round_word(seq, 0, bw));
-  }
-}
-
-// Barrier over the nblk active blocks of the launch (all resident: the grid
-// is at most one block per CU); a lone block needs only its own barrier.
-// Each thread's memory operations are performed before its block arrives.  A
-// wait longer than the limit, or an error raised elsewhere, sets the error
-// flag and returns false in every block that sees it.
-__device__ bool grid_sync(const WinCtl& ctl, uint32_t nblk, unsigned long long wait_ticks) {
-  __shared__ uint32_t s_ok;
-  wait_vmem();
-  __syncthreads();
-  if (nblk == 1u) return true;
-  if (threadIdx.x == 0) {
-    uint32_t ok = 1;
-    const uint32_t g = ld_sync32(ctl.gen());
-    const uint32_t arrived = __hip_atomic_fetch_add(ctl.arrive(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblk - 1u) {
-      __hip_atomic_store(ctl.arrive(), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      wait_vmem();  // the reset is performed before the generation moves on
-      __hip_atomic_fetch_add(ctl.gen(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const unsigned long long t0 = wall_clock64();
-      uint32_t spins = 0;
-      while (ld_sync32(ctl.gen()) == g) {
-        __builtin_amdgcn_s_sleep(2);
-        if ((++spins & 63u) == 0 && (ld_sync32(ctl.error()) != 0u || wall_clock64() - t0 > wait_ticks)) {
-          atomicOr(ctl.error(), 1u);
-          ok = 0;
-          break;
-        }
+      if (e != kNoEntry) {
+        fw = went_fw(set[e]);
+        pend = bw > fw;
       }
     }
-    s_ok = ok;
+    if (ty == OP_DEFERRED || ty == kApplyDelete || ty == kApplyInsert) ty = HIVE_OP_NONE;  // not input types
+    wtype[i - wlo] = pend ? static_cast<uint8_t>(OP_DEFERRED) : static_cast<uint8_t>(ty);
+    if (pend) {
+      pbid[i - wlo] = (fw << 16) | bw;
+      pe[i - wlo] = e;
+    }
   }
-  __syncthreads();
-  return s_ok != 0u;
+  const unsigned long long m = __ballot(pend);
+  if (m != 0ull && (threadIdx.x & 63u) == static_cast<uint32_t>(__builtin_ctzll(m)) && ctl.pending()[0] == 0u)
+    atomicOr(ctl.pending(), 1u);
 }
 
-struct RoundArgs {
+// ---------------------------------------------------------------- chains
+// Chain state, indexed by key entry e or by write-batch slot h.  Every word
+// is tagged with the window's epoch, so nothing is cleared between windows:
+//   head[e]    epoch of the last window whose chain head looked the key up
+//   init[e]    epoch << 33 | found << 32 | value: the key after fw
+//   lastw[e]   epoch << 32 | the chain's last written batch
+//   wc[e]      epoch << 16 | number of written batches in the chain
+//   wl[4e..]   epoch << 16 | batch, the first kListed of them
+//   wbk[h]     epoch << 48 | e << 12 | batch: a (key, written batch) pair
+//   insmax[h]  epoch << 32 | (op + 1) of the pair's last insert
+//   insmin[h]  epoch << 32 | ~op of the pair's first insert
+//   delmin[h]  epoch << 32 | ~op of the pair's first delete
+struct ChainArgs {
   const uint8_t* types;
   const uint32_t* keys;
   const uint32_t* vals;
   uint32_t* out_vals;
   uint8_t* status;
-  const uint32_t* pidx;
-  uint32_t* pbid;             // kDoneBatch once the op has run
-  const uint32_t* pe;
-  unsigned long long* rw;     // 2 slots x cap round words
-  uint32_t cap;
-  unsigned long long seq;     // this window's sequence number
+  uint8_t* wtype;        // the window's internal op types (OP_DEFERRED, kApply*)
+  const uint32_t* pbid;  // per window position: fw << 16 | batch of a deferred op
+  const uint32_t* pe;    // per window position: key entry of a deferred op
+  uint32_t wlo, n;       // the window's first op and op count
+  unsigned long long *wbk, *insmax, *insmin, *delmin, *init, *lastw;
+  uint32_t *head, *wc, *wl;
+  uint32_t mask;   // write-batch slots - 1
+  uint32_t epoch;
+  uint32_t cleared;  // 1 right after the chain words were cleared: a cached word may be older than the clear
   WinCtl ctl;
-  unsigned long long wait_ticks;
-  unsigned long long* ts;     // completion stamps of the window's batches
+  unsigned long long* ts;  // completion stamps of the window's batches
   uint32_t nts;
 };
 
+__device__ __forceinline__ uint32_t tag32(unsigned long long w) { return static_cast<uint32_t>(w >> 32); }
+__device__ __forceinline__ uint32_t wb_hash(uint32_t e, uint32_t b) {
+  return fmix32(e * 0x9E3779B1u + b * 0x85EBCA77u + 0x165667B1u);
+}
+__device__ __forceinline__ unsigned long long wb_word(uint32_t epoch, uint32_t e, uint32_t b) {
+  return (static_cast<unsigned long long>(epoch) << 48) | (static_cast<unsigned long long>(e) << 12) | b;
+}
+// slot of a (key entry, batch) pair noted in this window, or kNoEntry
+__device__ __forceinline__ uint32_t wb_find(const ChainArgs& a, uint32_t e, uint32_t b) {
+  const unsigned long long want = wb_word(a.epoch, e, b);
+  for (uint32_t h = wb_hash(e, b) & a.mask, probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
+    const unsigned long long cur = a.wbk[h];
+    if (cur == want) return h;
+    if (static_cast<uint32_t>(cur >> 48) != a.epoch) return kNoEntry;
+  }
+  return kNoEntry;
+}
+__device__ __forceinline__ uint32_t op_value(const ChainArgs& a, uint32_t i) {
+  return a.vals ? a.vals[i] : a.keys[i] + 1u;
+}
+
+// Chain pass 1 (reads the table, writes none of it): the first op of every
+// chain looks its key up (the state after fw); every deferred write notes
+// its (key, batch) pair, the pair's first/last insert and first delete, and
+// the chain's written batches.
+template <class P>
+__global__ __launch_bounds__(kWinBlock) void k_chain_note(TableView t, ChainArgs a) {
+  if (a.ctl.pending()[0] == 0u) return;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
+    if (a.wtype[j] != OP_DEFERRED) continue;
+    const uint32_t i = a.wlo + j, b = a.pbid[j] & 0xFFFFu, e = a.pe[j];
+    const int ty = a.types[i];
+    // Hot keys put thousands of ops on one chain: every shared word is read
+    // first and only written by an atomic that can still change it (words only
+    // grow between clears; after a clear every op takes the atomic).
+    // The one op that raises head[e] to the epoch is the chain's head.
+    const bool c = a.cleared != 0u;
+    if ((c || a.head[e] != a.epoch) && atomicMax(a.head + e, a.epoch) != a.epoch) {
+      uint32_t v;
+      const bool found = lookup_one<P>(t, a.keys[i], v) == R_FOUND;
+      a.init[e] = (static_cast<unsigned long long>(a.epoch) << 33) | (static_cast<unsigned long long>(found) << 32) | v;
+    }
+    if (ty != OP_INSERT && ty != OP_DELETE) continue;
+    const unsigned long long tag = static_cast<unsigned long long>(a.epoch) << 32;
+    if (c || a.lastw[e] < (tag | b)) atomicMax(a.lastw + e, tag | b);
+    const unsigned long long want = wb_word(a.epoch, e, b);
+    uint32_t h = wb_hash(e, b) & a.mask;
+    bool claimed = false;
+    for (uint32_t probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
+      unsigned long long cur = a.wbk[h];
+      if (cur != want && static_cast<uint32_t>(cur >> 48) != a.epoch) {
+        const unsigned long long seen = cas64(a.wbk + h, cur, want);
+        if (seen == cur) { claimed = true; break; }
+        cur = seen;
+      }
+      if (cur == want) break;
+    }
+    if (claimed) {  // a new written batch of the chain: count it, keep the first kListed
+      uint32_t cur = a.wc[e];
+      for (;;) {
+        const uint32_t n = (cur >> 16) != a.epoch ? 0u : (cur & 0xFFFFu);
+        const uint32_t seen = atomicCAS(a.wc + e, cur, (a.epoch << 16) | (n + 1u));
+        if (seen == cur) {
+          if (n < kListed) a.wl[static_cast<size_t>(e) * kListed + n] = (a.epoch << 16) | b;
+          break;
+        }
+        cur = seen;
+      }
+    }
+    if (ty == OP_INSERT) {
+      if (c || a.insmax[h] < (tag | (i + 1u))) atomicMax(a.insmax + h, tag | (i + 1u));
+      if (c || a.insmin[h] < (tag | (0xFFFFFFFFu - i))) atomicMax(a.insmin + h, tag | (0xFFFFFFFFu - i));
+    } else if (c || a.delmin[h] < (tag | (0xFFFFFFFFu - i))) {
+      atomicMax(a.delmin + h, tag | (0xFFFFFFFFu - i));
+    }
+  }
+}
+
+// Chain pass 2 (no table access): every deferred op's result from the key's
+// state after fw and the latest written batch w of its chain before its own
+// batch b (state after w: present with w's last insert's value if w inserts,
+// absent if it only deletes).  Within batch b, an insert sees b's deletes
+// first; the first delete / insert of b (lowest op) is the one that finds
+// the key as it was, the later ones find it erased / present.  The op that
+// carries the chain's final state (the last insert of the last written
+// batch, or its first delete if it only deletes) is re-typed to be applied.
+__global__ __launch_bounds__(kWinBlock) void k_chain_resolve(ChainArgs a) {
+  if (a.ctl.pending()[0] == 0u) return;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
+    if (a.wtype[j] != OP_DEFERRED) continue;
+    const uint32_t i = a.wlo + j, pb = a.pbid[j], b = pb & 0xFFFFu, fw = pb >> 16, e = a.pe[j];
+    const int ty = a.types[i];
+    const unsigned long long ini = a.init[e];
+    bool present = ((ini >> 32) & 1ull) != 0ull;
+    uint32_t val = static_cast<uint32_t>(ini);
+    // the chain's latest written batch before b
+    const uint32_t wcv = a.wc[e];
+    const uint32_t nw = (wcv >> 16) == a.epoch ? (wcv & 0xFFFFu) : 0u;
+    uint32_t hw = kNoEntry;
+    if (nw <= kListed) {
+      uint32_t w = kNoWrite;
+      for (uint32_t s = 0; s < nw; ++s) {
+        const uint32_t bb = a.wl[static_cast<size_t>(e) * kListed + s] & 0xFFFFu;
+        if (bb < b && (w == kNoWrite || bb > w)) w = bb;
+      }
+      if (w != kNoWrite) hw = wb_find(a, e, w);
+    } else {  // a chain with many written batches: they are dense, walk back
+      for (uint32_t bb = b; bb > fw + 1u && hw == kNoEntry;) hw = wb_find(a, e, --bb);
+    }
+    if (hw != kNoEntry) {
+      const unsigned long long im = a.insmax[hw];
+      present = tag32(im) == a.epoch;
+      val = present ? op_value(a, static_cast<uint32_t>(im) - 1u) : 0u;
+    }
+    uint8_t res;
+    uint32_t out = 0;
+    if (ty == OP_LOOKUP) {
+      res = present ? R_FOUND : R_NOT_FOUND;
+      out = present ? val : 0u;
+    } else if (const uint32_t hb = wb_find(a, e, b); hb == kNoEntry) {
+      res = R_FAILED;  // unreachable: pass 1 noted every deferred write's pair
+    } else {
+      const unsigned long long dm = a.delmin[hb], im = a.insmax[hb];
+      const bool has_del = tag32(dm) == a.epoch, has_ins = tag32(im) == a.epoch;
+      const bool last_batch = static_cast<uint32_t>(a.lastw[e]) == b;
+      if (ty == OP_DELETE) {
+        const bool first = 0xFFFFFFFFu - static_cast<uint32_t>(dm) == i;
+        res = first && present ? R_ERASED : R_NOT_FOUND;
+        if (last_batch && !has_ins && first) a.wtype[j] = kApplyDelete;
+      } else {
+        const bool first = 0xFFFFFFFFu - static_cast<uint32_t>(a.insmin[hb]) == i;
+        res = first && !(present && !has_del) ? R_INSERTED : R_UPDATED;
+        if (last_batch && static_cast<uint32_t>(im) - 1u == i) a.wtype[j] = kApplyInsert;
+      }
+    }
+    if (a.status) a.status[i] = res;
+    if (a.out_vals) a.out_vals[i] = out;
+  }
+}
+
 // one clock reading for every batch of the window (called by a whole block)
-__device__ void stamp_window(const RoundArgs& a) {
+__device__ void stamp_window(const ChainArgs& a) {
   __shared__ unsigned long long s_now;
   if (threadIdx.x == 0) s_now = wall_clock64();
   __syncthreads();
@@ -251,133 +345,48 @@ __device__ void stamp_window(const RoundArgs& a) {
     __hip_atomic_store(a.ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deferred ops, round by round.  With m the key's earliest pending written
-// batch (round word of slot r%2, complete since round r-1 or k_win_level; no
-// word: no pending write), round r runs the key's pending lookups of batches
-// up to m -- lookups between two writes commute, so they all see the writes
-// before them -- then the deletes and inserts of batch m.  Writes that stay
-// fold their batch into slot (r+1)%2 under round r+1's tag.  The rounds a key
-// needs are its written batches, not every batch that touches it.  The lookup
-// pass also counts the round's deletes and inserts and the ops that stay, so
-// the barriers after it are only taken when a later pass has work.  Table
-// reads are device-coherent: passes are separated by barriers, not kernel
-// boundaries.  The last block stamps the window's batches complete.
-// Only the blocks the pending list needs (one op per thread where it fits)
-// take part; the others return at once.
-template <class P0>
-__global__ __launch_bounds__(kRoundBlock) void k_win_rounds(TableView t, RoundArgs a) {
-  using P = CoherentReads<P0>;
-  __shared__ uint32_t s_cnt[3];
-  const uint32_t npend = ld_sync32(a.ctl.pending(0));
-  if (npend == 0u) {  // the same answer in every block; the phase kernels are done
-    if (blockIdx.x == 0) stamp_window(a);
-    return;
+// Chain pass 3: erase the keys whose chain ends deleted.
+template <class P>
+__global__ __launch_bounds__(kWinBlock) void k_chain_erase(TableView t, ChainArgs a) {
+  if (a.ctl.pending()[0] == 0u) return;
+  int dtab = 0, dstash = 0;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
+    if (a.wtype[j] != kApplyDelete) continue;
+    int rt, rs;
+    (void)erase_one<P>(t, a.keys[a.wlo + j], rt, rs);
+    dtab -= rt;
+    dstash -= rs;
   }
-  const uint32_t nblk = min(gridDim.x, (npend + blockDim.x - 1u) / blockDim.x);
-  if (blockIdx.x >= nblk) return;
+  wave_count(t, dtab, dstash);
+}
+
+// Chain pass 4: upsert the final value of the keys whose chain ends with an
+// insert; an insert the chain reports INSERTED takes the table's placement
+// (STASHED) or failure.  The last block to finish stamps the window's batches.
+template <class P>
+__global__ __launch_bounds__(kWinBlock) void k_chain_insert(TableView t, ChainArgs a) {
+  const bool any = a.ctl.pending()[0] != 0u;
   int dtab = 0, dstash = 0, dslow = 0;
-  const uint32_t stride = nblk * blockDim.x;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t r = 0;; ++r) {
-    const unsigned long long* cur = a.rw + (r & 1u) * a.cap;
-    unsigned long long* nxt = a.rw + ((r + 1u) & 1u) * a.cap;
-    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
-    const uint32_t cs = r % 3u;
-    if (tid == 0) {
-      __hip_atomic_store(a.ctl.pending((r + 2u) % 3u), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(a.ctl.count(OP_DELETE, (r + 1u) % 3u), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(a.ctl.count(OP_INSERT, (r + 1u) % 3u), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; any && j < a.n; j += gridDim.x * blockDim.x) {
+    if (a.wtype[j] != kApplyInsert) continue;
+    const uint32_t i = a.wlo + j, key = a.keys[i], val = op_value(a, i);
+    uint8_t r = fast_insert<P>(t, key, val);
+    if (r == R_PENDING) {
+      r = slow_insert<P>(t, key, val);
+      ++dslow;
+      dstash += (r == R_STASHED);
     }
-    __syncthreads();
-    // lookups of this round; ops of later rounds fold their batch into the next word
-    uint32_t stay = 0, ndel = 0, nins = 0;
-    for (uint32_t j = tid; j < npend; j += stride) {
-      const uint32_t b = a.pbid[j];
-      if (b == kDoneBatch) continue;
-      const uint32_t e = a.pe[j];
-      const uint32_t m = round_write_batch(ld_sync(cur + e), a.seq, r);
-      const uint32_t i = a.pidx[j];
-      const int ty = a.types[i];
-      if (ty != OP_LOOKUP) {
-        if (b != m) {  // a later written batch: a later round
-          atomicMax(nxt + e, round_word(a.seq, r + 1u, b));
-          ++stay;
-          continue;
-        }
-        ndel += ty == OP_DELETE;
-        nins += ty == OP_INSERT;
-        continue;
-      }
-      if (b > m) {  // after this round's write of the key
-        ++stay;
-        continue;
-      }
-      uint32_t v;
-      const uint8_t res = lookup_one<P>(t, a.keys[i], v);
-      if (a.status) a.status[i] = res;
-      if (a.out_vals) a.out_vals[i] = v;
-      a.pbid[j] = kDoneBatch;
-    }
-    if (stay) atomicAdd(&s_cnt[0], stay);
-    if (ndel) atomicAdd(&s_cnt[1], ndel);
-    if (nins) atomicAdd(&s_cnt[2], nins);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (s_cnt[0]) atomicAdd(a.ctl.pending((r + 1u) % 3u), s_cnt[0]);
-      if (s_cnt[1]) atomicAdd(a.ctl.count(OP_DELETE, cs), s_cnt[1]);
-      if (s_cnt[2]) atomicAdd(a.ctl.count(OP_INSERT, cs), s_cnt[2]);
-    }
-    if (!grid_sync(a.ctl, nblk, a.wait_ticks)) break;
-    // final after the barrier, and the same in every block
-    const bool more = ld_sync32(a.ctl.pending((r + 1u) % 3u)) != 0u;
-    const bool any_ins = ld_sync32(a.ctl.count(OP_INSERT, cs)) != 0u;
-    if (ld_sync32(a.ctl.count(OP_DELETE, cs)) != 0u) {
-      for (uint32_t j = tid; j < npend; j += stride) {  // deletes of this round
-        const uint32_t b = a.pbid[j];
-        if (b == kDoneBatch || round_write_batch(ld_sync(cur + a.pe[j]), a.seq, r) != b) continue;
-        const uint32_t i = a.pidx[j];
-        if (a.types[i] != OP_DELETE) continue;
-        int rt, rs;
-        const uint8_t res = erase_one<P>(t, a.keys[i], rt, rs);
-        if (a.status) a.status[i] = res;
-        if (a.out_vals) a.out_vals[i] = 0;
-        dtab -= rt;
-        dstash -= rs;
-        a.pbid[j] = kDoneBatch;
-      }
-      if ((any_ins || more) && !grid_sync(a.ctl, nblk, a.wait_ticks)) break;
-    }
-    if (any_ins) {
-      for (uint32_t j = tid; j < npend; j += stride) {  // inserts of this round
-        const uint32_t b = a.pbid[j];
-        if (b == kDoneBatch || round_write_batch(ld_sync(cur + a.pe[j]), a.seq, r) != b) continue;
-        const uint32_t i = a.pidx[j];
-        const uint32_t key = a.keys[i];
-        const uint32_t val = a.vals ? a.vals[i] : key + 1u;
-        uint8_t res = fast_insert<P>(t, key, val);
-        if (res == R_PENDING) {
-          res = slow_insert<P>(t, key, val);
-          ++dslow;
-          dstash += (res == R_STASHED);
-        }
-        dtab += (res == R_INSERTED);
-        if (a.status) a.status[i] = res;
-        if (a.out_vals) a.out_vals[i] = 0;
-        a.pbid[j] = kDoneBatch;
-      }
-      if (more && !grid_sync(a.ctl, nblk, a.wait_ticks)) break;
-    }
-    if (!more) break;
+    dtab += (r == R_INSERTED);
+    if (a.status && (r == R_FAILED || (r == R_STASHED && a.status[i] == R_INSERTED))) a.status[i] = r;
   }
-  // the last block to finish stamps the window's batches complete
+  wave_count(t, dtab, dstash, dslow);
   __shared__ uint32_t s_last;
   wait_vmem();
   __syncthreads();
   if (threadIdx.x == 0)
-    s_last = __hip_atomic_fetch_add(a.ctl.finished(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nblk - 1u;
+    s_last = __hip_atomic_fetch_add(a.ctl.finished(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
   __syncthreads();
   if (s_last) stamp_window(a);
-  wave_count(t, dtab, dstash, dslow);
 }
 
 __global__ void k_stamp_range(unsigned long long* ts, uint32_t n) {
@@ -393,19 +402,20 @@ __global__ void k_stamp_range(unsigned long long* ts, uint32_t n) {
 namespace hive {
 
 namespace {
-// win_set (u64): the key set [cap], then per buffer the round words [2 cap].
-// win_buf (u32): per buffer eidx, pidx, pbid, pe [cap / 2 each], wtype
-// [cap / 2 bytes], its control block on a 128-byte line; then the shared
-// error word's line.
+// win_set (u64, 7 cap): the key set, then the chain words wbk, insmax,
+// insmin, delmin, init, lastw [cap each].
+// win_buf (u32): per window buffer eidx, pbid, pe [cap / 2 each, by window
+// position], wtype [cap / 2 bytes] and its control block on a 128-byte line;
+// then the chain words head, wc [cap each], wl [kListed cap].
 size_t ctl_offset(size_t cap) {
   const size_t half = cap / 2;
-  return (4 * half + (half + 3) / 4 + kLine - 1) / kLine * kLine;
+  return (3 * half + (half + 3) / 4 + kLine - 1) / kLine * kLine;
 }
 size_t buffer_words(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
+size_t chain_words(size_t cap) { return (2 + kListed) * cap; }
 
 struct WindowBuffer {
-  unsigned long long* rw;
-  uint32_t *eidx, *pidx, *pbid, *pe;
+  uint32_t *eidx, *pbid, *pe;
   uint8_t* wtype;
   WinCtl ctl;
 };
@@ -414,32 +424,46 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
   const size_t cap = t->win_cap, half = cap / 2;
   uint32_t* base = t->win_buf + static_cast<size_t>(p) * buffer_words(cap);
   WindowBuffer w;
-  w.rw = t->win_set + cap + static_cast<size_t>(p) * 2 * cap;
   w.eidx = base;
-  w.pidx = base + half;
-  w.pbid = base + 2 * half;
-  w.pe = base + 3 * half;
-  w.wtype = reinterpret_cast<uint8_t*>(base + 4 * half);
-  w.ctl = WinCtl{base + ctl_offset(cap), t->win_buf + 2 * buffer_words(cap)};
+  w.pbid = base + half;
+  w.pe = base + 2 * half;
+  w.wtype = reinterpret_cast<uint8_t*>(base + 3 * half);
+  w.ctl = WinCtl{base + ctl_offset(cap)};
   return w;
+}
+
+// the chain words: one set, since the chain passes of consecutive windows
+// run in order on the table stream
+ChainArgs chain_args(const hive_table* t) {
+  const size_t cap = t->win_cap;
+  ChainArgs a{};
+  unsigned long long* q = t->win_set + cap;
+  a.wbk = q;
+  a.insmax = q + cap;
+  a.insmin = q + 2 * cap;
+  a.delmin = q + 3 * cap;
+  a.init = q + 4 * cap;
+  a.lastw = q + 5 * cap;
+  uint32_t* c = t->win_buf + 2 * buffer_words(cap);
+  a.head = c;
+  a.wc = c + cap;
+  a.wl = c + 2 * cap;
+  a.mask = static_cast<uint32_t>(cap - 1);
+  return a;
 }
 }  // namespace
 
-unsigned int* window_error_word(const hive_table* t) {
-  return t->win_buf ? t->win_buf + 2 * buffer_words(t->win_cap) : nullptr;
-}
-
-// key set (2 entries per op, a power of two) and two window buffers (round
-// words, per-op and pending arrays) for windows of up to `ops` ops
+// key set (2 entries per op, a power of two), chain words, and two window
+// buffers (per-op and pending arrays) for windows of up to `ops` ops
 hive_status_t ensure_window(hive_table* t, size_t ops) {
   size_t cap = 1024;
   while (cap < 2 * ops) cap <<= 1;
   if (t->win_set && cap <= t->win_cap) return HIVE_OK;
-  if (cap > (size_t(1) << 31)) return fail(HIVE_EINVAL, "window too large");
-  const size_t words = 2 * buffer_words(cap) + kLine;
+  if (cap > (size_t(1) << 26)) return fail(HIVE_EINVAL, "window too large");
+  const size_t words = 2 * buffer_words(cap) + chain_words(cap);
   unsigned long long* set = nullptr;
   uint32_t* buf = nullptr;
-  if (hipMalloc(&set, 5 * cap * 8) != hipSuccess) {
+  if (hipMalloc(&set, 7 * cap * 8) != hipSuccess) {
     (void)hipGetLastError();
     return fail(HIVE_ENOMEM, "window key set");
   }
@@ -448,7 +472,7 @@ hive_status_t ensure_window(hive_table* t, size_t ops) {
     (void)hipFree(set);
     return fail(HIVE_ENOMEM, "window buffers");
   }
-  if (hipMemset(set, 0, 5 * cap * 8) != hipSuccess || hipMemset(buf, 0, words * 4) != hipSuccess) {
+  if (hipMemset(set, 0, 7 * cap * 8) != hipSuccess || hipMemset(buf, 0, words * 4) != hipSuccess) {
     (void)hipFree(set);
     (void)hipFree(buf);
     return fail(HIVE_EDEVICE, "window buffers: clear");
@@ -461,8 +485,74 @@ hive_status_t ensure_window(hive_table* t, size_t ops) {
   t->win_set = set;
   t->win_buf = buf;
   t->win_cap = cap;
-  t->win_epoch = 0;  // every entry and round word is zero: older than any window
-  t->win_seq = 0;
+  t->win_epoch = 0;  // every tagged word is zero: older than any window
+  return HIVE_OK;
+}
+
+// One multi-batch window: ops [lo, lo + n), described by ntiles tiles from
+// desc (batch numbers base .. base + nbatches - 1), in window buffer p.  The
+// key-set passes run on ks, the table and chain passes on s; when they
+// differ, s waits for the key-set passes through `ready`, and the caller
+// orders ks after the buffer's last use.  ts/nts: completion stamps of the
+// window's batches (nts may be 0).
+hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                             uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
+                             uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
+                             hipStream_t ks, hipStream_t s, hipEvent_t ready) {
+  const size_t cap = t->win_cap;
+  unsigned long long* set = t->win_set;
+  const uint32_t mask = static_cast<uint32_t>(cap - 1);
+  const WindowBuffer wb = window_buffer(t, p);
+  ChainArgs ca = chain_args(t);
+  uint32_t coherent = 0;
+  if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry and chain word
+    k_fill_u64<<<fill_grid(cap), 256, 0, ks>>>(set, cap, 0ull);
+    HIP_CHECK_LAUNCH();
+    k_fill_u64<<<fill_grid(6 * cap), 256, 0, s>>>(ca.wbk, 6 * cap, 0ull);
+    HIP_CHECK_LAUNCH();
+    k_fill_u32<<<fill_grid((2 + kListed) * cap), 256, 0, s>>>(ca.head, (2 + kListed) * cap, 0u);
+    HIP_CHECK_LAUNCH();
+    t->win_epoch = 1;
+    coherent = 1;
+  }
+  k_win_mark<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
+                                           t->win_epoch, coherent, wb.eidx, wb.ctl);
+  HIP_CHECK_LAUNCH();
+  k_win_level<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
+                                            t->win_epoch, wb.eidx, wb.wtype, wb.pbid, wb.pe, wb.ctl);
+  HIP_CHECK_LAUNCH();
+  if (ks != s) {
+    HIP_TRY(hipEventRecord(ready, ks));
+    HIP_TRY(hipStreamWaitEvent(s, ready, 0));
+  }
+  hive_status_t r = launch_mixed_native(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, wb.wtype}, n,
+                                        OutNative{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr}, s);
+  if (r != HIVE_OK) return r;
+  ca.types = types;
+  ca.keys = keys;
+  ca.vals = vals;
+  ca.out_vals = out_vals;
+  ca.status = status;
+  ca.wtype = wb.wtype;
+  ca.pbid = wb.pbid;
+  ca.pe = wb.pe;
+  ca.wlo = static_cast<uint32_t>(lo);
+  ca.n = static_cast<uint32_t>(n);
+  ca.epoch = t->win_epoch;
+  ca.cleared = coherent;
+  ca.ctl = wb.ctl;
+  ca.ts = ts;
+  ca.nts = nts;
+  // grid-stride loops over the window's ops
+  const unsigned grid = std::max(1u, std::min(ntiles, static_cast<uint32_t>(std::max(ncu, 1)) * 8u));
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_note<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIP_CHECK_LAUNCH();
+  k_chain_resolve<<<grid, kWinBlock, 0, s>>>(ca);
+  HIP_CHECK_LAUNCH();
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_erase<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIP_CHECK_LAUNCH();
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_insert<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }
 
@@ -480,61 +570,16 @@ struct Pipeline {
 };
 }  // namespace
 
-// One multi-batch window: ops [lo, lo + n), described by ntiles tiles from
-// desc (batch numbers base .. base + nbatches - 1), in window buffer p.  The
-// key-set passes run on ks, the table passes on s; when they differ, s waits
-// for the key-set passes through `ready`, and the caller orders ks after the
-// buffer's last use.  ts/nts: completion stamps of the window's batches (nts
-// may be 0).
-hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
-                             uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
-                             uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
-                             unsigned long long wait_ticks, hipStream_t ks, hipStream_t s, hipEvent_t ready) {
-  const size_t cap = t->win_cap;
-  unsigned long long* set = t->win_set;
-  const uint32_t mask = static_cast<uint32_t>(cap - 1);
-  const WindowBuffer wb = window_buffer(t, p);
-  uint32_t coherent = 0;
-  if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry
-    k_fill_u64<<<fill_grid(cap), 256, 0, ks>>>(set, cap, 0ull);
-    HIP_CHECK_LAUNCH();
-    t->win_epoch = 1;
-    coherent = 1;
-  }
-  const unsigned long long seq = ++t->win_seq;
-  if (seq >= (1ull << 36)) return fail(HIVE_EINVAL, "windowed batches: window sequence exhausted");
-  k_win_mark<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                           t->win_epoch, coherent, wb.eidx, wb.ctl);
-  HIP_CHECK_LAUNCH();
-  k_win_level<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                            t->win_epoch, wb.eidx, wb.wtype, wb.rw, seq, wb.pidx, wb.pbid, wb.pe,
-                                            wb.ctl);
-  HIP_CHECK_LAUNCH();
-  if (ks != s) {
-    HIP_TRY(hipEventRecord(ready, ks));
-    HIP_TRY(hipStreamWaitEvent(s, ready, 0));
-  }
-  hive_status_t r = launch_mixed_native(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, wb.wtype}, n,
-                                        OutNative{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr}, s);
-  if (r != HIVE_OK) return r;
-  RoundArgs ra{types, keys, vals, out_vals, status, wb.pidx, wb.pbid, wb.pe, wb.rw, static_cast<uint32_t>(cap), seq,
-               wb.ctl, wait_ticks, ts, nts};
-  // a small grid, every block resident: the rounds are separated by grid barriers
-  const unsigned round_grid = static_cast<unsigned>(std::max(1, std::min(ncu, 64)));
-  HIVE_POLICY_SWITCH(t->policy, (k_win_rounds<P><<<round_grid, kRoundBlock, 0, s>>>(t->v, ra)));
-  HIP_CHECK_LAUNCH();
-  return HIVE_OK;
-}
-
 // Multi-batch windows alternate between two buffers.  The key-set passes
-// (k_win_mark, k_win_level) touch only the inputs and the window buffer, so
-// they run on a side stream one window ahead, overlapping the table work of
-// the previous window; a window's phase kernels wait for its key-set passes,
-// and a buffer's next key-set passes wait for its last window's rounds.
+// (k_win_mark, k_win_level) touch only the inputs, the key set and the window
+// buffer, so they run on a side stream one window ahead, overlapping the
+// table work of the previous window; a window's phase kernels wait for its
+// key-set passes, and a buffer's next key-set passes wait for its last
+// window's chain passes.
 hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                               const uint64_t* off, uint32_t* out_vals, uint8_t* status, unsigned int* ctrs,
                               unsigned long long* ts, const std::vector<WindowPlan>& plan, const uint4* d_desc,
-                              int ncu, unsigned long long wait_ticks, hipStream_t s) {
+                              int ncu, hipStream_t s) {
   Pipeline pl;
   HIP_TRY(hipStreamCreateWithFlags(&pl.ms, hipStreamNonBlocking));
   for (hipEvent_t* e : {&pl.start, &pl.done, &pl.ready[0], &pl.ready[1], &pl.free[0], &pl.free[1]})
@@ -562,7 +607,7 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
     if (used[p]) HIP_TRY(hipStreamWaitEvent(pl.ms, pl.free[p], 0));
     hive_status_t r = enqueue_window(t, types, keys, vals, lo, n, out_vals, status, d_desc + w.t0, ntiles,
                                      static_cast<uint32_t>(w.b0), ts + w.b0 + 1, static_cast<uint32_t>(w.b1 - w.b0),
-                                     p, ncu, wait_ticks, pl.ms, s, pl.ready[p]);
+                                     p, ncu, pl.ms, s, pl.ready[p]);
     if (r != HIVE_OK) return r;
     HIP_TRY(hipEventRecord(pl.free[p], s));
     used[p] = true;
