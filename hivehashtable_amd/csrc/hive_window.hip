@@ -18,7 +18,7 @@
 //                   the first batch of the window that writes the key (fw);
 //   k_win_level     ops of batches up to fw keep their type (before fw the
 //                   key is only looked up); later ones are deferred
-//                   (OP_DEFERRED) and appended to a pending list;
+//                   (OP_DEFERRED, marked in place);
 //   phase kernels   the window's lookups, deletes, inserts, as one mixed
 //                   batch: for each key, its lookups before fw, then fw's ops
 //                   in phase order;
@@ -44,6 +44,7 @@ namespace {
 
 constexpr uint32_t kWinBlock = 256;       // ops per tile of the mark/level kernels
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
+constexpr uint32_t kClaimer = 0x80000000u;  // eidx flag: this op claimed its key's entry (entries < 2^26)
 constexpr uint32_t kNoWrite = 0xFFFFu;    // no written batch
 constexpr uint32_t kListed = 4;           // written batches kept per chain for the chain lookups
 constexpr uint32_t kLine = 32;            // u32 words per control counter (own 128-byte line)
@@ -76,13 +77,12 @@ constexpr uint8_t kApplyDelete = 0xFC, kApplyInsert = 0xFD;
 // Block 0 also resets the control block.
 // Only writes (inserts, deletes) enter the set: a key's first written batch
 // is all that decides deferral, and lookups find their key's entry, if any,
-// by a read-only probe in k_win_level.  Slot reads are plain (a stale view
-// only makes a CAS fail and return the current entry) except right after the
-// set was cleared (coherent = 1).
+// by a read-only probe in k_win_level.  Slot reads are device-coherent, so
+// the ops of a hot key find its entry claimed and skip the CAS.  The one op
+// that claims a key's entry flags its eidx with kClaimer.
 __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint32_t base, const uint8_t* types,
                                                         const uint32_t* keys, uint32_t wlo, unsigned long long* set,
-                                                        uint32_t mask, uint32_t epoch, uint32_t coherent, uint32_t* eidx,
-                                                        WinCtl ctl) {
+                                                        uint32_t mask, uint32_t epoch, uint32_t* eidx, WinCtl ctl) {
   if (blockIdx.x == 0 && threadIdx.x < kCtlCounters)
     __hip_atomic_store(ctl.c + threadIdx.x * kLine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint4 d = desc[blockIdx.x];
@@ -95,11 +95,11 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint3
     const uint32_t fw = d.z - base, cap = mask + 1u;
     uint32_t h = win_hash(key) & mask;
     for (uint32_t probe = 0; probe < cap && e == kNoEntry; ++probe, h = (h + 1u) & mask) {
-      unsigned long long cur = coherent ? ld_sync(set + h) : set[h];
+      unsigned long long cur = ld_sync(set + h);
       for (;;) {
         if (went_epoch(cur) != epoch) {  // free in this window: claim it
           const unsigned long long seen = cas64(set + h, cur, went(epoch, fw, key));
-          if (seen == cur) { e = h; break; }
+          if (seen == cur) { e = h | kClaimer; break; }
           cur = seen;
           continue;
         }
@@ -116,15 +116,15 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint3
 
 // Ops in a batch after their key's first write in the window are deferred:
 // typed OP_DEFERRED in place, with their key entry and fw << 16 | batch at
-// their window position (no list, so no shared counter), and the window's
-// pending flag raised.  Ops up to and including that batch keep their type:
+// their window position (no list, so no shared counter); the key is marked
+// chained and the window's pending flag raised.  Ops up to and including that batch keep their type:
 // before it the key is only looked up, so they see the table as it was
 // before the window.
 __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint32_t base, const uint8_t* types,
                                                          const uint32_t* keys, uint32_t wlo,
                                                          const unsigned long long* set, uint32_t mask, uint32_t epoch,
                                                          const uint32_t* eidx, uint8_t* wtype, uint32_t* pbid,
-                                                         uint32_t* pe, WinCtl ctl) {
+                                                         uint32_t* pe, uint32_t* chained, WinCtl ctl) {
   const uint4 d = desc[blockIdx.x];
   const uint32_t i = d.x + threadIdx.x;
   const uint32_t bw = d.z - base;  // batch index in the window
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
     } else {
       e = eidx[i - wlo];
       if (e != kNoEntry) {
+        e &= ~kClaimer;
         fw = went_fw(set[e]);
         pend = bw > fw;
       }
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
     if (pend) {
       pbid[i - wlo] = (fw << 16) | bw;
       pe[i - wlo] = e;
+      chained[e] = epoch;  // every op of the chain stores the same word
     }
   }
   const unsigned long long m = __ballot(pend);
@@ -167,7 +169,6 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
 // ---------------------------------------------------------------- chains
 // Chain state, indexed by key entry e or by write-batch slot h.  Every word
 // is tagged with the window's epoch, so nothing is cleared between windows:
-//   head[e]    epoch of the last window whose chain head looked the key up
 //   init[e]    epoch << 33 | found << 32 | value: the key after fw
 //   lastw[e]   epoch << 32 | the chain's last written batch
 //   wc[e]      epoch << 16 | number of written batches in the chain
@@ -185,12 +186,13 @@ struct ChainArgs {
   uint8_t* wtype;        // the window's internal op types (OP_DEFERRED, kApply*)
   const uint32_t* pbid;  // per window position: fw << 16 | batch of a deferred op
   const uint32_t* pe;    // per window position: key entry of a deferred op
+  const uint32_t* eidx;  // per window position: key entry of a write, kClaimer-flagged for the entry's claimer
+  const uint32_t* chained;  // per key entry: the window's epoch if the key has deferred ops
   uint32_t wlo, n;       // the window's first op and op count
   unsigned long long *wbk, *insmax, *insmin, *delmin, *init, *lastw;
-  uint32_t *head, *wc, *wl;
+  uint32_t *wc, *wl;
   uint32_t mask;   // write-batch slots - 1
   uint32_t epoch;
-  uint32_t cleared;  // 1 right after the chain words were cleared: a cached word may be older than the clear
   WinCtl ctl;
   unsigned long long* ts;  // completion stamps of the window's batches
   uint32_t nts;
@@ -217,35 +219,34 @@ __device__ __forceinline__ uint32_t op_value(const ChainArgs& a, uint32_t i) {
   return a.vals ? a.vals[i] : a.keys[i] + 1u;
 }
 
-// Chain pass 1 (reads the table, writes none of it): the first op of every
-// chain looks its key up (the state after fw); every deferred write notes
-// its (key, batch) pair, the pair's first/last insert and first delete, and
-// the chain's written batches.
+// Chain pass 1 (reads the table, writes none of it): the claimer of every
+// chained key's entry looks the key up (its state after fw); every deferred
+// write notes its (key, batch) pair and the pair's first/last insert and
+// first delete; the op that notes a new pair adds the batch to its chain.
+// A hot key puts thousands of ops on its words: they are read coherently
+// first and written only by an atomic that can still change them.
 template <class P>
 __global__ __launch_bounds__(kWinBlock) void k_chain_note(TableView t, ChainArgs a) {
   if (a.ctl.pending()[0] == 0u) return;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
-    if (a.wtype[j] != OP_DEFERRED) continue;
-    const uint32_t i = a.wlo + j, b = a.pbid[j] & 0xFFFFu, e = a.pe[j];
-    const int ty = a.types[i];
-    // Hot keys put thousands of ops on one chain: every shared word is read
-    // first and only written by an atomic that can still change it (words only
-    // grow between clears; after a clear every op takes the atomic).
-    // The one op that raises head[e] to the epoch is the chain's head.
-    const bool c = a.cleared != 0u;
-    if ((c || a.head[e] != a.epoch) && atomicMax(a.head + e, a.epoch) != a.epoch) {
+    const uint32_t ce = a.eidx[j];
+    if (ce != kNoEntry && (ce & kClaimer) != 0u && a.chained[ce & ~kClaimer] == a.epoch) {
       uint32_t v;
-      const bool found = lookup_one<P>(t, a.keys[i], v) == R_FOUND;
-      a.init[e] = (static_cast<unsigned long long>(a.epoch) << 33) | (static_cast<unsigned long long>(found) << 32) | v;
+      const bool found = lookup_one<P>(t, a.keys[a.wlo + j], v) == R_FOUND;
+      a.init[ce & ~kClaimer] =
+          (static_cast<unsigned long long>(a.epoch) << 33) | (static_cast<unsigned long long>(found) << 32) | v;
     }
+    if (a.wtype[j] != OP_DEFERRED) continue;
+    const uint32_t i = a.wlo + j;
+    const int ty = a.types[i];
     if (ty != OP_INSERT && ty != OP_DELETE) continue;
+    const uint32_t b = a.pbid[j] & 0xFFFFu, e = a.pe[j];
     const unsigned long long tag = static_cast<unsigned long long>(a.epoch) << 32;
-    if (c || a.lastw[e] < (tag | b)) atomicMax(a.lastw + e, tag | b);
     const unsigned long long want = wb_word(a.epoch, e, b);
     uint32_t h = wb_hash(e, b) & a.mask;
     bool claimed = false;
     for (uint32_t probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
-      unsigned long long cur = a.wbk[h];
+      unsigned long long cur = ld_sync(a.wbk + h);
       if (cur != want && static_cast<uint32_t>(cur >> 48) != a.epoch) {
         const unsigned long long seen = cas64(a.wbk + h, cur, want);
         if (seen == cur) { claimed = true; break; }
@@ -253,8 +254,9 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_note(TableView t, ChainArgs
       }
       if (cur == want) break;
     }
-    if (claimed) {  // a new written batch of the chain: count it, keep the first kListed
-      uint32_t cur = a.wc[e];
+    if (claimed) {  // a new written batch of the chain: the latest so far?, count it, keep the first kListed
+      atomicMax(a.lastw + e, tag | b);
+      uint32_t cur = ld_sync32(a.wc + e);
       for (;;) {
         const uint32_t n = (cur >> 16) != a.epoch ? 0u : (cur & 0xFFFFu);
         const uint32_t seen = atomicCAS(a.wc + e, cur, (a.epoch << 16) | (n + 1u));
@@ -266,9 +268,9 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_note(TableView t, ChainArgs
       }
     }
     if (ty == OP_INSERT) {
-      if (c || a.insmax[h] < (tag | (i + 1u))) atomicMax(a.insmax + h, tag | (i + 1u));
-      if (c || a.insmin[h] < (tag | (0xFFFFFFFFu - i))) atomicMax(a.insmin + h, tag | (0xFFFFFFFFu - i));
-    } else if (c || a.delmin[h] < (tag | (0xFFFFFFFFu - i))) {
+      if (ld_sync(a.insmax + h) < (tag | (i + 1u))) atomicMax(a.insmax + h, tag | (i + 1u));
+      if (ld_sync(a.insmin + h) < (tag | (0xFFFFFFFFu - i))) atomicMax(a.insmin + h, tag | (0xFFFFFFFFu - i));
+    } else if (ld_sync(a.delmin + h) < (tag | (0xFFFFFFFFu - i))) {
       atomicMax(a.delmin + h, tag | (0xFFFFFFFFu - i));
     }
   }
@@ -405,17 +407,20 @@ namespace {
 // win_set (u64, 7 cap): the key set, then the chain words wbk, insmax,
 // insmin, delmin, init, lastw [cap each].
 // win_buf (u32): per window buffer eidx, pbid, pe [cap / 2 each, by window
-// position], wtype [cap / 2 bytes] and its control block on a 128-byte line;
-// then the chain words head, wc [cap each], wl [kListed cap].
+// position], chained [cap, by key entry], wtype [cap / 2 bytes] and its
+// control block on a 128-byte line; then the chain words wc [cap], wl
+// [kListed cap].  chained is never cleared: every chained key's word holds
+// the window's epoch, and an old word that happens to hold it only costs a
+// lookup nobody reads.
 size_t ctl_offset(size_t cap) {
   const size_t half = cap / 2;
-  return (3 * half + (half + 3) / 4 + kLine - 1) / kLine * kLine;
+  return (3 * half + cap + (half + 3) / 4 + kLine - 1) / kLine * kLine;
 }
 size_t buffer_words(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
-size_t chain_words(size_t cap) { return (2 + kListed) * cap; }
+size_t chain_words(size_t cap) { return (1 + kListed) * cap; }
 
 struct WindowBuffer {
-  uint32_t *eidx, *pbid, *pe;
+  uint32_t *eidx, *pbid, *pe, *chained;
   uint8_t* wtype;
   WinCtl ctl;
 };
@@ -427,7 +432,8 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
   w.eidx = base;
   w.pbid = base + half;
   w.pe = base + 2 * half;
-  w.wtype = reinterpret_cast<uint8_t*>(base + 3 * half);
+  w.chained = base + 3 * half;
+  w.wtype = reinterpret_cast<uint8_t*>(w.chained + cap);
   w.ctl = WinCtl{base + ctl_offset(cap)};
   return w;
 }
@@ -445,9 +451,8 @@ ChainArgs chain_args(const hive_table* t) {
   a.init = q + 4 * cap;
   a.lastw = q + 5 * cap;
   uint32_t* c = t->win_buf + 2 * buffer_words(cap);
-  a.head = c;
-  a.wc = c + cap;
-  a.wl = c + 2 * cap;
+  a.wc = c;
+  a.wl = c + cap;
   a.mask = static_cast<uint32_t>(cap - 1);
   return a;
 }
@@ -504,22 +509,20 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   const uint32_t mask = static_cast<uint32_t>(cap - 1);
   const WindowBuffer wb = window_buffer(t, p);
   ChainArgs ca = chain_args(t);
-  uint32_t coherent = 0;
   if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry and chain word
     k_fill_u64<<<fill_grid(cap), 256, 0, ks>>>(set, cap, 0ull);
     HIP_CHECK_LAUNCH();
     k_fill_u64<<<fill_grid(6 * cap), 256, 0, s>>>(ca.wbk, 6 * cap, 0ull);
     HIP_CHECK_LAUNCH();
-    k_fill_u32<<<fill_grid((2 + kListed) * cap), 256, 0, s>>>(ca.head, (2 + kListed) * cap, 0u);
+    k_fill_u32<<<fill_grid((1 + kListed) * cap), 256, 0, s>>>(ca.wc, (1 + kListed) * cap, 0u);
     HIP_CHECK_LAUNCH();
     t->win_epoch = 1;
-    coherent = 1;
   }
   k_win_mark<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                           t->win_epoch, coherent, wb.eidx, wb.ctl);
+                                           t->win_epoch, wb.eidx, wb.ctl);
   HIP_CHECK_LAUNCH();
   k_win_level<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
-                                            t->win_epoch, wb.eidx, wb.wtype, wb.pbid, wb.pe, wb.ctl);
+                                            t->win_epoch, wb.eidx, wb.wtype, wb.pbid, wb.pe, wb.chained, wb.ctl);
   HIP_CHECK_LAUNCH();
   if (ks != s) {
     HIP_TRY(hipEventRecord(ready, ks));
@@ -536,10 +539,11 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   ca.wtype = wb.wtype;
   ca.pbid = wb.pbid;
   ca.pe = wb.pe;
+  ca.eidx = wb.eidx;
+  ca.chained = wb.chained;
   ca.wlo = static_cast<uint32_t>(lo);
   ca.n = static_cast<uint32_t>(n);
   ca.epoch = t->win_epoch;
-  ca.cleared = coherent;
   ca.ctl = wb.ctl;
   ca.ts = ts;
   ca.nts = nts;
