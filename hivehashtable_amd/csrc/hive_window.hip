@@ -77,21 +77,36 @@ constexpr uint8_t kApplyDelete = 0xFC, kApplyInsert = 0xFD;
 // Block 0 also resets the control block.
 // Only writes (inserts, deletes) enter the set: a key's first written batch
 // is all that decides deferral, and lookups find their key's entry, if any,
-// by a read-only probe in k_win_level.  Slot reads are device-coherent, so
-// the ops of a hot key find its entry claimed and skip the CAS.  The one op
-// that claims a key's entry flags its eidx with kClaimer.
+// by a read-only probe in k_win_level.  The writes of one tile (one batch)
+// meet in LDS first, so a hot key costs the set one probe per tile; slot
+// reads are device-coherent, so the probe finds a claimed entry without a
+// CAS.  The one op that claims a key's entry flags its eidx with kClaimer.
+constexpr uint32_t kMarkSlots = 2 * kWinBlock;
 __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint32_t base, const uint8_t* types,
                                                         const uint32_t* keys, uint32_t wlo, unsigned long long* set,
                                                         uint32_t mask, uint32_t epoch, uint32_t* eidx, WinCtl ctl) {
+  __shared__ uint32_t s_key[kMarkSlots], s_e[kMarkSlots];  // tile's written keys (0 = free), their entries
   if (blockIdx.x == 0 && threadIdx.x < kCtlCounters)
     __hip_atomic_store(ctl.c + threadIdx.x * kLine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t q = threadIdx.x; q < kMarkSlots; q += kWinBlock) s_key[q] = 0u;
+  __syncthreads();
   const uint4 d = desc[blockIdx.x];
   const uint32_t i = d.x + threadIdx.x;
-  if (i >= d.y) return;
-  const int ty = types[i];
-  const uint32_t key = keys[i];
+  const int ty = i < d.y ? types[i] : HIVE_OP_NONE;
+  const uint32_t key = i < d.y ? keys[i] : 0u;
+  const bool write = key != 0u && (ty == OP_INSERT || ty == OP_DELETE);
+  // the first write of each key in the tile probes the set for all of them
+  uint32_t q = win_hash(key) & (kMarkSlots - 1u);
+  bool lead = false;
+  if (write)
+    for (;;) {  // at most kWinBlock keys in kMarkSlots slots: a free slot always exists
+      const uint32_t cur = atomicCAS(s_key + q, 0u, key);
+      if (cur == 0u) { lead = true; break; }
+      if (cur == key) break;
+      q = (q + 1u) & (kMarkSlots - 1u);
+    }
   uint32_t e = kNoEntry;
-  if (key != 0u && (ty == OP_INSERT || ty == OP_DELETE)) {
+  if (lead) {
     const uint32_t fw = d.z - base, cap = mask + 1u;
     uint32_t h = win_hash(key) & mask;
     for (uint32_t probe = 0; probe < cap && e == kNoEntry; ++probe, h = (h + 1u) & mask) {
@@ -110,8 +125,11 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint3
         cur = seen;
       }
     }
+    s_e[q] = e == kNoEntry ? kNoEntry : (e & ~kClaimer);
   }
-  eidx[i - wlo] = e;
+  __syncthreads();
+  if (write && !lead) e = s_e[q];
+  if (i < d.y) eidx[i - wlo] = e;
 }
 
 // Ops in a batch after their key's first write in the window are deferred:
@@ -167,16 +185,26 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
 }
 
 // ---------------------------------------------------------------- chains
-// Chain state, indexed by key entry e or by write-batch slot h.  Every word
-// is tagged with the window's epoch, so nothing is cleared between windows:
-//   init[e]    epoch << 33 | found << 32 | value: the key after fw
-//   lastw[e]   epoch << 32 | the chain's last written batch
-//   wc[e]      epoch << 16 | number of written batches in the chain
-//   wl[4e..]   epoch << 16 | batch, the first kListed of them
-//   wbk[h]     epoch << 48 | e << 12 | batch: a (key, written batch) pair
-//   insmax[h]  epoch << 32 | (op + 1) of the pair's last insert
-//   insmin[h]  epoch << 32 | ~op of the pair's first insert
-//   delmin[h]  epoch << 32 | ~op of the pair's first delete
+// Chain state.  Every word is tagged with the window's epoch, so nothing is
+// cleared between windows.  A record is 32 bytes, so one access brings all of
+// its words.
+// A (key entry, written batch) pair of the window, in a write-batch hash
+// table (linear probing, slot h):
+struct PairRec {
+  unsigned long long key;     // epoch << 48 | e << 12 | batch
+  unsigned long long insmax;  // epoch << 32 | (op + 1) of the pair's last insert
+  unsigned long long insmin;  // epoch << 32 | ~op of the pair's first insert
+  unsigned long long delmin;  // epoch << 32 | ~op of the pair's first delete
+};
+// A key's chain, by key entry e:
+struct KeyRec {
+  unsigned long long init;   // epoch << 33 | found << 32 | value: the key after fw
+  unsigned long long chain;  // epoch << 48 | written batches << 16 | the last of them
+  uint32_t wl[kListed];      // epoch << 16 | batch: the first kListed written batches
+};
+
+static_assert(sizeof(PairRec) == 32 && sizeof(KeyRec) == 32, "chain records are 32 bytes");
+
 struct ChainArgs {
   const uint8_t* types;
   const uint32_t* keys;
@@ -189,8 +217,8 @@ struct ChainArgs {
   const uint32_t* eidx;  // per window position: key entry of a write, kClaimer-flagged for the entry's claimer
   const uint32_t* chained;  // per key entry: the window's epoch if the key has deferred ops
   uint32_t wlo, n;       // the window's first op and op count
-  unsigned long long *wbk, *insmax, *insmin, *delmin, *init, *lastw;
-  uint32_t *wc, *wl;
+  PairRec* pr;
+  KeyRec* kr;
   uint32_t mask;   // write-batch slots - 1
   uint32_t epoch;
   WinCtl ctl;
@@ -209,7 +237,7 @@ __device__ __forceinline__ unsigned long long wb_word(uint32_t epoch, uint32_t e
 __device__ __forceinline__ uint32_t wb_find(const ChainArgs& a, uint32_t e, uint32_t b) {
   const unsigned long long want = wb_word(a.epoch, e, b);
   for (uint32_t h = wb_hash(e, b) & a.mask, probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
-    const unsigned long long cur = a.wbk[h];
+    const unsigned long long cur = a.pr[h].key;
     if (cur == want) return h;
     if (static_cast<uint32_t>(cur >> 48) != a.epoch) return kNoEntry;
   }
@@ -219,60 +247,120 @@ __device__ __forceinline__ uint32_t op_value(const ChainArgs& a, uint32_t i) {
   return a.vals ? a.vals[i] : a.keys[i] + 1u;
 }
 
+// Note one (key entry e, batch b) pair with its last insert (op + 1, or 0),
+// first insert and first delete (op, or ~0u).  The op that adds the pair to
+// the write-batch table adds b to the key's chain.  A hot pair is noted by
+// many blocks: its words are read coherently first and written only by an
+// atomic that can still change them.
+__device__ void note_pair(const ChainArgs& a, uint32_t e, uint32_t b, uint32_t imax1, uint32_t imin, uint32_t dmin) {
+  const unsigned long long want = wb_word(a.epoch, e, b);
+  uint32_t h = wb_hash(e, b) & a.mask;
+  bool claimed = false;
+  for (uint32_t probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
+    unsigned long long cur = ld_sync(&a.pr[h].key);
+    if (cur != want && static_cast<uint32_t>(cur >> 48) != a.epoch) {
+      const unsigned long long seen = cas64(&a.pr[h].key, cur, want);
+      if (seen == cur) { claimed = true; break; }
+      cur = seen;
+    }
+    if (cur == want) break;
+  }
+  if (claimed) {  // a new written batch of the chain: count it, keep the latest and the first kListed
+    KeyRec& k = a.kr[e];
+    unsigned long long cur = ld_sync(&k.chain);
+    for (;;) {
+      const bool fresh = static_cast<uint32_t>(cur >> 48) == a.epoch;
+      const uint32_t n = fresh ? static_cast<uint32_t>(cur >> 16) & 0xFFFFu : 0u;
+      const uint32_t last = fresh ? std::max(static_cast<uint32_t>(cur) & 0xFFFFu, b) : b;
+      const unsigned long long next = (static_cast<unsigned long long>(a.epoch) << 48) |
+                                      (static_cast<unsigned long long>(n + 1u) << 16) | last;
+      const unsigned long long seen = cas64(&k.chain, cur, next);
+      if (seen == cur) {
+        if (n < kListed) k.wl[n] = (a.epoch << 16) | b;
+        break;
+      }
+      cur = seen;
+    }
+  }
+  // the claimer knows the pair's words are older than the window; anyone
+  // else reads them first
+  PairRec& r = a.pr[h];
+  const unsigned long long tag = static_cast<unsigned long long>(a.epoch) << 32;
+  if (imax1 != 0u && (claimed || ld_sync(&r.insmax) < (tag | imax1))) atomicMax(&r.insmax, tag | imax1);
+  if (imin != ~0u && (claimed || ld_sync(&r.insmin) < (tag | ~imin))) atomicMax(&r.insmin, tag | ~imin);
+  if (dmin != ~0u && (claimed || ld_sync(&r.delmin) < (tag | ~dmin))) atomicMax(&r.delmin, tag | ~dmin);
+}
+
+// k_chain_note aggregates the deferred writes of a block's kWinBlock
+// consecutive window positions per (key, batch) pair in LDS; the pair's first
+// op there notes the aggregate, so a hot pair costs one note per block
+// instead of one per op, and notes of different pairs still run in parallel.
+constexpr uint32_t kNoteSlots = 2 * kWinBlock;  // at most kWinBlock pairs: a free slot always exists
+
 // Chain pass 1 (reads the table, writes none of it): the claimer of every
 // chained key's entry looks the key up (its state after fw); every deferred
-// write notes its (key, batch) pair and the pair's first/last insert and
-// first delete; the op that notes a new pair adds the batch to its chain.
-// A hot key puts thousands of ops on its words: they are read coherently
-// first and written only by an atomic that can still change them.
+// write's (key, batch) pair is noted with the pair's first/last insert and
+// first delete.
 template <class P>
 __global__ __launch_bounds__(kWinBlock) void k_chain_note(TableView t, ChainArgs a) {
-  if (a.ctl.pending()[0] == 0u) return;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
-    const uint32_t ce = a.eidx[j];
-    if (ce != kNoEntry && (ce & kClaimer) != 0u && a.chained[ce & ~kClaimer] == a.epoch) {
-      uint32_t v;
-      const bool found = lookup_one<P>(t, a.keys[a.wlo + j], v) == R_FOUND;
-      a.init[ce & ~kClaimer] =
-          (static_cast<unsigned long long>(a.epoch) << 33) | (static_cast<unsigned long long>(found) << 32) | v;
-    }
-    if (a.wtype[j] != OP_DEFERRED) continue;
-    const uint32_t i = a.wlo + j;
-    const int ty = a.types[i];
-    if (ty != OP_INSERT && ty != OP_DELETE) continue;
-    const uint32_t b = a.pbid[j] & 0xFFFFu, e = a.pe[j];
-    const unsigned long long tag = static_cast<unsigned long long>(a.epoch) << 32;
-    const unsigned long long want = wb_word(a.epoch, e, b);
-    uint32_t h = wb_hash(e, b) & a.mask;
-    bool claimed = false;
-    for (uint32_t probe = 0; probe <= a.mask; ++probe, h = (h + 1u) & a.mask) {
-      unsigned long long cur = ld_sync(a.wbk + h);
-      if (cur != want && static_cast<uint32_t>(cur >> 48) != a.epoch) {
-        const unsigned long long seen = cas64(a.wbk + h, cur, want);
-        if (seen == cur) { claimed = true; break; }
-        cur = seen;
+  __shared__ unsigned long long s_pair[kNoteSlots];  // (e << 12 | batch) + 1, 0 = free
+  __shared__ uint32_t s_imax[kNoteSlots], s_imin[kNoteSlots], s_dmin[kNoteSlots];
+  if (a.ctl.pending()[0] == 0u) return;  // the same for every block
+  for (uint32_t q = threadIdx.x; q < kNoteSlots; q += kWinBlock) {
+    s_pair[q] = 0ull;
+    s_imax[q] = 0u;
+    s_imin[q] = ~0u;
+    s_dmin[q] = ~0u;
+  }
+  __syncthreads();
+  // whole blocks step together (the LDS barriers): every thread runs every step
+  for (uint32_t j0 = blockIdx.x * kWinBlock; j0 < a.n; j0 += gridDim.x * kWinBlock) {
+    const uint32_t j = j0 + threadIdx.x;
+    bool lead = false, any = false;
+    uint32_t q = 0, e = 0, b = 0;
+    if (j < a.n) {
+      const uint32_t ce = a.eidx[j];
+      if (ce != kNoEntry && (ce & kClaimer) != 0u && a.chained[ce & ~kClaimer] == a.epoch) {
+        uint32_t v;
+        const bool found = lookup_one<P>(t, a.keys[a.wlo + j], v) == R_FOUND;
+        a.kr[ce & ~kClaimer].init =
+            (static_cast<unsigned long long>(a.epoch) << 33) | (static_cast<unsigned long long>(found) << 32) | v;
       }
-      if (cur == want) break;
-    }
-    if (claimed) {  // a new written batch of the chain: the latest so far?, count it, keep the first kListed
-      atomicMax(a.lastw + e, tag | b);
-      uint32_t cur = ld_sync32(a.wc + e);
-      for (;;) {
-        const uint32_t n = (cur >> 16) != a.epoch ? 0u : (cur & 0xFFFFu);
-        const uint32_t seen = atomicCAS(a.wc + e, cur, (a.epoch << 16) | (n + 1u));
-        if (seen == cur) {
-          if (n < kListed) a.wl[static_cast<size_t>(e) * kListed + n] = (a.epoch << 16) | b;
-          break;
+      const uint32_t i = a.wlo + j;
+      const int ty = a.wtype[j] == OP_DEFERRED ? a.types[i] : HIVE_OP_NONE;
+      if (ty == OP_INSERT || ty == OP_DELETE) {
+        e = a.pe[j];
+        b = a.pbid[j] & 0xFFFu;
+        const unsigned long long pk = ((static_cast<unsigned long long>(e) << 12) | b) + 1ull;
+        q = wb_hash(e, b) & (kNoteSlots - 1u);
+        for (;;) {
+          const unsigned long long cur = atomicCAS(s_pair + q, 0ull, pk);
+          if (cur == 0ull) { lead = true; break; }
+          if (cur == pk) break;
+          q = (q + 1u) & (kNoteSlots - 1u);
         }
-        cur = seen;
+        if (ty == OP_INSERT) {
+          atomicMax(s_imax + q, i + 1u);
+          atomicMin(s_imin + q, i);
+        } else {
+          atomicMin(s_dmin + q, i);
+        }
+        any = true;
       }
     }
-    if (ty == OP_INSERT) {
-      if (ld_sync(a.insmax + h) < (tag | (i + 1u))) atomicMax(a.insmax + h, tag | (i + 1u));
-      if (ld_sync(a.insmin + h) < (tag | (0xFFFFFFFFu - i))) atomicMax(a.insmin + h, tag | (0xFFFFFFFFu - i));
-    } else if (ld_sync(a.delmin + h) < (tag | (0xFFFFFFFFu - i))) {
-      atomicMax(a.delmin + h, tag | (0xFFFFFFFFu - i));
+    if (!__syncthreads_or(any)) continue;
+    uint32_t imax1 = 0u, imin = ~0u, dmin = ~0u;
+    if (lead) {  // take the aggregate and free the slot for the next step
+      imax1 = s_imax[q];
+      imin = s_imin[q];
+      dmin = s_dmin[q];
+      s_pair[q] = 0ull;
+      s_imax[q] = 0u;
+      s_imin[q] = ~0u;
+      s_dmin[q] = ~0u;
     }
+    __syncthreads();
+    if (lead) note_pair(a, e, b, imax1, imin, dmin);
   }
 }
 
@@ -290,25 +378,25 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_resolve(ChainArgs a) {
     if (a.wtype[j] != OP_DEFERRED) continue;
     const uint32_t i = a.wlo + j, pb = a.pbid[j], b = pb & 0xFFFFu, fw = pb >> 16, e = a.pe[j];
     const int ty = a.types[i];
-    const unsigned long long ini = a.init[e];
-    bool present = ((ini >> 32) & 1ull) != 0ull;
-    uint32_t val = static_cast<uint32_t>(ini);
+    const KeyRec kr = a.kr[e];
+    bool present = ((kr.init >> 32) & 1ull) != 0ull;
+    uint32_t val = static_cast<uint32_t>(kr.init);
     // the chain's latest written batch before b
-    const uint32_t wcv = a.wc[e];
-    const uint32_t nw = (wcv >> 16) == a.epoch ? (wcv & 0xFFFFu) : 0u;
+    const bool fresh = static_cast<uint32_t>(kr.chain >> 48) == a.epoch;
+    const uint32_t nw = fresh ? static_cast<uint32_t>(kr.chain >> 16) & 0xFFFFu : 0u;
     uint32_t hw = kNoEntry;
     if (nw <= kListed) {
       uint32_t w = kNoWrite;
-      for (uint32_t s = 0; s < nw; ++s) {
-        const uint32_t bb = a.wl[static_cast<size_t>(e) * kListed + s] & 0xFFFFu;
-        if (bb < b && (w == kNoWrite || bb > w)) w = bb;
+      for (uint32_t s = 0; s < kListed; ++s) {
+        const uint32_t bb = kr.wl[s] & 0xFFFFu;
+        if (s < nw && bb < b && (w == kNoWrite || bb > w)) w = bb;
       }
       if (w != kNoWrite) hw = wb_find(a, e, w);
     } else {  // a chain with many written batches: they are dense, walk back
       for (uint32_t bb = b; bb > fw + 1u && hw == kNoEntry;) hw = wb_find(a, e, --bb);
     }
     if (hw != kNoEntry) {
-      const unsigned long long im = a.insmax[hw];
+      const unsigned long long im = a.pr[hw].insmax;
       present = tag32(im) == a.epoch;
       val = present ? op_value(a, static_cast<uint32_t>(im) - 1u) : 0u;
     }
@@ -320,15 +408,16 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_resolve(ChainArgs a) {
     } else if (const uint32_t hb = wb_find(a, e, b); hb == kNoEntry) {
       res = R_FAILED;  // unreachable: pass 1 noted every deferred write's pair
     } else {
-      const unsigned long long dm = a.delmin[hb], im = a.insmax[hb];
+      const PairRec pr = a.pr[hb];
+      const unsigned long long dm = pr.delmin, im = pr.insmax;
       const bool has_del = tag32(dm) == a.epoch, has_ins = tag32(im) == a.epoch;
-      const bool last_batch = static_cast<uint32_t>(a.lastw[e]) == b;
+      const bool last_batch = fresh && (static_cast<uint32_t>(kr.chain) & 0xFFFFu) == b;
       if (ty == OP_DELETE) {
         const bool first = 0xFFFFFFFFu - static_cast<uint32_t>(dm) == i;
         res = first && present ? R_ERASED : R_NOT_FOUND;
         if (last_batch && !has_ins && first) a.wtype[j] = kApplyDelete;
       } else {
-        const bool first = 0xFFFFFFFFu - static_cast<uint32_t>(a.insmin[hb]) == i;
+        const bool first = 0xFFFFFFFFu - static_cast<uint32_t>(pr.insmin) == i;
         res = first && !(present && !has_del) ? R_INSERTED : R_UPDATED;
         if (last_batch && static_cast<uint32_t>(im) - 1u == i) a.wtype[j] = kApplyInsert;
       }
@@ -404,20 +493,19 @@ __global__ void k_stamp_range(unsigned long long* ts, uint32_t n) {
 namespace hive {
 
 namespace {
-// win_set (u64, 7 cap): the key set, then the chain words wbk, insmax,
-// insmin, delmin, init, lastw [cap each].
+// win_set (u64, 9 cap): the key set [cap], then the chain records: pairs
+// [cap PairRec], keys [cap KeyRec].
 // win_buf (u32): per window buffer eidx, pbid, pe [cap / 2 each, by window
 // position], chained [cap, by key entry], wtype [cap / 2 bytes] and its
-// control block on a 128-byte line; then the chain words wc [cap], wl
-// [kListed cap].  chained is never cleared: every chained key's word holds
-// the window's epoch, and an old word that happens to hold it only costs a
-// lookup nobody reads.
+// control block on a 128-byte line.  chained is never cleared: every chained
+// key's word holds the window's epoch, and an old word that happens to hold
+// it only costs a lookup nobody reads.
 size_t ctl_offset(size_t cap) {
   const size_t half = cap / 2;
   return (3 * half + cap + (half + 3) / 4 + kLine - 1) / kLine * kLine;
 }
 size_t buffer_words(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
-size_t chain_words(size_t cap) { return (1 + kListed) * cap; }
+constexpr size_t kSetWords = 9;  // u64 words of win_set per key-set slot
 
 struct WindowBuffer {
   uint32_t *eidx, *pbid, *pe, *chained;
@@ -443,16 +531,8 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
 ChainArgs chain_args(const hive_table* t) {
   const size_t cap = t->win_cap;
   ChainArgs a{};
-  unsigned long long* q = t->win_set + cap;
-  a.wbk = q;
-  a.insmax = q + cap;
-  a.insmin = q + 2 * cap;
-  a.delmin = q + 3 * cap;
-  a.init = q + 4 * cap;
-  a.lastw = q + 5 * cap;
-  uint32_t* c = t->win_buf + 2 * buffer_words(cap);
-  a.wc = c;
-  a.wl = c + cap;
+  a.pr = reinterpret_cast<PairRec*>(t->win_set + cap);
+  a.kr = reinterpret_cast<KeyRec*>(t->win_set + 5 * cap);
   a.mask = static_cast<uint32_t>(cap - 1);
   return a;
 }
@@ -465,10 +545,10 @@ hive_status_t ensure_window(hive_table* t, size_t ops) {
   while (cap < 2 * ops) cap <<= 1;
   if (t->win_set && cap <= t->win_cap) return HIVE_OK;
   if (cap > (size_t(1) << 26)) return fail(HIVE_EINVAL, "window too large");
-  const size_t words = 2 * buffer_words(cap) + chain_words(cap);
+  const size_t words = 2 * buffer_words(cap);
   unsigned long long* set = nullptr;
   uint32_t* buf = nullptr;
-  if (hipMalloc(&set, 7 * cap * 8) != hipSuccess) {
+  if (hipMalloc(&set, kSetWords * cap * 8) != hipSuccess) {
     (void)hipGetLastError();
     return fail(HIVE_ENOMEM, "window key set");
   }
@@ -477,7 +557,7 @@ hive_status_t ensure_window(hive_table* t, size_t ops) {
     (void)hipFree(set);
     return fail(HIVE_ENOMEM, "window buffers");
   }
-  if (hipMemset(set, 0, 7 * cap * 8) != hipSuccess || hipMemset(buf, 0, words * 4) != hipSuccess) {
+  if (hipMemset(set, 0, kSetWords * cap * 8) != hipSuccess || hipMemset(buf, 0, words * 4) != hipSuccess) {
     (void)hipFree(set);
     (void)hipFree(buf);
     return fail(HIVE_EDEVICE, "window buffers: clear");
@@ -512,9 +592,7 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   if (++t->win_epoch > 0xFFFFu) {  // epochs wrapped: free every entry and chain word
     k_fill_u64<<<fill_grid(cap), 256, 0, ks>>>(set, cap, 0ull);
     HIP_CHECK_LAUNCH();
-    k_fill_u64<<<fill_grid(6 * cap), 256, 0, s>>>(ca.wbk, 6 * cap, 0ull);
-    HIP_CHECK_LAUNCH();
-    k_fill_u32<<<fill_grid((1 + kListed) * cap), 256, 0, s>>>(ca.wc, (1 + kListed) * cap, 0u);
+    k_fill_u64<<<fill_grid((kSetWords - 1) * cap), 256, 0, s>>>(set + cap, (kSetWords - 1) * cap, 0ull);
     HIP_CHECK_LAUNCH();
     t->win_epoch = 1;
   }
