@@ -31,8 +31,8 @@ constexpr int kCounterStride = 16;  // u64 words between counter lanes (128 B)
 constexpr uint8_t R_FAILED = 0, R_INSERTED = 1, R_UPDATED = 2, R_STASHED = 3, R_FOUND = 4,
                   R_NOT_FOUND = 5, R_ERASED = 6, R_INVALID_KEY = 7, R_NOOP = 8, R_PENDING = 0xFF;
 constexpr int OP_INSERT = 0, OP_LOOKUP = 1, OP_DELETE = 2;
-// internal op type of a windowed stream: an op held back for a later round;
-// every phase kernel skips it and writes nothing for it
+// internal op type of a windowed stream: an op resolved by the window's chain
+// kernels; every phase kernel skips it and writes nothing for it
 constexpr int OP_DEFERRED = 0xFE;
 // per-bucket hint bits in TableView::meta: kHintBits per bucket, of which the
 // low kHintBits - 1 are overflow fingerprint classes and the top one is the

@@ -21,16 +21,25 @@ using namespace hive;
 namespace {
 
 // inserts of each request: the admission test needs an upper bound of new keys
-__global__ void k_req_inserts(const uint8_t* types, const unsigned long long* off, uint32_t* counts) {
-  __shared__ uint32_t total;
-  if (threadIdx.x == 0) total = 0;
+// per request: its inserts, and its inserts plus deletes (writes)
+__global__ void k_req_inserts(const uint8_t* types, const unsigned long long* off, uint32_t* counts,
+                              uint32_t* writes) {
+  __shared__ uint32_t total, wtotal;
+  if (threadIdx.x == 0) total = wtotal = 0;
   __syncthreads();
   const unsigned long long lo = off[blockIdx.x], hi = off[blockIdx.x + 1];
-  uint32_t c = 0;
-  for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) c += types[i] == OP_INSERT;
+  uint32_t c = 0, w = 0;
+  for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    c += types[i] == OP_INSERT;
+    w += types[i] == OP_INSERT || types[i] == OP_DELETE;
+  }
   if (c) atomicAdd(&total, c);
+  if (w) atomicAdd(&wtotal, w);
   __syncthreads();
-  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+  if (threadIdx.x == 0) {
+    counts[blockIdx.x] = total;
+    writes[blockIdx.x] = wtotal;
+  }
 }
 
 // a rejected request's ops: status FAILED, value 0, table untouched
@@ -106,21 +115,26 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   hipStream_t s = static_cast<hipStream_t>(stream);
 
   // ---- set-up, before the trace clock starts
-  std::vector<uint32_t> h_ins(nreq);
+  std::vector<uint32_t> h_ins(nreq), h_wr(nreq);
   {
     DevArena ar;
     unsigned long long* d_off = nullptr;
     uint32_t* d_cnt = nullptr;
     HIP_TRY(ar.alloc(&d_off, nreq + 1));
-    HIP_TRY(ar.alloc(&d_cnt, nreq));
+    HIP_TRY(ar.alloc(&d_cnt, 2 * nreq));
     HIP_TRY(hipMemcpyAsync(d_off, h_req_offsets, (nreq + 1) * 8, hipMemcpyHostToDevice, s));
     for (size_t lo = 0; lo < nreq; lo += 0x7FFFFFFF) {  // grid.x limit
       const size_t cnt = std::min<size_t>(nreq - lo, 0x7FFFFFFF);
-      if (nops) k_req_inserts<<<static_cast<unsigned>(cnt), 256, 0, s>>>(types, d_off + lo, d_cnt + lo);
-      else HIP_TRY(hipMemsetAsync(d_cnt + lo, 0, cnt * 4, s));
+      if (nops) {
+        k_req_inserts<<<static_cast<unsigned>(cnt), 256, 0, s>>>(types, d_off + lo, d_cnt + lo, d_cnt + nreq + lo);
+      } else {
+        HIP_TRY(hipMemsetAsync(d_cnt + lo, 0, cnt * 4, s));
+        HIP_TRY(hipMemsetAsync(d_cnt + nreq + lo, 0, cnt * 4, s));
+      }
     }
     HIP_CHECK_LAUNCH();
     HIP_TRY(hipMemcpyAsync(h_ins.data(), d_cnt, nreq * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_wr.data(), d_cnt + nreq, nreq * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
   }
   uint64_t max_req = 0;
@@ -262,7 +276,11 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
         }
       }
       const size_t o = h_req_offsets[launched], n = h_req_offsets[end] - o;
-      if (c.ordered && end - launched > 1) {  // each request a batch of one window
+      // ordered: each request a batch of one window -- unless the requests
+      // write nothing, when every order gives the same results
+      bool writes = false;
+      for (size_t q = launched; c.ordered && q < end && !writes; ++q) writes = h_wr[q] != 0;
+      if (writes && end - launched > 1) {
         if (n && (r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
                                      tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, 0,
                                      ncu, s, s, nullptr)) != HIVE_OK)

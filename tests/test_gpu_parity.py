@@ -542,8 +542,8 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
 def test_windowed_stream_orders_each_key_by_batch(H, O, policy):
     """Windows of many small batches in which the same keys recur in every
     batch (a hot set in all of them, cold keys from a small space): each
-    key's ops must see exactly the batches before theirs, one round per
-    batch, against the oracle run batch by batch.  No key repeats inside a
+    key's ops must see exactly the batches before theirs (chains 63 batches
+    long), against the oracle run batch by batch.  No key repeats inside a
     batch, so every op has one admissible outcome.  A few ops are NONE /
     ATOMIC_INC or carry key 0, which a window must answer like a batch."""
     print(f"[case] windowed stream: recurring keys, {policy}", flush=True)
@@ -616,9 +616,10 @@ def test_bulk_autotune_leaves_table_unchanged(H, O):
 
 def test_windowed_stream_thousands_of_tiny_batches(H, O):
     """5000 batches of 2 ops over 4 keys: windows reach their 4096-batch cap,
-    and each key has a pending op in most batches of its window, so the
-    rounds kernel runs about two thousand rounds; results equal the oracle
-    run batch by batch (no key repeats inside a batch)."""
+    and each key has a deferred op in most batches of its window, so its
+    chain has about a thousand written batches (resolved by walking back);
+    results equal the oracle run batch by batch (no key repeats inside a
+    batch)."""
     print("[case] windowed stream: 5000 two-op batches over 4 keys", flush=True)
     rng = np.random.default_rng(17)
     nb = 5000

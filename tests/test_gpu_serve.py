@@ -191,3 +191,17 @@ def test_serve_ordered_results_same_alone_even_when_requests_conflict(H, O):
         assert np.array_equal(gv, ov), name
         assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), keyset), name
         t.close()
+    # requests that write nothing take the snapshot path in ordered mode
+    # (every order gives the same results): one batch, the oracle's lookups
+    lk = np.full(keys.size, 1, np.uint8)
+    o2 = O.Oracle(slots // 8)
+    o2.insert(pre, pre_v)
+    ov2, os2 = o2.mixed(lk, keys, vals)
+    t = H.HiveTable(slots)
+    t.insert(pre, pre_v)
+    gv, gs, lat, rep = t.serve(lk, keys, vals, off, np.zeros(nreq), max_batch_ops=1 << 20, ordered=1)
+    print(f"[case] serve ordered, lookups only: batches {rep['batches']}", flush=True)
+    assert rep["batches"] == 1
+    assert np.array_equal(norm(np_(gs)), norm(os2))
+    assert np.array_equal(np_(gv, np.uint32), ov2)
+    t.close()
