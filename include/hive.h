@@ -251,8 +251,10 @@ hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, c
  * requests runs as a window in which each request is a batch of its own
  * (HIVE_STREAM_WINDOWED), so every request gets the results it would get
  * alone, run after the requests that arrived before it (at most 4096
- * requests are coalesced into one batch); a batch whose requests neither
- * insert nor delete runs as a plain batch, which gives the same results.
+ * requests are coalesced into one batch, and a batch holds at most 2^25 ops:
+ * a larger max_batch_ops or request is HIVE_EINVAL); a batch whose requests
+ * neither insert nor delete runs as a plain batch, which gives the same
+ * results.
  * Admission: before a batch whose inserts could lift the load factor above
  * max_load_factor, the table is resized in place by `growth` if free HBM
  * keeps hbm_reserve_bytes after the new table; if not, requests that insert
