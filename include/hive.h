@@ -251,8 +251,9 @@ hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, c
  * requests runs as a window in which each request is a batch of its own
  * (HIVE_STREAM_WINDOWED), so every request gets the results it would get
  * alone, run after the requests that arrived before it (at most 4096
- * requests are coalesced into one batch, and a batch holds at most 2^25 ops:
- * a larger max_batch_ops or request is HIVE_EINVAL); a batch whose requests
+ * requests are coalesced into one batch; the call fails with HIVE_EINVAL
+ * when max_batch_ops or a request exceeds 2^25 ops in a trace of more than
+ * 2^25 ops); a batch whose requests
  * neither insert nor delete runs as a plain batch, which gives the same
  * results.
  * Admission: before a batch whose inserts could lift the load factor above
