@@ -205,3 +205,33 @@ def test_serve_ordered_results_same_alone_even_when_requests_conflict(H, O):
     assert np.array_equal(norm(np_(gs)), norm(os2))
     assert np.array_equal(np_(gv, np.uint32), ov2)
     t.close()
+
+
+def test_serve_ordered_survives_window_epoch_wrap(H, O):
+    """Window scratch is tagged with a 16-bit epoch and cleared when the
+    epoch wraps, after 65535 windows on one table.  One-op requests over 64
+    keys, coalesced two at a time, give about 75 k windows (a pair of two
+    lookups runs as a plain batch), so the wrap happens mid-trace; every
+    result must still equal the oracle run request by request."""
+    print("[case] serve ordered: window epoch wrap", flush=True)
+    rng = np.random.default_rng(2024)
+    n = 180000
+    keys = rng.integers(1, 65, n).astype(np.uint32)
+    types = rng.choice(3, size=n, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    vals = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    off = np.arange(n + 1, dtype=np.uint64)
+    writes = (types != 1).reshape(-1, 2).any(axis=1)
+    assert int(writes.sum()) > 70000  # windows: beyond the 65535 before the wrap
+    o = O.Oracle(1 << 9)
+    ov = np.empty(n, np.uint32)
+    os_ = np.empty(n, np.uint8)
+    for r in range(n):
+        ov[r:r + 1], os_[r:r + 1] = o.mixed(types[r:r + 1], keys[r:r + 1], vals[r:r + 1])
+    t = H.HiveTable(1 << 12)
+    gv, gs, lat, rep = t.serve(types, keys, vals, off, np.zeros(n), max_batch_ops=2, ordered=1)
+    print(f"[case] serve ordered wrap: batches {rep['batches']}", flush=True)
+    assert rep["batches"] == n // 2
+    assert np.array_equal(norm(np_(gs)), norm(os_))
+    assert np.array_equal(np_(gv, np.uint32), ov)
+    assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), np.sort(o.dump() & 0xFFFFFFFF))
+    t.close()
