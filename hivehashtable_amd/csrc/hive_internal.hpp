@@ -47,8 +47,8 @@ struct hive_table {
   size_t part_cap = 0;                   // ops the lists can hold
   uint64_t stream_window_ops = 0;        // windowed streams: most ops per window (0 = default)
   uint64_t stream_window_batch = 0;      // windowed streams: batches below this size are coalesced (0 = default)
-  unsigned long long* win_set = nullptr; // windowed streams: key set of the current window, then chain words
-  uint32_t* win_buf = nullptr;           // windowed streams: per-op and pending arrays, control blocks, chains
+  unsigned long long* win_set = nullptr; // windowed streams: key set of the current window, then chain records
+  uint32_t* win_buf = nullptr;           // windowed streams: two buffers of per-op arrays, chain marks, control block
   size_t win_cap = 0;                    // key-set entries (a power of two; windows of up to win_cap / 2 ops)
   uint32_t win_epoch = 0;                // epoch of the last window (entries of other epochs are free)
 };

@@ -526,7 +526,7 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
   return w;
 }
 
-// the chain words: one set, since the chain passes of consecutive windows
+// the chain records: one set, since the chain passes of consecutive windows
 // run in order on the table stream
 ChainArgs chain_args(const hive_table* t) {
   const size_t cap = t->win_cap;
@@ -538,8 +538,9 @@ ChainArgs chain_args(const hive_table* t) {
 }
 }  // namespace
 
-// key set (2 entries per op, a power of two), chain words, and two window
-// buffers (per-op and pending arrays) for windows of up to `ops` ops
+// key set (2 entries per op, a power of two), chain records, and two window
+// buffers (per-op arrays, chain marks, control block) for windows of up to
+// `ops` ops
 hive_status_t ensure_window(hive_table* t, size_t ops) {
   size_t cap = 1024;
   while (cap < 2 * ops) cap <<= 1;
