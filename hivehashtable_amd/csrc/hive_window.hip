@@ -135,9 +135,9 @@ __global__ __launch_bounds__(kWinBlock) void k_win_mark(const uint4* desc, uint3
 // Ops in a batch after their key's first write in the window are deferred:
 // typed OP_DEFERRED in place, with their key entry and fw << 16 | batch at
 // their window position (no list, so no shared counter); the key is marked
-// chained and the window's pending flag raised.  Ops up to and including that batch keep their type:
-// before it the key is only looked up, so they see the table as it was
-// before the window.
+// chained and the window's pending flag raised.  Ops up to and including that
+// batch keep their type: before it the key is only looked up, so they see the
+// table as it was before the window.
 __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint32_t base, const uint8_t* types,
                                                          const uint32_t* keys, uint32_t wlo,
                                                          const unsigned long long* set, uint32_t mask, uint32_t epoch,
