@@ -220,7 +220,7 @@ hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream)
     return r;
   }
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
-  const size_t nsrc = 2 * static_cast<size_t>(t->v.nb) * kSlots;
+  const size_t nsrc = rehash_threads(t->v.nb);
   const uint32_t blk = kDefaultBlock;
   HIVE_POLICY_SWITCH(t->policy, (k_rehash<P><<<grid_for(nsrc, blk), blk, 0, s>>>(nv, t->v, t->scratch)));
   if (hipGetLastError() != hipSuccess) {

@@ -460,8 +460,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_
 }
 
 // ---------------------------------------------------------------- rehash
-// One thread per source entry: slots [0, nb*8) then stash slots.  Entries are
-// inserted into dst with the same fast/slow protocol.
+// One thread per source entry, inserted into dst with the same fast/slow
+// protocol.  A wavefront takes one slot of a row of 64 consecutive source
+// buckets; region 0 holds the bucket slots, region 1 the stash slots.  The 8
+// keys of one source bucket share H0 in the source and land in the same two
+// destination buckets, where lanes running at the same time would race for
+// the same empty slots.  So a row's 8 slots go to wavefronts kRehashChunk
+// apart (rows are taken in chunks of kRehashChunk, slot by slot: about one
+// chip's worth of resident wavefronts between them), and a chunk's source
+// lines stay cached for its 8 passes.
+constexpr size_t kRehashRows = 64;      // source buckets per wavefront
+constexpr size_t kRehashChunk = 8192;   // rows per chunk
+static_assert(kStashSlots == kSlots, "a region covers the bucket or the stash slots");
+__host__ __device__ inline size_t rehash_rows(uint32_t nb) { return (static_cast<size_t>(nb) + kRehashRows - 1) / kRehashRows; }
+// threads of a k_rehash launch over a source of nb buckets
+inline size_t rehash_threads(uint32_t nb) { return 2 * rehash_rows(nb) * kSlots * kRehashRows; }
 template <class P>
 __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView src, unsigned long long* failed) {
   __shared__ unsigned long long queue[kMaxBlock];
@@ -469,15 +482,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
   if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t ntab = static_cast<size_t>(src.nb) * kSlots;
+  const size_t rows = rehash_rows(src.nb), per_region = rows * kSlots;
+  const size_t w = i / kRehashRows;
   unsigned long long kv = 0;
-  if (i < ntab) {
-    kv = src.slots[i];
-  } else if (i < 2 * ntab) {
-    const size_t e = i - ntab;
-    const uint32_t b = static_cast<uint32_t>(e / kStashSlots);
-    const uint32_t j = static_cast<uint32_t>(e % kStashSlots);
-    if (j < src.stash_count[b]) kv = src.stash[e];
+  if (w < 2 * per_region) {
+    const bool in_stash = w >= per_region;
+    const size_t wr = in_stash ? w - per_region : w;
+    const size_t chunk = wr / (kRehashChunk * kSlots), within = wr % (kRehashChunk * kSlots);
+    const size_t crow = chunk * kRehashChunk, nrow = rows - crow < kRehashChunk ? rows - crow : kRehashChunk;
+    const uint32_t slot = static_cast<uint32_t>(within / nrow);
+    const size_t b = (crow + within % nrow) * kRehashRows + i % kRehashRows;
+    if (b < src.nb) {
+      const size_t e = b * kSlots + slot;
+      if (!in_stash) kv = src.slots[e];
+      else if (slot < src.stash_count[b]) kv = src.stash[e];
+    }
   }
   int dtab = 0, dstash = 0;
   if (kv != 0ull) {

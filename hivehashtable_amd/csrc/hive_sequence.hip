@@ -132,7 +132,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       for (size_t b = 0; b < nbatch && rc == HIVE_OK; ++b) {
         if (steps[b].grow >= 0) {
           const TableView nv = views[steps[b].grow];
-          const size_t nsrc = 2 * static_cast<size_t>(t->v.nb) * kSlots;
+          const size_t nsrc = rehash_threads(t->v.nb);
           HIVE_POLICY_SWITCH(t->policy, (k_rehash<P><<<grid_for(nsrc, kDefaultBlock), kDefaultBlock, 0, cs>>>(
                                             nv, t->v, d_fail)));
           t->v = nv;
