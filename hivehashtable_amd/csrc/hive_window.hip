@@ -25,8 +25,8 @@
 //   chain kernels   the deferred ops of a key form its chain.  Its state
 //                   after fw is looked up once; every deferred op's result
 //                   follows from that state and the latest deferred write
-//                   batch before its own; only the chain's last write batch
-//                   is applied to the table.  No op waits for another.
+//                   batch before its own; only the chain's final state is
+//                   applied to the table.  No op waits for another.
 // Ops on different keys run concurrently, as inside one batch.  What the
 // table's capacity decides -- whether an insert lands in a bucket or the stash,
 // and an insert failing on a full stash -- can differ, as it can between two
@@ -436,30 +436,26 @@ __device__ void stamp_window(const ChainArgs& a) {
     __hip_atomic_store(a.ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Chain pass 3: erase the keys whose chain ends deleted.
+// Chain pass 3: apply every chain's final state -- erase the keys whose
+// chain ends deleted, upsert the final value of those whose chain ends with
+// an insert.  A chain applies one op, so the ops of this pass are on distinct
+// keys and run concurrently.  An insert the chain reports INSERTED takes the
+// table's placement (STASHED) or failure.  The last block to finish stamps
+// the window's batches.
 template <class P>
-__global__ __launch_bounds__(kWinBlock) void k_chain_erase(TableView t, ChainArgs a) {
-  if (a.ctl.pending()[0] == 0u) return;
-  int dtab = 0, dstash = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += gridDim.x * blockDim.x) {
-    if (a.wtype[j] != kApplyDelete) continue;
-    int rt, rs;
-    (void)erase_one<P>(t, a.keys[a.wlo + j], rt, rs);
-    dtab -= rt;
-    dstash -= rs;
-  }
-  wave_count(t, dtab, dstash);
-}
-
-// Chain pass 4: upsert the final value of the keys whose chain ends with an
-// insert; an insert the chain reports INSERTED takes the table's placement
-// (STASHED) or failure.  The last block to finish stamps the window's batches.
-template <class P>
-__global__ __launch_bounds__(kWinBlock) void k_chain_insert(TableView t, ChainArgs a) {
+__global__ __launch_bounds__(kWinBlock) void k_chain_apply(TableView t, ChainArgs a) {
   const bool any = a.ctl.pending()[0] != 0u;
   int dtab = 0, dstash = 0, dslow = 0;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; any && j < a.n; j += gridDim.x * blockDim.x) {
-    if (a.wtype[j] != kApplyInsert) continue;
+    const uint8_t wt = a.wtype[j];
+    if (wt == kApplyDelete) {
+      int rt, rs;
+      (void)erase_one<P>(t, a.keys[a.wlo + j], rt, rs);
+      dtab -= rt;
+      dstash -= rs;
+      continue;
+    }
+    if (wt != kApplyInsert) continue;
     const uint32_t i = a.wlo + j, key = a.keys[i], val = op_value(a, i);
     uint8_t r = fast_insert<P>(t, key, val);
     if (r == R_PENDING) {
@@ -632,9 +628,7 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   HIP_CHECK_LAUNCH();
   k_chain_resolve<<<grid, kWinBlock, 0, s>>>(ca);
   HIP_CHECK_LAUNCH();
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_erase<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
-  HIP_CHECK_LAUNCH();
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_insert<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }
