@@ -339,6 +339,33 @@ def test_dump_is_the_content_multiset(H, O):
     assert t.dump().numel() == 0
 
 
+@pytest.mark.parametrize("src_slots,n_keys,dst_slots", [
+    (1 << 22, 3_850_000, 9_000_000),   # LF 0.92 with a stash; 1 125 000 buckets: 3 row chunks, a partial row
+    (1 << 9, 200, 296),                # 37 destination buckets: one partial row
+])
+def test_resize_keeps_content_at_odd_sizes(H, src_slots, n_keys, dst_slots):
+    """Resize moves every entry, bucket slots and stash slots, whatever the
+    bucket count: the content multiset is unchanged and every key is found
+    with its value."""
+    print(f"[case] resize {src_slots} -> {dst_slots} slots, {n_keys} keys", flush=True)
+    keys = W.distinct_keys(n_keys, 808)
+    vals = W.values(n_keys, 809)
+    t = H.HiveTable(src_slots)
+    st = np_(t.insert(keys, vals))
+    assert placed(st).all()
+    before = np.sort(np_(t.dump()))
+    stash = t.stats()["stash_live"]
+    print(f"[case] stash entries before: {stash}", flush=True)
+    if n_keys > 1000:
+        assert stash > 0
+    t.resize(dst_slots)
+    assert t.num_buckets == dst_slots // 8
+    assert np.array_equal(np.sort(np_(t.dump())), before)
+    gv, gs = t.lookup(keys)
+    assert (np_(gs) == 4).all() and np.array_equal(np_(gv, np.uint32), vals)
+    t.close()
+
+
 def test_stash_drain_and_reinsert(H, O):
     print("[case] stash drain / reinsert", flush=True)
     nb = 1 << 10
