@@ -46,7 +46,7 @@ constexpr uint32_t kWinBlock = 256;       // ops per tile of the mark/level kern
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 constexpr uint32_t kClaimer = 0x80000000u;  // eidx flag: this op claimed its key's entry (entries < 2^26)
 constexpr uint32_t kNoWrite = 0xFFFFu;    // no written batch
-constexpr uint32_t kListed = 4;           // written batches kept per chain for the chain lookups
+constexpr uint32_t kListed = 24;          // written batches kept per chain for the chain lookups
 constexpr uint32_t kLine = 32;            // u32 words per control counter (own 128-byte line)
 
 // Window key-set entry: epoch (16) | first batch of the window that writes
@@ -185,9 +185,10 @@ __global__ __launch_bounds__(kWinBlock) void k_win_level(const uint4* desc, uint
 }
 
 // ---------------------------------------------------------------- chains
-// Chain state.  Every word is tagged with the window's epoch, so nothing is
-// cleared between windows.  A record is 32 bytes, so one access brings all of
-// its words.
+// Chain state.  Every word is tagged with the window's epoch (the listed
+// batches are read only below the count of a chain word of this epoch), so
+// nothing is cleared between windows.  A record is one access: 32 bytes for
+// a pair, a 64-byte line for a key.
 // A (key entry, written batch) pair of the window, in a write-batch hash
 // table (linear probing, slot h):
 struct PairRec {
@@ -200,10 +201,10 @@ struct PairRec {
 struct KeyRec {
   unsigned long long init;   // epoch << 33 | found << 32 | value: the key after fw
   unsigned long long chain;  // epoch << 48 | written batches << 16 | the last of them
-  uint32_t wl[kListed];      // epoch << 16 | batch: the first kListed written batches
+  uint16_t wl[kListed];      // the first kListed written batches, in noting order
 };
 
-static_assert(sizeof(PairRec) == 32 && sizeof(KeyRec) == 32, "chain records are 32 bytes");
+static_assert(sizeof(PairRec) == 32 && sizeof(KeyRec) == 64, "chain records are 32 and 64 bytes");
 
 struct ChainArgs {
   const uint8_t* types;
@@ -276,7 +277,7 @@ __device__ void note_pair(const ChainArgs& a, uint32_t e, uint32_t b, uint32_t i
                                       (static_cast<unsigned long long>(n + 1u) << 16) | last;
       const unsigned long long seen = cas64(&k.chain, cur, next);
       if (seen == cur) {
-        if (n < kListed) k.wl[n] = (a.epoch << 16) | b;
+        if (n < kListed) k.wl[n] = static_cast<uint16_t>(b);
         break;
       }
       cur = seen;
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_resolve(ChainArgs a) {
     if (nw <= kListed) {
       uint32_t w = kNoWrite;
       for (uint32_t s = 0; s < kListed; ++s) {
-        const uint32_t bb = kr.wl[s] & 0xFFFFu;
+        const uint32_t bb = kr.wl[s];
         if (s < nw && bb < b && (w == kNoWrite || bb > w)) w = bb;
       }
       if (w != kNoWrite) hw = wb_find(a, e, w);
@@ -489,7 +490,7 @@ __global__ void k_stamp_range(unsigned long long* ts, uint32_t n) {
 namespace hive {
 
 namespace {
-// win_set (u64, 9 cap): the key set [cap], then the chain records: pairs
+// win_set (u64, 13 cap): the key set [cap], then the chain records: pairs
 // [cap PairRec], keys [cap KeyRec].
 // win_buf (u32): per window buffer eidx, pbid, pe [cap / 2 each, by window
 // position], chained [cap, by key entry], wtype [cap / 2 bytes] and its
@@ -501,7 +502,7 @@ size_t ctl_offset(size_t cap) {
   return (3 * half + cap + (half + 3) / 4 + kLine - 1) / kLine * kLine;
 }
 size_t buffer_words(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
-constexpr size_t kSetWords = 9;  // u64 words of win_set per key-set slot
+constexpr size_t kSetWords = 1 + sizeof(PairRec) / 8 + sizeof(KeyRec) / 8;  // u64 words of win_set per key-set slot
 
 struct WindowBuffer {
   uint32_t *eidx, *pbid, *pe, *chained;
