@@ -77,6 +77,24 @@ struct EventPool {
   }
 };
 
+// Ordered windows alternate between the two window buffers; a window's
+// key-set passes run on a side stream as soon as it is launched, overlapping
+// the previous window's table work.  A window's table passes wait for its
+// key-set passes (ready), and a buffer's next key-set passes wait for its
+// last window's chain passes (done).
+struct SidePipe {
+  hipStream_t ms = nullptr;
+  hipEvent_t start = nullptr, ready[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+  bool used[2] = {false, false};
+  uint32_t k = 0;
+  ~SidePipe() {
+    if (ms) (void)hipStreamSynchronize(ms);
+    for (hipEvent_t e : {start, ready[0], ready[1], done[0], done[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (ms) (void)hipStreamDestroy(ms);
+  }
+};
+
 }  // namespace
 
 extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
@@ -144,6 +162,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   // ordered: tiles of 256 ops of one request each (numbered by request) for
   // every request of the trace, and window scratch for the largest batch
   DevArena ord_ar;
+  SidePipe pipe;
   std::vector<uint32_t> tile0;  // first tile of request r
   uint4* d_desc = nullptr;
   int ncu = 0;
@@ -167,6 +186,11 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     if ((r = ensure_window(t, std::min<uint64_t>(nops, std::max<uint64_t>(c.max_batch_ops, max_req)))) != HIVE_OK)
       return r;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
+    HIP_TRY(hipStreamCreateWithFlags(&pipe.ms, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&pipe.start, &pipe.ready[0], &pipe.ready[1], &pipe.done[0], &pipe.done[1]})
+      HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(pipe.start, s));  // inputs and tile descriptors written on s
+    HIP_TRY(hipStreamWaitEvent(pipe.ms, pipe.start, 0));
     HIP_TRY(hipStreamSynchronize(s));
   }
   hive_stats_t st{};
@@ -280,11 +304,15 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       // write nothing, when every order gives the same results
       bool writes = false;
       for (size_t q = launched; c.ordered && q < end && !writes; ++q) writes = h_wr[q] != 0;
-      if (writes && end - launched > 1) {
-        if (n && (r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
-                                     tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, 0,
-                                     ncu, s, s, nullptr)) != HIVE_OK)
+      if (writes && end - launched > 1 && n) {
+        const int p = static_cast<int>(pipe.k++ & 1u);
+        if (pipe.used[p]) HIP_TRY(hipStreamWaitEvent(pipe.ms, pipe.done[p], 0));
+        if ((r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
+                                tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, p, ncu,
+                                pipe.ms, s, pipe.ready[p])) != HIVE_OK)
           return r;
+        HIP_TRY(hipEventRecord(pipe.done[p], s));
+        pipe.used[p] = true;
       } else {
         SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
         OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
