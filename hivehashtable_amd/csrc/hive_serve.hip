@@ -21,7 +21,8 @@ using namespace hive;
 namespace {
 
 // inserts of each request: the admission test needs an upper bound of new keys
-// per request: its inserts, and its inserts plus deletes (writes)
+// per request: its inserts, and its inserts plus deletes (writes); the
+// request's types are read 16 at a time between its unaligned ends
 __global__ void k_req_inserts(const uint8_t* types, const unsigned long long* off, uint32_t* counts,
                               uint32_t* writes) {
   __shared__ uint32_t total, wtotal;
@@ -29,10 +30,21 @@ __global__ void k_req_inserts(const uint8_t* types, const unsigned long long* of
   __syncthreads();
   const unsigned long long lo = off[blockIdx.x], hi = off[blockIdx.x + 1];
   uint32_t c = 0, w = 0;
-  for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    c += types[i] == OP_INSERT;
-    w += types[i] == OP_INSERT || types[i] == OP_DELETE;
+  auto count = [&](uint32_t ty) {
+    c += ty == OP_INSERT;
+    w += ty == OP_INSERT || ty == OP_DELETE;
+  };
+  const uintptr_t base = reinterpret_cast<uintptr_t>(types);
+  unsigned long long a = lo + ((16u - ((base + lo) & 15u)) & 15u);  // first 16-byte aligned op
+  if (a > hi) a = hi;
+  const unsigned long long z = a + ((hi - a) & ~15ull);
+  for (unsigned long long i = lo + threadIdx.x; i < a; i += blockDim.x) count(types[i]);
+  for (unsigned long long i = a + 16ull * threadIdx.x; i < z; i += 16ull * blockDim.x) {
+    const uint4 v = *reinterpret_cast<const uint4*>(types + i);
+    for (uint32_t word : {v.x, v.y, v.z, v.w})
+      for (int k = 0; k < 4; ++k) count((word >> (8 * k)) & 0xFFu);
   }
+  for (unsigned long long i = z + threadIdx.x; i < hi; i += blockDim.x) count(types[i]);
   if (c) atomicAdd(&total, c);
   if (w) atomicAdd(&wtotal, w);
   __syncthreads();
