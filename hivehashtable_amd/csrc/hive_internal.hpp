@@ -161,17 +161,26 @@ inline hive_status_t ensure_partition(hive_table* t, size_t n) {
 }
 
 // a native mixed batch: split by type when it is large and scratch is there
-inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n, OutNative out, hipStream_t s,
-                                         BatchClock clk = BatchClock{nullptr, nullptr}) {
-  if (!t->launch[OP_INSERT].split_by_type || n < kCompactMinOps || n > t->part_cap || !src.types)
-    return launch_mixed(t, src, n, out, s, clk);
-  uint32_t* lkeys = t->part;
-  uint32_t* lvals = t->part + 3 * n;
-  uint32_t* lidx = t->part + 6 * n;
-  uint32_t* cnt = t->part + 9 * n;
-  k_fill_u32<<<1, 64, 0, s>>>(cnt, 3, 0u);
-  k_partition<<<grid_for(n, 1024), 1024, 0, s>>>(src.types, src.keys, src.vals, n, lkeys, lvals, lidx, cnt);
+// Type lists of a mixed batch of n ops in `part` (9 n + 3 words): keys, values
+// and op indices [3 n each, by type], then the three counts.
+inline size_t partition_words(size_t n) { return 9 * n + 3; }
+inline bool use_partition(const hive_table* t, size_t n, const uint8_t* types) {
+  return t->launch[OP_INSERT].split_by_type && n >= kCompactMinOps && n <= 0xFFFFFFFFull && types;
+}
+inline hive_status_t partition_batch(SrcNative src, size_t n, uint32_t* part, hipStream_t s) {
+  k_fill_u32<<<1, 64, 0, s>>>(part + 9 * n, 3, 0u);
+  k_partition<<<grid_for(n, 1024), 1024, 0, s>>>(src.types, src.keys, src.vals, n, part, part + 3 * n, part + 6 * n,
+                                                 part + 9 * n);
   HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+// the phases of a mixed batch whose inserts and deletes partition_batch listed
+inline hive_status_t launch_mixed_listed(hive_table* t, SrcNative src, size_t n, OutNative out, const uint32_t* part,
+                                         hipStream_t s, BatchClock clk = BatchClock{nullptr, nullptr}) {
+  const uint32_t* lkeys = part;
+  const uint32_t* lvals = part + 3 * n;
+  const uint32_t* lidx = part + 6 * n;
+  const uint32_t* cnt = part + 9 * n;
   hive_status_t r;
   auto list = [&](int ty) {
     const size_t o = static_cast<size_t>(ty) * n;
@@ -181,6 +190,14 @@ inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n,
   if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;  // also answers NONE ops
   if ((r = launch_erase(t, list(OP_DELETE), n, outs(OP_DELETE), OP_DELETE, s)) != HIVE_OK) return r;
   return launch_insert(t, list(OP_INSERT), n, outs(OP_INSERT), OP_INSERT, s, clk);
+}
+
+inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n, OutNative out, hipStream_t s,
+                                         BatchClock clk = BatchClock{nullptr, nullptr}) {
+  if (!use_partition(t, n, src.types) || n > t->part_cap) return launch_mixed(t, src, n, out, s, clk);
+  hive_status_t r;
+  if ((r = partition_batch(src, n, t->part, s)) != HIVE_OK) return r;
+  return launch_mixed_listed(t, src, n, out, t->part, s, clk);
 }
 
 // ---- windowed streams (hive_window.hip)

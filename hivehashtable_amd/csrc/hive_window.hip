@@ -493,19 +493,21 @@ namespace {
 // win_set (u64, 13 cap): the key set [cap], then the chain records: pairs
 // [cap PairRec], keys [cap KeyRec].
 // win_buf (u32): per window buffer eidx, pbid, pe [cap / 2 each, by window
-// position], chained [cap, by key entry], wtype [cap / 2 bytes] and its
-// control block on a 128-byte line.  chained is never cleared: every chained
-// key's word holds the window's epoch, and an old word that happens to hold
-// it only costs a lookup nobody reads.
+// position], chained [cap, by key entry], wtype [cap / 2 bytes], its control
+// block on a 128-byte line, and the type lists of its phase kernels
+// [partition_words(cap / 2)].  chained is never cleared: every chained key's
+// word holds the window's epoch, and an old word that happens to hold it
+// only costs a lookup nobody reads.
 size_t ctl_offset(size_t cap) {
   const size_t half = cap / 2;
   return (3 * half + cap + (half + 3) / 4 + kLine - 1) / kLine * kLine;
 }
-size_t buffer_words(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
+size_t part_offset(size_t cap) { return ctl_offset(cap) + kCtlCounters * kLine; }
+size_t buffer_words(size_t cap) { return (part_offset(cap) + partition_words(cap / 2) + kLine - 1) / kLine * kLine; }
 constexpr size_t kSetWords = 1 + sizeof(PairRec) / 8 + sizeof(KeyRec) / 8;  // u64 words of win_set per key-set slot
 
 struct WindowBuffer {
-  uint32_t *eidx, *pbid, *pe, *chained;
+  uint32_t *eidx, *pbid, *pe, *chained, *part;
   uint8_t* wtype;
   WinCtl ctl;
 };
@@ -520,6 +522,7 @@ WindowBuffer window_buffer(const hive_table* t, int p) {
   w.chained = base + 3 * half;
   w.wtype = reinterpret_cast<uint8_t*>(w.chained + cap);
   w.ctl = WinCtl{base + ctl_offset(cap)};
+  w.part = base + part_offset(cap);
   return w;
 }
 
@@ -600,12 +603,20 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   k_win_level<<<ntiles, kWinBlock, 0, ks>>>(desc, base, types, keys, static_cast<uint32_t>(lo), set, mask,
                                             t->win_epoch, wb.eidx, wb.wtype, wb.pbid, wb.pe, wb.chained, wb.ctl);
   HIP_CHECK_LAUNCH();
+  // the type lists of the phase kernels are built with the key-set passes,
+  // in the window's own buffer, so they too run ahead of the table stream
+  const SrcNative src{keys + lo, vals ? vals + lo : nullptr, wb.wtype};
+  const bool listed = use_partition(t, n, wb.wtype);
+  if (listed) {
+    hive_status_t rp = partition_batch(src, n, wb.part, ks);
+    if (rp != HIVE_OK) return rp;
+  }
   if (ks != s) {
     HIP_TRY(hipEventRecord(ready, ks));
     HIP_TRY(hipStreamWaitEvent(s, ready, 0));
   }
-  hive_status_t r = launch_mixed_native(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, wb.wtype}, n,
-                                        OutNative{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr}, s);
+  const OutNative out{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr};
+  hive_status_t r = listed ? launch_mixed_listed(t, src, n, out, wb.part, s) : launch_mixed(t, src, n, out, s);
   if (r != HIVE_OK) return r;
   ca.types = types;
   ca.keys = keys;

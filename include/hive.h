@@ -178,8 +178,8 @@ hive_status_t hive_set_stream_shape(hive_table_t* t, uint32_t tile_ops, uint32_t
 /* Windows of HIVE_STREAM_WINDOWED: runs of consecutive batches of fewer
  * than batch_ops ops each (0 = 2^19, at most 2^24) are coalesced into windows
  * of at most window_ops ops (2^10..2^24, 0 = 2^21); every other batch runs
- * alone.  The handle keeps 250-500 bytes of device scratch per window op
- * (125 bytes per slot of a power-of-two set of at least 2 slots per op) once
+ * alone.  The handle keeps 322-644 bytes of device scratch per window op
+ * (161 bytes per slot of a power-of-two set of at least 2 slots per op) once
  * a windowed stream has run. */
 hive_status_t hive_set_stream_window(hive_table_t* t, uint64_t window_ops, uint64_t batch_ops);
 
