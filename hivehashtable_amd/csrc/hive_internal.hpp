@@ -128,12 +128,18 @@ hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase,
 }
 
 // a mixed batch = three stream-ordered phases: lookups, deletes, inserts
+// Phases of a mixed batch that the caller knows to have ops (the lookup phase
+// always runs: it also answers NONE / ATOMIC_INC ops).  A phase without ops
+// still costs a launch whose blocks find nothing to do.
+constexpr uint32_t kPhaseDelete = 1, kPhaseInsert = 2, kAllPhases = 3;
+
 template <class Src, class Out>
 hive_status_t launch_mixed(hive_table* t, Src src, size_t n, Out out, hipStream_t s,
-                           BatchClock clk = BatchClock{nullptr, nullptr}) {
+                           BatchClock clk = BatchClock{nullptr, nullptr}, uint32_t phases = kAllPhases) {
   hive_status_t r;
   if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;
-  if ((r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
+  if ((phases & kPhaseDelete) && (r = launch_erase(t, src, n, out, OP_DELETE, s)) != HIVE_OK) return r;
+  if (!(phases & kPhaseInsert) && clk.ctr == nullptr) return HIVE_OK;
   return launch_insert(t, src, n, out, OP_INSERT, s, clk);
 }
 
@@ -176,7 +182,8 @@ inline hive_status_t partition_batch(SrcNative src, size_t n, uint32_t* part, hi
 }
 // the phases of a mixed batch whose inserts and deletes partition_batch listed
 inline hive_status_t launch_mixed_listed(hive_table* t, SrcNative src, size_t n, OutNative out, const uint32_t* part,
-                                         hipStream_t s, BatchClock clk = BatchClock{nullptr, nullptr}) {
+                                         hipStream_t s, BatchClock clk = BatchClock{nullptr, nullptr},
+                                         uint32_t phases = kAllPhases) {
   const uint32_t* lkeys = part;
   const uint32_t* lvals = part + 3 * n;
   const uint32_t* lidx = part + 6 * n;
@@ -188,16 +195,20 @@ inline hive_status_t launch_mixed_listed(hive_table* t, SrcNative src, size_t n,
   };
   auto outs = [&](int ty) { return OutIdx{out.status, out.vals, lidx + static_cast<size_t>(ty) * n}; };
   if ((r = launch_lookup(t, src, n, out, OP_LOOKUP, s)) != HIVE_OK) return r;  // also answers NONE ops
-  if ((r = launch_erase(t, list(OP_DELETE), n, outs(OP_DELETE), OP_DELETE, s)) != HIVE_OK) return r;
+  if ((phases & kPhaseDelete) &&
+      (r = launch_erase(t, list(OP_DELETE), n, outs(OP_DELETE), OP_DELETE, s)) != HIVE_OK)
+    return r;
+  if (!(phases & kPhaseInsert) && clk.ctr == nullptr) return HIVE_OK;
   return launch_insert(t, list(OP_INSERT), n, outs(OP_INSERT), OP_INSERT, s, clk);
 }
 
 inline hive_status_t launch_mixed_native(hive_table* t, SrcNative src, size_t n, OutNative out, hipStream_t s,
-                                         BatchClock clk = BatchClock{nullptr, nullptr}) {
-  if (!use_partition(t, n, src.types) || n > t->part_cap) return launch_mixed(t, src, n, out, s, clk);
+                                         BatchClock clk = BatchClock{nullptr, nullptr}, uint32_t phases = kAllPhases) {
+  if (!(phases & (kPhaseDelete | kPhaseInsert)) || !use_partition(t, n, src.types) || n > t->part_cap)
+    return launch_mixed(t, src, n, out, s, clk, phases);
   hive_status_t r;
   if ((r = partition_batch(src, n, t->part, s)) != HIVE_OK) return r;
-  return launch_mixed_listed(t, src, n, out, t->part, s, clk);
+  return launch_mixed_listed(t, src, n, out, t->part, s, clk, phases);
 }
 
 // ---- windowed streams (hive_window.hip)
@@ -219,7 +230,7 @@ hive_status_t enqueue_windows(hive_table* t, const uint8_t* types, const uint32_
 hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                              uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
                              uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
-                             hipStream_t ks, hipStream_t s, hipEvent_t ready);
+                             hipStream_t ks, hipStream_t s, hipEvent_t ready, uint32_t phases = kAllPhases);
 
 // device buffers freed together when a call returns
 struct DevArena {

@@ -312,23 +312,26 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
         }
       }
       const size_t o = h_req_offsets[launched], n = h_req_offsets[end] - o;
-      // ordered: each request a batch of one window -- unless the requests
-      // write nothing, when every order gives the same results
-      bool writes = false;
-      for (size_t q = launched; c.ordered && q < end && !writes; ++q) writes = h_wr[q] != 0;
-      if (writes && end - launched > 1 && n) {
+      // the phases the batch has ops for; ordered: each request a batch of
+      // one window -- unless the requests write nothing, when every order
+      // gives the same results
+      uint64_t wr = 0;
+      for (size_t q = launched; q < end; ++q) wr += h_wr[q];
+      const bool writes = wr != 0;
+      const uint32_t phases = (ins ? kPhaseInsert : 0u) | (wr > ins ? kPhaseDelete : 0u);
+      if (c.ordered && writes && end - launched > 1 && n) {
         const int p = static_cast<int>(pipe.k++ & 1u);
         if (pipe.used[p]) HIP_TRY(hipStreamWaitEvent(pipe.ms, pipe.done[p], 0));
         if ((r = enqueue_window(t, types, keys, vals, o, n, out_vals, status, d_desc + tile0[launched],
                                 tile0[end] - tile0[launched], static_cast<uint32_t>(launched), nullptr, 0, p, ncu,
-                                pipe.ms, s, pipe.ready[p])) != HIVE_OK)
+                                pipe.ms, s, pipe.ready[p], phases)) != HIVE_OK)
           return r;
         HIP_TRY(hipEventRecord(pipe.done[p], s));
         pipe.used[p] = true;
       } else {
         SrcNative src{keys + o, vals ? vals + o : nullptr, types + o};
         OutNative out{status ? status + o : nullptr, out_vals ? out_vals + o : nullptr};
-        if ((r = launch_mixed_native(t, src, n, out, s)) != HIVE_OK) return r;
+        if ((r = launch_mixed_native(t, src, n, out, s, BatchClock{nullptr, nullptr}, phases)) != HIVE_OK) return r;
       }
       hipEvent_t ev = pool.free.back();
       pool.free.pop_back();

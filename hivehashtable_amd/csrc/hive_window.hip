@@ -584,7 +584,7 @@ hive_status_t ensure_window(hive_table* t, size_t ops) {
 hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                              uint64_t lo, uint64_t n, uint32_t* out_vals, uint8_t* status, const uint4* desc,
                              uint32_t ntiles, uint32_t base, unsigned long long* ts, uint32_t nts, int p, int ncu,
-                             hipStream_t ks, hipStream_t s, hipEvent_t ready) {
+                             hipStream_t ks, hipStream_t s, hipEvent_t ready, uint32_t phases) {
   const size_t cap = t->win_cap;
   unsigned long long* set = t->win_set;
   const uint32_t mask = static_cast<uint32_t>(cap - 1);
@@ -606,7 +606,7 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   // the type lists of the phase kernels are built with the key-set passes,
   // in the window's own buffer, so they too run ahead of the table stream
   const SrcNative src{keys + lo, vals ? vals + lo : nullptr, wb.wtype};
-  const bool listed = use_partition(t, n, wb.wtype);
+  const bool listed = (phases & (kPhaseDelete | kPhaseInsert)) && use_partition(t, n, wb.wtype);
   if (listed) {
     hive_status_t rp = partition_batch(src, n, wb.part, ks);
     if (rp != HIVE_OK) return rp;
@@ -616,7 +616,9 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
     HIP_TRY(hipStreamWaitEvent(s, ready, 0));
   }
   const OutNative out{status ? status + lo : nullptr, out_vals ? out_vals + lo : nullptr};
-  hive_status_t r = listed ? launch_mixed_listed(t, src, n, out, wb.part, s) : launch_mixed(t, src, n, out, s);
+  const BatchClock none{nullptr, nullptr};
+  hive_status_t r = listed ? launch_mixed_listed(t, src, n, out, wb.part, s, none, phases)
+                           : launch_mixed(t, src, n, out, s, none, phases);
   if (r != HIVE_OK) return r;
   ca.types = types;
   ca.keys = keys;
