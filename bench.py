@@ -405,26 +405,73 @@ def growth_leg(a, H, W, torch, dev):
     return out
 
 
-def cpu_baseline(keys, vals, lk, threads, gpu_found, gpu_vals, policy, nb):
+# CPU share of a one-GPU job on the MI355X hosts: 16 threads, whatever the
+# host's hardware thread count is
+JOB_CPU_SHARE = 16
+
+
+def cpu_allotment():
+    """CPU threads this process may run at once: the job's CPU share, the
+    cgroup quota (cpu.max), the affinity mask, and the host's hardware
+    threads, whichever is lowest; plus the host's hardware thread count and
+    CPU model for the report."""
+    hw = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else hw
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) // int(period)))
+    except Exception:
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except Exception:
+        pass
+    use = min(x for x in (hw, allowed, quota, JOB_CPU_SHARE) if x)
+    return use, {"hardware_threads": hw, "affinity_threads": allowed, "cgroup_quota_cpus": quota,
+                 "job_cpu_share": JOB_CPU_SHARE, "cpu_model": model}
+
+
+def cpu_baseline(keys, vals, lk, threads, host, gpu_found, gpu_vals, policy, nb, runs=5):
+    """The multithreaded CPU path on one full step (clear, bulk insert, bulk
+    lookup) of the same workload, median of `runs` timed runs after one
+    untimed warm-up run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
     c = O.CpuTable(nb, policy, threads=threads)
-    st, t_ins = c.insert(keys, vals)
-    v, s, t_look = c.lookup(lk)
-    agree = bool(np.array_equal(s == O.R_FOUND, gpu_found) and np.array_equal(v, gpu_vals))
-    ops = keys.size + lk.size
+    t_ins, t_look = [], []
+    agree = True
+    for r in range(runs + 1):
+        c.clear()
+        st, ti = c.insert(keys, vals)
+        v, s, tl = c.lookup(lk)
+        if r == 0:
+            agree = bool(np.array_equal(s == O.R_FOUND, gpu_found) and np.array_equal(v, gpu_vals))
+            continue
+        t_ins.append(ti)
+        t_look.append(tl)
     del c
+    ti, tl = float(np.median(t_ins)), float(np.median(t_look))
+    ops = keys.size + lk.size
     return {
-        "value": round(ops / (t_ins + t_look) / 1e6, 3),
+        "value": round(ops / (ti + tl) / 1e6, 3),
         "unit": "Mops/s",
         "cores": threads,
         "kind": "port",
         "sample": (f"one full step at the same size: {keys.size} inserts + {lk.size} lookups, "
-                   f"{threads} std::thread workers"),
-        "insert_mops": round(keys.size / t_ins / 1e6, 3),
-        "lookup_mops": round(lk.size / t_look / 1e6, 3),
-        "seconds": round(t_ins + t_look, 3),
+                   f"{threads} std::thread workers, median of {runs} runs"),
+        "insert_mops": round(keys.size / ti / 1e6, 3),
+        "lookup_mops": round(lk.size / tl / 1e6, 3),
+        "seconds_per_run": round(ti + tl, 3),
+        "host": host,
         "lookups_agree_with_gpu": agree,
     }
 
@@ -602,10 +649,11 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline: {threads} threads, one full step")
-        cpu = cpu_baseline(keys, vals, lk, threads, found, lv, a.policy, nb)
-        log(f"cpu: {cpu['value']} Mops/s in {cpu['seconds']} s, agree={cpu['lookups_agree_with_gpu']}")
+        threads, host = cpu_allotment()
+        threads = a.cpu_threads or threads
+        log(f"cpu baseline: {threads} threads ({host}), one full step, median of 5")
+        cpu = cpu_baseline(keys, vals, lk, threads, host, found, lv, a.policy, nb)
+        log(f"cpu: {cpu['value']} Mops/s, {cpu['seconds_per_run']} s per run, agree={cpu['lookups_agree_with_gpu']}")
 
     out = {
         "metric": METRIC,

@@ -16,13 +16,20 @@
 namespace hive {
 
 // ---------------------------------------------------------------- sources
+// deferred_type(): the one op type a phase kernel skips without writing a
+// result -- OP_DEFERRED for the internal types of a window (an op its chain
+// kernels resolve), -1 for every caller-supplied source, whose ops of any
+// type other than insert / lookup / delete get R_NOOP
+// (include/hive_kernels.cuh:181-184).
 struct SrcNative {
   const uint32_t* keys;
   const uint32_t* vals;   // NULL: value = key + 1
   const uint8_t* types;   // NULL: every op has the kernel's own type
+  int deferred = -1;      // OP_DEFERRED only for a window's internal types
   __device__ __forceinline__ uint32_t key(size_t i) const { return keys[i]; }
   __device__ __forceinline__ uint32_t val(size_t i, uint32_t k) const { return vals ? vals[i] : k + 1u; }
   __device__ __forceinline__ int type(size_t i, int dflt) const { return types ? static_cast<int>(types[i]) : dflt; }
+  __device__ __forceinline__ int deferred_type() const { return deferred; }
 };
 
 struct SrcOps {  // value = key + 1 (include/hive_kernels.cuh:156,238)
@@ -30,6 +37,7 @@ struct SrcOps {  // value = key + 1 (include/hive_kernels.cuh:156,238)
   __device__ __forceinline__ uint32_t key(size_t i) const { return static_cast<uint32_t>(ops[i].key); }
   __device__ __forceinline__ uint32_t val(size_t, uint32_t k) const { return k + 1u; }
   __device__ __forceinline__ int type(size_t i, int) const { return ops[i].type; }
+  __device__ __forceinline__ int deferred_type() const { return -1; }
 };
 
 struct SrcKV {  // packed kv records (stash re-insert)
@@ -37,6 +45,7 @@ struct SrcKV {  // packed kv records (stash re-insert)
   __device__ __forceinline__ uint32_t key(size_t i) const { return kv_key(kv[i]); }
   __device__ __forceinline__ uint32_t val(size_t i, uint32_t) const { return kv_val(kv[i]); }
   __device__ __forceinline__ int type(size_t, int dflt) const { return dflt; }
+  __device__ __forceinline__ int deferred_type() const { return -1; }
 };
 
 template <class Src>
@@ -57,6 +66,7 @@ struct SrcList {
   __device__ __forceinline__ uint32_t key(size_t j) const { return keys[j]; }
   __device__ __forceinline__ uint32_t val(size_t j, uint32_t k) const { return vals ? vals[j] : k + 1u; }
   __device__ __forceinline__ int type(size_t j, int) const { return j < *cnt ? own : (own + 1) % 3; }
+  __device__ __forceinline__ int deferred_type() const { return -1; }
 };
 
 template <>
@@ -341,7 +351,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
   if (phase >= 0) {
     const int ty = src.type(i, OP_LOOKUP);
     if (ty != OP_LOOKUP) {
-      if (ty != OP_INSERT && ty != OP_DELETE && ty != OP_DEFERRED) out.put(i, R_NOOP, 0);
+      if (ty != OP_INSERT && ty != OP_DELETE && ty != src.deferred_type()) out.put(i, R_NOOP, 0);
       return;
     }
   }
@@ -407,7 +417,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
   if (phase >= 0) {
     const int ty = src.type(i, OP_LOOKUP);
     if (ty != OP_LOOKUP) {
-      if (r == 0 && ty != OP_INSERT && ty != OP_DELETE && ty != OP_DEFERRED) out.put(i, R_NOOP, 0);
+      if (r == 0 && ty != OP_INSERT && ty != OP_DELETE && ty != src.deferred_type()) out.put(i, R_NOOP, 0);
       return;
     }
   }

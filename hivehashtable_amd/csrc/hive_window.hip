@@ -437,23 +437,30 @@ __device__ void stamp_window(const ChainArgs& a) {
     __hip_atomic_store(a.ts + j, s_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Chain pass 3: apply every chain's final state -- erase the keys whose
-// chain ends deleted, upsert the final value of those whose chain ends with
-// an insert.  A chain applies one op, so the ops of this pass are on distinct
-// keys and run concurrently.  An insert the chain reports INSERTED takes the
-// table's placement (STASHED) or failure.  The last block to finish stamps
-// the window's batches.
+// Chain pass 3, two launches: apply every chain's final state -- first erase
+// the keys whose chain ends deleted (inserts = false), then upsert the final
+// value of those whose chain ends with an insert (inserts = true).  A chain
+// applies one op, so the ops of one launch are on distinct keys and run
+// concurrently.  Erases and inserts stay in separate launches, as in every
+// phase sequence of this library: an erase reads a key's hint word and its
+// buckets with plain loads and no lock, so a concurrent cuckoo kick of that
+// key (hint bit set, copy to the alternate bucket, removal from H0 between
+// the erase's two reads) could hide it from the erase.  An insert the chain
+// reports INSERTED takes the table's placement (STASHED) or failure.  The
+// last block of the insert launch stamps the window's batches.
 template <class P>
-__global__ __launch_bounds__(kWinBlock) void k_chain_apply(TableView t, ChainArgs a) {
+__global__ __launch_bounds__(kWinBlock) void k_chain_apply(TableView t, ChainArgs a, bool inserts) {
   const bool any = a.ctl.pending()[0] != 0u;
   int dtab = 0, dstash = 0, dslow = 0;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; any && j < a.n; j += gridDim.x * blockDim.x) {
     const uint8_t wt = a.wtype[j];
-    if (wt == kApplyDelete) {
-      int rt, rs;
-      (void)erase_one<P>(t, a.keys[a.wlo + j], rt, rs);
-      dtab -= rt;
-      dstash -= rs;
+    if (!inserts) {
+      if (wt == kApplyDelete) {
+        int rt, rs;
+        (void)erase_one<P>(t, a.keys[a.wlo + j], rt, rs);
+        dtab -= rt;
+        dstash -= rs;
+      }
       continue;
     }
     if (wt != kApplyInsert) continue;
@@ -468,6 +475,7 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_apply(TableView t, ChainArg
     if (a.status && (r == R_FAILED || (r == R_STASHED && a.status[i] == R_INSERTED))) a.status[i] = r;
   }
   wave_count(t, dtab, dstash, dslow);
+  if (!inserts) return;
   __shared__ uint32_t s_last;
   wait_vmem();
   __syncthreads();
@@ -605,7 +613,7 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   HIP_CHECK_LAUNCH();
   // the type lists of the phase kernels are built with the key-set passes,
   // in the window's own buffer, so they too run ahead of the table stream
-  const SrcNative src{keys + lo, vals ? vals + lo : nullptr, wb.wtype};
+  const SrcNative src{keys + lo, vals ? vals + lo : nullptr, wb.wtype, OP_DEFERRED};
   const bool listed = (phases & (kPhaseDelete | kPhaseInsert)) && use_partition(t, n, wb.wtype);
   if (listed) {
     hive_status_t rp = partition_batch(src, n, wb.part, ks);
@@ -642,7 +650,9 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   HIP_CHECK_LAUNCH();
   k_chain_resolve<<<grid, kWinBlock, 0, s>>>(ca);
   HIP_CHECK_LAUNCH();
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, false)));
+  HIP_CHECK_LAUNCH();
+  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, true)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }

@@ -389,4 +389,37 @@ int hcpu_hardware_threads(void) {
   return n ? static_cast<int>(n) : 1;
 }
 
+// Full-domain image size of a raw mix: thread j evaluates a contiguous share
+// of the 2^32 inputs and sets each output's bit in one shared bitmap (atomic
+// or), then the set bits are counted.
+uint64_t ho_distinct_outputs(int fn, int threads) {
+  uint32_t (*mix)(uint32_t) = nullptr;
+  switch (fn) {
+    case 0: mix = ho_hash1; break;
+    case 1: mix = ho_hash2; break;
+    case 2: mix = ho_murmur; break;
+    case 3: mix = ho_city; break;
+    default: return 0;
+  }
+  if (threads < 1) threads = 1;
+  constexpr uint64_t kDomain = uint64_t(1) << 32;
+  std::vector<uint64_t> bits(kDomain / 64, 0ull);
+  uint64_t* bm = bits.data();
+  std::vector<std::thread> pool;
+  for (int j = 0; j < threads; ++j) {
+    pool.emplace_back([=] {
+      const uint64_t lo = kDomain * static_cast<uint64_t>(j) / static_cast<uint64_t>(threads);
+      const uint64_t hi = kDomain * static_cast<uint64_t>(j + 1) / static_cast<uint64_t>(threads);
+      for (uint64_t x = lo; x < hi; ++x) {
+        const uint32_t y = mix(static_cast<uint32_t>(x));
+        __atomic_fetch_or(&bm[y >> 6], 1ull << (y & 63u), __ATOMIC_RELAXED);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  uint64_t n = 0;
+  for (uint64_t w : bits) n += static_cast<uint64_t>(__builtin_popcountll(w));
+  return n;
+}
+
 }  // extern "C"

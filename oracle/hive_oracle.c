@@ -53,7 +53,11 @@ uint32_t ho_hash2(uint32_t key) {
 
 static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
-static uint32_t fmix32(uint32_t h) {
+/* The MurmurHash3 32-bit finalizer.  include/hash.hpp:100-104 (cityhash)
+ * and :130-134 (murmurhash) both end with these five steps; the published
+ * MurmurHash3_x86_32 vectors of the empty string pin it directly, since for
+ * len 0 the hash is fmix32(seed). */
+uint32_t ho_fmix32(uint32_t h) {
   h ^= h >> 16;
   h *= 0x85ebca6bu;
   h ^= h >> 13;
@@ -91,7 +95,7 @@ uint32_t ho_murmur_seeded(const uint8_t* data, size_t len, uint32_t seed) {
       h ^= k1;
   }
   h ^= (uint32_t)len;
-  return fmix32(h);
+  return ho_fmix32(h);
 }
 
 /* include/hash.hpp:110-138: MurmurHash3_x86_32 of the 4 little-endian key
@@ -104,12 +108,17 @@ uint32_t ho_murmur(uint32_t key) {
   h = rotl32(h, 13);
   h = h * 5u + 0xe6546b64u;
   h ^= 4u;
-  return fmix32(h);
+  return ho_fmix32(h);
 }
 
-/* include/hash.hpp:80-108.  The 64-bit constants are combined with a 32-bit
- * variable, so every step is 32-bit modular arithmetic. */
-uint32_t ho_city(uint32_t key) {
+/* include/hash.hpp:80-108.  The constants k0..k3 are declared uint64_t there
+ * and combined with uint32_t variables by compound assignment, which
+ * truncates to 32 bits after every step; the sister project's copy
+ * (UniformHashFunction/src/hash_function.cu:191-219) declares the same
+ * constants uint32_t.  Both are 32-bit modular arithmetic: xor by a constant
+ * and multiplication by an odd constant (both invertible on u32), then the
+ * invertible fmix32, so the whole function is a permutation of u32. */
+uint32_t ho_city_premix(uint32_t key) {
   uint32_t k = key;
   k ^= 0x9ae16a3bu;
   k *= 0xc2b2ae35u;
@@ -120,8 +129,10 @@ uint32_t ho_city(uint32_t key) {
   h ^= 0x1b873593u;
   h *= 0x85ebca6bu;
   h ^= 4u;
-  return fmix32(h);
+  return h;
 }
+
+uint32_t ho_city(uint32_t key) { return ho_fmix32(ho_city_premix(key)); }
 
 /* include/hash.hpp:143-179: reflected table CRCs with the reference's own
  * polynomials, init all-ones, final xor all-ones, over the 4 LE key bytes. */

@@ -60,6 +60,12 @@ uint32_t ho_hash2(uint32_t key);
 uint32_t ho_murmur(uint32_t key);
 uint32_t ho_murmur_seeded(const uint8_t* data, size_t len, uint32_t seed);
 uint32_t ho_city(uint32_t key);
+uint32_t ho_city_premix(uint32_t key);  /* cityhash before its fmix32 finalizer */
+uint32_t ho_fmix32(uint32_t h);         /* MurmurHash3 finalizer shared by murmur and city */
+/* Number of distinct values of a raw mix over all 2^32 inputs (fn: 0 hash1,
+ * 1 hash2, 2 murmur, 3 city), counted in a 512 MiB bitmap by `threads`
+ * threads; 2^32 iff the mix is a permutation of u32.  0 on bad arguments. */
+uint64_t ho_distinct_outputs(int fn, int threads);
 uint32_t ho_crc64(uint32_t key);
 uint32_t ho_crc32(uint32_t key);
 uint32_t ho_crc64_bitwise(uint32_t key);

@@ -48,10 +48,12 @@ def lib():
         build()
     L = C.CDLL(LIB_PATH)
     u32, u64, sz, p = C.c_uint32, C.c_uint64, C.c_size_t, C.c_void_p
-    for name in ("ho_hash1", "ho_hash2", "ho_murmur", "ho_city", "ho_crc64", "ho_crc32",
-                 "ho_crc64_bitwise", "ho_crc32_bitwise"):
+    for name in ("ho_hash1", "ho_hash2", "ho_murmur", "ho_city", "ho_city_premix", "ho_fmix32", "ho_crc64",
+                 "ho_crc32", "ho_crc64_bitwise", "ho_crc32_bitwise"):
         getattr(L, name).restype = u32
         getattr(L, name).argtypes = [u32]
+    L.ho_distinct_outputs.restype = u64
+    L.ho_distinct_outputs.argtypes = [C.c_int, C.c_int]
     L.ho_murmur_seeded.restype = u32
     L.ho_murmur_seeded.argtypes = [C.c_char_p, sz, u32]
     L.ho_num_hashes.restype = C.c_int

@@ -235,3 +235,82 @@ def test_serve_ordered_survives_window_epoch_wrap(H, O):
     assert np.array_equal(np_(gv, np.uint32), ov)
     assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), np.sort(o.dump() & 0xFFFFFFFF))
     t.close()
+
+
+def dedupe_request_writes(types, keys, off):
+    """Make every op of a request single-outcome: a key written twice inside
+    one request (Zipf-hot keys are) keeps its first write, and the later
+    writes become lookups (two inserts of one key in one batch may apply in
+    either order; across requests, order is what ordered serving fixes)."""
+    types = types.copy()
+    nreq = off.size - 1
+    req = np.repeat(np.arange(nreq), np.diff(off).astype(np.int64))
+    w = np.flatnonzero(types != 1)
+    comb = (req[w].astype(np.uint64) << np.uint64(32)) | keys[w].astype(np.uint64)
+    _, first = np.unique(comb, return_index=True)
+    later = np.ones(w.size, bool)
+    later[first] = False
+    types[w[later]] = 1
+    return types
+
+
+@pytest.mark.parametrize("mix", ["A", "B", "C", "D"])
+def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
+    """The serving configuration at reduced size: a 2^16-key preload, 200
+    requests of log-uniform 2^4..2^10 ops, Zipf theta 0.99 over the preload
+    (YCSB A 50/50 read/update, B 95/5, C reads), and D (95 % reads of the
+    latest keys, 5 % new keys) on a table sized just under LF 0.9, so that
+    admission resizes it.  Ordered serving, with Poisson arrivals and all at
+    once, must give every request exactly the oracle's results run request by
+    request (bit-exact status and value, and the final content with values).
+    Snapshot serving must agree on the ops whose key no other request writes."""
+    import math
+    print(f"[case] serve cfg5 mix {mix}", flush=True)
+    n_pre = 1 << 16
+    pre = W.distinct_keys(n_pre, 601)
+    pre_v = W.values(n_pre, 602)
+    fresh = W.distinct_keys(20000, 601, start=n_pre + 2)
+    types, keys, vals, off, arr = W.serving_trace(mix, pre, 200, 50.0, 603 + ord(mix), size_lo=1 << 4,
+                                                  size_hi=1 << 10, fresh=fresh if mix == "D" else None)
+    types = dedupe_request_writes(types, keys, off)
+    nreq = off.size - 1
+    slots = int(math.ceil(n_pre / 0.899 / 8)) * 8 if mix == "D" else 1 << 17
+    o = O.Oracle(slots // 8)
+    o.insert(pre, pre_v)
+    ov = np.empty(keys.size, np.uint32)
+    os_ = np.empty(keys.size, np.uint8)
+    for r in range(nreq):
+        lo, hi = int(off[r]), int(off[r + 1])
+        ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+    ref_content = np.sort(o.dump())
+    # single-outcome ops of snapshot serving: keys no other request writes
+    req = np.repeat(np.arange(nreq), np.diff(off).astype(np.int64))
+    w = types != 1
+    writers = {}
+    for k, r in zip(keys[w].tolist(), req[w].tolist()):
+        writers.setdefault(k, set()).add(r)
+    alone = np.array([len(writers.get(k, ())) == 0 or writers[k] == {r} for k, r in
+                      zip(keys.tolist(), req.tolist())])
+    if mix != "C":
+        assert (~alone).any()  # the trace has conflicting requests
+    for name, arrivals, ordered in (("ordered poisson", arr, 1), ("ordered saturated", np.zeros(nreq), 1),
+                                    ("snapshot poisson", arr, 0)):
+        t = H.HiveTable(slots)
+        t.set_launch(0, 1024, inline_slow_path=1, combine_duplicates=1, split_by_type=1)
+        t.insert(pre, pre_v)
+        gv, gs, lat, rep = t.serve(types, keys, vals, off, arrivals, ordered=ordered)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        print(f"[case] serve cfg5 {mix} {name}: batches {rep['batches']}, resizes {rep['resizes']}, "
+              f"p50 {rep['p50_us']:.1f} us, p99 {rep['p99_us']:.1f} us", flush=True)
+        assert rep["rejected_requests"] == 0 and rep["ops"] == keys.size
+        if mix == "D":
+            assert rep["resizes"] >= 1 and rep["final_slots"] > slots
+        if ordered:
+            assert np.array_equal(norm(gs), norm(os_)), name
+            assert np.array_equal(gv, ov), name
+            assert np.array_equal(np.sort(np_(t.dump()).view(np.uint64)), ref_content), name
+        else:
+            assert np.array_equal(norm(gs)[alone], norm(os_)[alone]), name
+            assert np.array_equal(gv[alone], ov[alone]), name
+            assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), np.sort(ref_content & np.uint64(0xFFFFFFFF))), name
+        t.close()

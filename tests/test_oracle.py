@@ -97,6 +97,81 @@ def test_city_matches_cxx_promotion_semantics(L):
     assert np.array_equal(got, hashref.city(sub))
 
 
+# The published MurmurHash3_x86_32 vectors of the empty string are fmix32(seed)
+# (no blocks, no tail, length 0), so they pin the finalizer on its own.
+EMPTY_STRING_VECTORS = [(s, h) for d, s, h in MURMUR_VECTORS if d == b""]
+
+
+def test_fmix32_pinned_by_published_vectors(L):
+    assert len(EMPTY_STRING_VECTORS) == 3
+    for seed, expect in EMPTY_STRING_VECTORS:
+        assert L.ho_fmix32(seed) == expect
+
+
+def test_city_is_fmix32_of_its_premix(L):
+    # include/hash.hpp:100-104 ends cityhash with the same five finalizer steps
+    # as murmurhash (:130-134), pinned above by the published vectors
+    for k in SAMPLE[:3000].tolist():
+        assert L.ho_city(k) == L.ho_fmix32(L.ho_city_premix(k))
+
+
+def _city_mutant(keys, wide_state, wide_finalizer):
+    """Two plausible misreadings of include/hash.hpp:80-108, on uint64
+    arithmetic: the 64-bit constants carried into a 64-bit state with no
+    truncation after the multiplies (wide_state), and the finalizer run on a
+    64-bit value (wide_finalizer), truncated to 32 bits only at the end."""
+    M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+    M32 = np.uint64(0xFFFFFFFF)
+    k = keys.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        k = k ^ np.uint64(0x9ae16a3b)
+        k = k * np.uint64(0xc2b2ae35)
+        k = k ^ np.uint64(0x1b873593)
+        k = k * np.uint64(0x85ebca6b)
+        if not wide_state:
+            k &= M32
+        h = k * np.uint64(0xc2b2ae35)
+        h = h ^ np.uint64(0x1b873593)
+        h = h * np.uint64(0x85ebca6b)
+        h = h ^ np.uint64(4)
+        h = h & (M64 if wide_finalizer else M32)
+        h ^= h >> np.uint64(16)
+        h = (h * np.uint64(0x85ebca6b)) & (M64 if wide_finalizer else M32)
+        h ^= h >> np.uint64(13)
+        h = (h * np.uint64(0xc2b2ae35)) & (M64 if wide_finalizer else M32)
+        h ^= h >> np.uint64(16)
+    return (h & M32).astype(np.uint32)
+
+
+def test_city_collision_check_rejects_plausible_mistakes(L):
+    # the full-domain check below fails a mix that is not a permutation of u32;
+    # both misreadings collide already on 2^20 consecutive inputs (about 128
+    # collisions expected for a random function), while ho_city does not
+    ks = np.arange(1, (1 << 20) + 1, dtype=np.uint32)
+    ref = np.array([L.ho_city(k) for k in ks[:2000].tolist()], np.uint32)
+    assert np.array_equal(_city_mutant(ks[:2000], False, False), ref)
+    # the low 32 bits of xors and products depend only on the low 32 bits of
+    # their operands, so a 64-bit state truncated before the finalizer is the
+    # same function; what changes the result is a finalizer that sees the
+    # high bits (its right shifts bring them down)
+    assert np.array_equal(_city_mutant(ks[:2000], True, False), ref)
+    for wide_state in (False, True):
+        m = _city_mutant(ks, wide_state, True)
+        assert np.unique(m).size < ks.size, wide_state
+
+
+def test_city_is_a_permutation_of_u32(L):
+    # every step of cityhash is xor by a constant, multiplication by an odd
+    # constant (mod 2^32) or fmix32, all invertible: all 2^32 inputs must give
+    # 2^32 distinct outputs
+    for c in (0x9ae16a3b, 0xc2b2ae35, 0x1b873593, 0x85ebca6b):
+        assert c < 2**32
+    for c in (0xc2b2ae35, 0x85ebca6b):
+        assert c & 1
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    assert L.ho_distinct_outputs(3, threads) == 1 << 32
+
+
 def test_crc_table_equals_bit_serial(L):
     for k in SAMPLE[:5000].tolist():
         assert L.ho_crc64(k) == L.ho_crc64_bitwise(k)
