@@ -263,7 +263,10 @@ def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
     admission resizes it.  Ordered serving, with Poisson arrivals and all at
     once, must give every request exactly the oracle's results run request by
     request (bit-exact status and value, and the final content with values).
-    Snapshot serving must agree on the ops whose key no other request writes."""
+    Snapshot serving must agree on the ops whose key no other request writes.
+    The oracle starts from the GPU table's content after the preload: at
+    LF 0.9 in a 9113-bucket table a few preload inserts can find their
+    canonical stash full, and which ones is a placement outcome."""
     import math
     print(f"[case] serve cfg5 mix {mix}", flush=True)
     n_pre = 1 << 16
@@ -275,14 +278,6 @@ def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
     types = dedupe_request_writes(types, keys, off)
     nreq = off.size - 1
     slots = int(math.ceil(n_pre / 0.899 / 8)) * 8 if mix == "D" else 1 << 17
-    o = O.Oracle(slots // 8)
-    o.insert(pre, pre_v)
-    ov = np.empty(keys.size, np.uint32)
-    os_ = np.empty(keys.size, np.uint8)
-    for r in range(nreq):
-        lo, hi = int(off[r]), int(off[r + 1])
-        ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
-    ref_content = np.sort(o.dump())
     # single-outcome ops of snapshot serving: keys no other request writes
     req = np.repeat(np.arange(nreq), np.diff(off).astype(np.int64))
     w = types != 1
@@ -297,7 +292,18 @@ def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
                                     ("snapshot poisson", arr, 0)):
         t = H.HiveTable(slots)
         t.set_launch(0, 1024, inline_slow_path=1, combine_duplicates=1, split_by_type=1)
-        t.insert(pre, pre_v)
+        pst = np_(t.insert(pre, pre_v))
+        assert (pst == 0).sum() <= 16
+        start = np_(t.dump()).view(np.uint64)
+        o = O.Oracle(4 * slots // 8)  # room for every insert of the trace
+        o.insert((start & np.uint64(0xFFFFFFFF)).astype(np.uint32), (start >> np.uint64(32)).astype(np.uint32))
+        ov = np.empty(keys.size, np.uint32)
+        os_ = np.empty(keys.size, np.uint8)
+        for r in range(nreq):
+            lo, hi = int(off[r]), int(off[r + 1])
+            ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+        assert not (os_ == 0).any()
+        ref_content = np.sort(o.dump())
         gv, gs, lat, rep = t.serve(types, keys, vals, off, arrivals, ordered=ordered)
         gv, gs = np_(gv, np.uint32), np_(gs)
         print(f"[case] serve cfg5 {mix} {name}: batches {rep['batches']}, resizes {rep['resizes']}, "
@@ -312,5 +318,6 @@ def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
         else:
             assert np.array_equal(norm(gs)[alone], norm(os_)[alone]), name
             assert np.array_equal(gv[alone], ov[alone]), name
-            assert np.array_equal(np.sort(np_(t.dump()) & 0xFFFFFFFF), np.sort(ref_content & np.uint64(0xFFFFFFFF))), name
+            assert np.array_equal(np.sort(np_(t.dump()).view(np.uint64) & np.uint64(0xFFFFFFFF)),
+                                  np.sort(ref_content & np.uint64(0xFFFFFFFF))), name
         t.close()
