@@ -118,6 +118,7 @@ def lib():
         "hive_lookup": [p, p, sz, p, p, p],
         "hive_erase": [p, p, sz, p, p],
         "hive_mixed": [p, p, p, p, sz, p, p, p],
+        "hive_mixed_noalloc": [p, p, p, p, sz, p, p, p],
         "hive_insert_ops": [p, p, sz, p, p],
         "hive_lookup_ops": [p, p, sz, p, p],
         "hive_mixed_ops": [p, p, sz, p, p],

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -61,6 +62,39 @@ hive_status_t reset_counters(hive_table* t, const TableView& v, hipStream_t s, u
   return HIVE_OK;
 }
 
+// ---- the no-allocation mixed batch (hive_mixed_noalloc)
+// The type lists of one mixed batch of up to kNoAllocMaxOps ops live in a
+// file-scope device array, one copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in, never the device
+// memory: its device address is read once per device with
+// hipGetSymbolAddress (noalloc_setup, from hive_create) and only that
+// address is passed to kernels.
+constexpr size_t kNoAllocMaxOps = size_t(1) << 20;
+constexpr size_t kNoAllocWords = (partition_words(kNoAllocMaxOps) + 63) / 64 * 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+uint32_t* g_noalloc_ptr[kMaxDevices] = {};  // scratch's device address on each device (once set up)
+bool g_noalloc_done[kMaxDevices] = {};
+hipError_t g_noalloc_err[kMaxDevices] = {};
+
+// with `dev` current: look up scratch's address on it, once
+hive_status_t noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(HIVE_EDEVICE, "device ordinal above 63");
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_done[dev]) {
+    void* p = nullptr;
+    g_noalloc_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (g_noalloc_err[dev] == hipSuccess) g_noalloc_ptr[dev] = static_cast<uint32_t*>(p);
+    else (void)hipGetLastError();
+    g_noalloc_done[dev] = true;
+  }
+  if (g_noalloc_err[dev] != hipSuccess)
+    return fail(HIVE_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_noalloc_err[dev]));
+  return HIVE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,6 +135,10 @@ hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
   HIP_TRY(hipGetDevice(&dev));
   auto* t = new (std::nothrow) hive_table();
   if (!t) return fail(HIVE_ENOMEM, "host allocation");
+  if (hive_status_t rs = noalloc_setup(dev); rs != HIVE_OK) {
+    delete t;
+    return rs;
+  }
   t->device = dev;
   t->policy = policy;
   hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb));
@@ -178,6 +216,22 @@ hive_status_t hive_mixed(hive_table_t* t, const uint8_t* types, const uint32_t* 
   if ((r = set_device(t)) != HIVE_OK || (r = ensure_partition(t, n)) != HIVE_OK) return r;
   return launch_mixed_native(t, SrcNative{keys, vals, types}, n, OutNative{status, out_vals},
                              static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_mixed_noalloc(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                                 size_t n, uint32_t* out_vals, uint8_t* status, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (n && (!keys || !types)) return fail(HIVE_EINVAL, "null keys or types");
+  if (n > kNoAllocMaxOps) return fail(HIVE_EINVAL, "hive_mixed_noalloc: more than 2^20 ops in one batch");
+  if ((r = set_device(t)) != HIVE_OK || (r = noalloc_setup(t->device)) != HIVE_OK) return r;
+  const SrcNative src{keys, vals, types};
+  const OutNative out{status, out_vals};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!use_partition(t, n, types)) return launch_mixed(t, src, n, out, s);
+  uint32_t* part = g_noalloc_ptr[t->device];
+  if ((r = partition_batch(src, n, part, s)) != HIVE_OK) return r;  // clears the list counts first
+  return launch_mixed_listed(t, src, n, out, part, s);
 }
 
 hive_status_t hive_insert_ops(hive_table_t* t, const hive_operation_t* ops, size_t n, uint64_t* results,

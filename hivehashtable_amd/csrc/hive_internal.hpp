@@ -169,7 +169,7 @@ inline hive_status_t ensure_partition(hive_table* t, size_t n) {
 // a native mixed batch: split by type when it is large and scratch is there
 // Type lists of a mixed batch of n ops in `part` (9 n + 3 words): keys, values
 // and op indices [3 n each, by type], then the three counts.
-inline size_t partition_words(size_t n) { return 9 * n + 3; }
+constexpr size_t partition_words(size_t n) { return 9 * n + 3; }
 inline bool use_partition(const hive_table* t, size_t n, const uint8_t* types) {
   return t->launch[OP_INSERT].split_by_type && n >= kCompactMinOps && n <= 0xFFFFFFFFull && types;
 }

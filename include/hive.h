@@ -136,6 +136,19 @@ hive_status_t hive_erase(hive_table_t* t, const uint32_t* keys, size_t n, uint8_
 hive_status_t hive_mixed(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                          size_t n, uint32_t* out_vals, uint8_t* status, void* stream);
 
+/* hive_mixed without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory during the call -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, semantics and results as hive_mixed; the type lists of a
+ * batch of >= 2^18 ops (split_by_type) live in a static device array of the
+ * library (36 MiB per GPU, held until the process ends) instead of a buffer
+ * the handle allocates on first use.  At most 2^20 ops per call (HIVE_EINVAL
+ * beyond).  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (other streams, other threads, other handles) race, so callers
+ * order them. */
+hive_status_t hive_mixed_noalloc(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
+                                 size_t n, uint32_t* out_vals, uint8_t* status, void* stream);
+
 /* The reference's kernel interface over its 16-byte Operation records with
  * uint64 0/1 results and value = key + 1:
  *   hive_insert_ops -> hive_insert_kernel (include/hive_kernels.cuh:205-266)
