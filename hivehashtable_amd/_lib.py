@@ -111,6 +111,9 @@ def lib():
     st, p, sz, u32, u64, i32 = C.c_int, C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
     proto = {
         "hive_create": [u64, i32, C.POINTER(p)],
+        "hive_create_ex": [u64, i32, u32, C.POINTER(p)],
+        "hive_set_slots_per_bucket": [p, u32, p],
+        "hive_ceiling_gather_bytes": [p, u64, u64, u32, u32, p, p],
         "hive_destroy": [p],
         "hive_clear": [p, p],
         "hive_policy_from_name": [C.c_char_p, C.POINTER(i32)],

@@ -40,7 +40,7 @@ hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_
 
 hive_status_t count_exact_view(hive_table* t, const TableView& v, hipStream_t s, uint64_t* tab, uint64_t* st) {
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
-  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
+  const size_t ns = static_cast<size_t>(v.nb) * v.S;
   k_count_exact<<<fill_grid(ns), 256, 0, s>>>(v, t->scratch);
   HIP_CHECK_LAUNCH();
   unsigned long long h[2];
@@ -95,6 +95,48 @@ hive_status_t noalloc_setup(int dev) {
   return HIVE_OK;
 }
 
+// Rehash every live entry (buckets, then stash) into a new table of
+// num_slots / width buckets of `width` slots, with the insert protocol; swap
+// it in, or keep the old table on HIVE_ECAPACITY.
+hive_status_t relayout(hive_table* t, uint64_t num_slots, uint32_t width, hipStream_t s) {
+  hive_status_t r;
+  if ((r = set_device(t)) != HIVE_OK) return r;
+  const uint64_t nb = num_slots / width;
+  if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "new size must give 1 .. 2^32-1 buckets");
+  TableView nv;
+  if ((r = alloc_view(nv, static_cast<uint32_t>(nb), width)) != HIVE_OK) return r;
+  if ((r = clear_view(nv, s)) != HIVE_OK) {
+    free_view(nv);
+    return r;
+  }
+  HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
+  const size_t nsrc = rehash_threads(t->v.nb, t->v.S);
+  const uint32_t blk = kDefaultBlock;
+  const bool src16 = t->v.S == 16;
+  HIVE_POLICY_SWITCH(table_kind(t->policy, width), (src16 ? k_rehash<P, 16><<<grid_for(nsrc, blk), blk, 0, s>>>(
+                                                                nv, t->v, t->scratch)
+                                                          : k_rehash<P, 8><<<grid_for(nsrc, blk), blk, 0, s>>>(
+                                                                nv, t->v, t->scratch)));
+  if (hipGetLastError() != hipSuccess) {
+    free_view(nv);
+    return fail(HIVE_EDEVICE, "rehash launch");
+  }
+  unsigned long long failed = 0;
+  if (hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    free_view(nv);
+    return fail(HIVE_EDEVICE, "rehash sync");
+  }
+  if (failed) {
+    free_view(nv);
+    return fail(HIVE_ECAPACITY, "resize target cannot hold every live entry");
+  }
+  free_view(t->v);
+  t->v = nv;
+  t->bytes = view_bytes(static_cast<uint32_t>(nb), width) + 16;
+  return HIVE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -126,10 +168,15 @@ hive_status_t hive_policy_from_name(const char* name, int* policy) {
 }
 
 hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
+  return hive_create_ex(num_slots, policy, kRefSlots, out);
+}
+
+hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_bucket, hive_table_t** out) {
   if (!out) return fail(HIVE_EINVAL, "null out");
   *out = nullptr;
   if (policy < 0 || policy > HIVE_POLICY_LOOKUP3) return fail(HIVE_EINVAL, "unknown policy");
-  const uint64_t nb = num_slots / kSlots;
+  if (!valid_width(slots_per_bucket)) return fail(HIVE_EINVAL, "slots_per_bucket must be 8 or 16");
+  const uint64_t nb = num_slots / slots_per_bucket;
   if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "num_slots must give 1 .. 2^32-1 buckets");
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -141,7 +188,7 @@ hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
   }
   t->device = dev;
   t->policy = policy;
-  hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb));
+  hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb), slots_per_bucket);
   if (r != HIVE_OK) {
     delete t;
     return r;
@@ -151,7 +198,7 @@ hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
     delete t;
     return fail(HIVE_ENOMEM, "hipMalloc scratch");
   }
-  t->bytes = view_bytes(static_cast<uint32_t>(nb)) + 16;
+  t->bytes = view_bytes(static_cast<uint32_t>(nb), slots_per_bucket) + 16;
   if ((r = clear_view(t->v, nullptr)) != HIVE_OK || hipStreamSynchronize(nullptr) != hipSuccess) {
     free_view(t->v);
     (void)hipFree(t->scratch);
@@ -263,38 +310,16 @@ hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_
 
 hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream) {
   hive_status_t r;
-  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
-  const uint64_t nb = new_num_slots / kSlots;
-  if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "new size must give 1 .. 2^32-1 buckets");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  TableView nv;
-  if ((r = alloc_view(nv, static_cast<uint32_t>(nb))) != HIVE_OK) return r;
-  if ((r = clear_view(nv, s)) != HIVE_OK) {
-    free_view(nv);
-    return r;
-  }
-  HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
-  const size_t nsrc = rehash_threads(t->v.nb);
-  const uint32_t blk = kDefaultBlock;
-  HIVE_POLICY_SWITCH(t->policy, (k_rehash<P><<<grid_for(nsrc, blk), blk, 0, s>>>(nv, t->v, t->scratch)));
-  if (hipGetLastError() != hipSuccess) {
-    free_view(nv);
-    return fail(HIVE_EDEVICE, "rehash launch");
-  }
-  unsigned long long failed = 0;
-  if (hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
-    free_view(nv);
-    return fail(HIVE_EDEVICE, "rehash sync");
-  }
-  if (failed) {
-    free_view(nv);
-    return fail(HIVE_ECAPACITY, "resize target cannot hold every live entry");
-  }
-  free_view(t->v);
-  t->v = nv;
-  t->bytes = view_bytes(static_cast<uint32_t>(nb)) + 16;
-  return HIVE_OK;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  return relayout(t, new_num_slots, t->v.S, static_cast<hipStream_t>(stream));
+}
+
+hive_status_t hive_set_slots_per_bucket(hive_table_t* t, uint32_t slots_per_bucket, void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK) return r;
+  if (!valid_width(slots_per_bucket)) return fail(HIVE_EINVAL, "slots_per_bucket must be 8 or 16");
+  if (slots_per_bucket == t->v.S) return HIVE_OK;
+  return relayout(t, static_cast<uint64_t>(t->v.nb) * t->v.S, slots_per_bucket, static_cast<hipStream_t>(stream));
 }
 
 hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream) {
@@ -337,7 +362,7 @@ hive_status_t hive_dump(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n
   if (!out_kv) return fail(HIVE_EINVAL, "null out_kv");
   auto* out = reinterpret_cast<unsigned long long*>(out_kv);
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 8, s));
-  k_dump_slots<<<grid_for(static_cast<size_t>(t->v.nb) * kSlots, 256), 256, 0, s>>>(t->v, out, t->scratch);
+  k_dump_slots<<<grid_for(static_cast<size_t>(t->v.nb) * t->v.S, 256), 256, 0, s>>>(t->v, out, t->scratch);
   k_stash_compact<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, out, t->scratch);
   HIP_CHECK_LAUNCH();
   unsigned long long written = 0;
@@ -379,8 +404,8 @@ hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream) {
   if ((r = read_counters(t, static_cast<hipStream_t>(stream), &tab, &st, &slow)) != HIVE_OK) return r;
   out->slow_path_ops = slow;
   out->num_buckets = t->v.nb;
-  out->slots_per_bucket = kSlots;
-  out->capacity = static_cast<uint64_t>(t->v.nb) * kSlots;
+  out->slots_per_bucket = t->v.S;
+  out->capacity = static_cast<uint64_t>(t->v.nb) * t->v.S;
   out->table_live = tab;
   out->stash_live = st;
   out->load_factor = static_cast<double>(tab) / static_cast<double>(out->capacity);
@@ -411,14 +436,7 @@ hive_status_t hive_probe_stats(hive_table_t* t, const uint32_t* keys, size_t n, 
   std::vector<unsigned long long> h(kWords, 0ull);
   hipError_t e = hipMemsetAsync(acc, 0, kWords * 8, s);
   if (e == hipSuccess && n) {
-    switch (t->policy) {
-      case HIVE_POLICY_DEFAULT2HASH: k_probe_stats<PolicyDefault2><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-      case HIVE_POLICY_MURMURCITY: k_probe_stats<PolicyMurmurCity><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-      case HIVE_POLICY_TRIPLE: k_probe_stats<PolicyTriple><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-      case HIVE_POLICY_MURMURCITYBIT: k_probe_stats<PolicyMurmurCityBit><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-      case HIVE_POLICY_LOOKUP2: k_probe_stats<PolicyLookup2><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-      default: k_probe_stats<PolicyLookup3><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc); break;
-    }
+    HIVE_POLICY_SWITCH(table_kind(t), (k_probe_stats<P><<<grid_for(n, 256), 256, 0, s>>>(t->v, keys, n, acc)));
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), acc, kWords * 8, hipMemcpyDeviceToHost, s);
@@ -440,7 +458,7 @@ hive_status_t hive_export_raw(hive_table_t* t, uint64_t* h_slots, uint64_t* h_st
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t nb = t->v.nb;
-  if (h_slots) HIP_TRY(hipMemcpyAsync(h_slots, t->v.slots, nb * kSlots * 8, hipMemcpyDeviceToHost, s));
+  if (h_slots) HIP_TRY(hipMemcpyAsync(h_slots, t->v.slots, nb * t->v.S * 8, hipMemcpyDeviceToHost, s));
   if (h_stash) HIP_TRY(hipMemcpyAsync(h_stash, t->v.stash, nb * kStashSlots * 8, hipMemcpyDeviceToHost, s));
   if (h_stash_count) HIP_TRY(hipMemcpyAsync(h_stash_count, t->v.stash_count, nb * 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -456,11 +474,11 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
   const size_t nb = t->v.nb;
   for (size_t b = 0; b < nb; ++b)
     if (h_stash_count[b] > static_cast<uint32_t>(kStashSlots)) return fail(HIVE_EINVAL, "stash count above 8");
-  HIP_TRY(hipMemcpyAsync(t->v.slots, h_slots, nb * kSlots * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(t->v.slots, h_slots, nb * t->v.S * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash_count, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));
   k_fill_u32<<<fill_grid(meta_words(t->v.nb)), 256, 0, s>>>(t->v.meta, meta_words(t->v.nb), 0u);
-  HIVE_POLICY_SWITCH(t->policy, (k_rebuild_meta<P><<<grid_for(nb * kSlots, 256), 256, 0, s>>>(t->v)));
+  HIVE_POLICY_SWITCH(table_kind(t), (k_rebuild_meta<P><<<grid_for(nb * t->v.S, 256), 256, 0, s>>>(t->v)));
   k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.locks, t->v.nb, 0u);
   HIP_CHECK_LAUNCH();
   uint64_t a = 0, b = 0;
@@ -499,7 +517,20 @@ hive_status_t hive_ceiling_gather(const uint64_t* buf, uint64_t footprint_bytes,
   const uint64_t nbk = footprint_bytes / 64;
   if (!buf || nbk == 0 || nbk > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "bad ceiling buffer");
   if (n == 0) return HIVE_OK;
-  k_ceiling_gather<<<grid_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+  k_ceiling_gather<kRefSlots><<<grid_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      reinterpret_cast<const unsigned long long*>(buf), static_cast<uint32_t>(nbk), n, seed, sink);
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
+
+hive_status_t hive_ceiling_gather_bytes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
+                                       uint32_t bucket_bytes, uint32_t seed, uint32_t* sink, void* stream) {
+  if (bucket_bytes == 64) return hive_ceiling_gather(buf, footprint_bytes, n, seed, sink, stream);
+  if (bucket_bytes != 128) return fail(HIVE_EINVAL, "bucket_bytes must be 64 or 128");
+  const uint64_t nbk = footprint_bytes / 128;
+  if (!buf || nbk == 0 || nbk > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "bad ceiling buffer");
+  if (n == 0) return HIVE_OK;
+  k_ceiling_gather<16><<<grid_for(n, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(
       reinterpret_cast<const unsigned long long*>(buf), static_cast<uint32_t>(nbk), n, seed, sink);
   HIP_CHECK_LAUNCH();
   return HIVE_OK;

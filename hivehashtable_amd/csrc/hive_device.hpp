@@ -21,7 +21,12 @@
 
 namespace hive {
 
-constexpr int kSlots = 8;        // reference bucket: 8 x 8 B (include/hive_hash_table_struct.cuh:20-22)
+// Bucket width: slots of 8 B per bucket.  8 is the reference bucket
+// (include/hive_hash_table_struct.cuh:20-22: 64 bytes of payload in a
+// 128-byte struct); 16 fills the 128-byte line a bucket read fetches anyway.
+// The width is a property of the policy type (P::kS), so every protocol
+// function and kernel is instantiated per width.
+constexpr int kRefSlots = 8;
 constexpr int kStashSlots = 8;   // include/hive_stash_table.cuh:20
 constexpr int kMaxKicks = 8;     // include/hive_hash_table.cuh:21
 constexpr int kCounterLanes = 64;
@@ -59,13 +64,14 @@ struct Op16 {  // the reference's Operation record (include/utils.h:18-21)
 };
 
 struct TableView {
-  unsigned long long* slots;        // nb * 8
+  unsigned long long* slots;        // nb * S
   unsigned long long* stash;        // nb * 8
   uint32_t* stash_count;            // nb
   uint32_t* meta;                   // kHintBits per bucket, kHintsPerWord buckets per word
   uint32_t* locks;                  // nb
   unsigned long long* counters;     // [3][kCounterLanes][kCounterStride]: table, stash, slow-path ops
   uint32_t nb;
+  uint32_t S;                       // slots per bucket (8 or 16), for code not instantiated per width
 };
 
 // ------------------------------------------------------------------ hashes
@@ -158,6 +164,7 @@ __device__ __forceinline__ uint32_t mix_crc32(uint32_t key) {
 // (cached) loads, valid because a kernel boundary separates every phase.
 struct PolicyBase {
   static constexpr bool kCoherentReads = false;
+  static constexpr int kS = kRefSlots;  // slots per bucket
 };
 // include/HashPolicies.cuh:10-65
 struct PolicyDefault2 : PolicyBase {
@@ -189,6 +196,12 @@ struct PolicyLookup3 : PolicyBase {
   __device__ static __forceinline__ uint32_t raw(int i, uint32_t k) {
     return i == 0 ? mix_crc64(k) : (i == 1 ? mix_crc32(k) : mix_hash1(k));
   }
+};
+
+// The same hashes over 16-slot buckets: one 128-byte line per bucket.
+template <class P>
+struct Wide16 : P {
+  static constexpr int kS = 16;
 };
 
 // The same hashes with every table read a device-coherent load: for code that
@@ -261,25 +274,27 @@ __device__ __forceinline__ void unlock(uint32_t* l) {
   __hip_atomic_store(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// plain 64-byte bucket read (four 16-byte loads)
-__device__ __forceinline__ void load_bucket(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+// plain bucket read (S / 2 16-byte loads: four for 64 bytes, eight for 128)
+template <int S>
+__device__ __forceinline__ void load_bucket(const unsigned long long* p, unsigned long long (&s)[S]) {
   const uint4* q = reinterpret_cast<const uint4*>(p);
 #pragma unroll
-  for (int j = 0; j < kSlots / 2; ++j) {
+  for (int j = 0; j < S / 2; ++j) {
     uint4 v = q[j];
     s[2 * j] = (static_cast<unsigned long long>(v.y) << 32) | v.x;
     s[2 * j + 1] = (static_cast<unsigned long long>(v.w) << 32) | v.z;
   }
 }
 
-// coherent 64-byte bucket read: four 16-byte volatile loads, which compile to
+// coherent bucket read: 16-byte volatile loads, which compile to
 // system-coherent loads (sc0 sc1) that bypass stale L2 lines; each aligned
 // 8-byte slot inside a 16-byte access is read whole.
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+template <int S>
+__device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, unsigned long long (&s)[S]) {
   const volatile u64x2* q = reinterpret_cast<const volatile u64x2*>(p);
 #pragma unroll
-  for (int j = 0; j < kSlots / 2; ++j) {
+  for (int j = 0; j < S / 2; ++j) {
     const u64x2 v = q[j];
     s[2 * j] = v.x;
     s[2 * j + 1] = v.y;
@@ -287,8 +302,8 @@ __device__ __forceinline__ void load_bucket_sync(const unsigned long long* p, un
 }
 
 // table reads outside locked sections: plain, or coherent under CoherentReads
-template <class P>
-__device__ __forceinline__ void read_bucket(const unsigned long long* p, unsigned long long (&s)[kSlots]) {
+template <class P, int S>
+__device__ __forceinline__ void read_bucket(const unsigned long long* p, unsigned long long (&s)[S]) {
   if constexpr (P::kCoherentReads) load_bucket_sync(p, s);
   else load_bucket(p, s);
 }
@@ -303,17 +318,19 @@ __device__ __forceinline__ unsigned long long read_u64(const unsigned long long*
   else return *p;
 }
 
-__device__ __forceinline__ uint32_t match_mask(const unsigned long long (&s)[kSlots], uint32_t key) {
+template <int S>
+__device__ __forceinline__ uint32_t match_mask(const unsigned long long (&s)[S], uint32_t key) {
   uint32_t m = 0;
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) m |= static_cast<uint32_t>(kv_key(s[j]) == key) << j;
+  for (int j = 0; j < S; ++j) m |= static_cast<uint32_t>(kv_key(s[j]) == key) << j;
   return m;  // key != 0, so an empty slot never matches
 }
 
-__device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[kSlots]) {
+template <int S>
+__device__ __forceinline__ uint32_t empty_mask(const unsigned long long (&s)[S]) {
   uint32_t m = 0;
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) m |= static_cast<uint32_t>(s[j] == 0ull) << j;
+  for (int j = 0; j < S; ++j) m |= static_cast<uint32_t>(s[j] == 0ull) << j;
   return m;
 }
 
@@ -350,8 +367,9 @@ __device__ __forceinline__ uint32_t ovf_bit(uint32_t key) {
   return 1u << static_cast<uint32_t>((static_cast<unsigned long long>(x) * kOverflowClasses) >> 32);
 }
 
+template <class P>
 __device__ __forceinline__ unsigned long long* bucket_ptr(const TableView& t, uint32_t b) {
-  return t.slots + static_cast<size_t>(b) * kSlots;
+  return t.slots + static_cast<size_t>(b) * P::kS;
 }
 __device__ __forceinline__ unsigned long long* stash_ptr(const TableView& t, uint32_t b) {
   return t.stash + static_cast<size_t>(b) * kStashSlots;
@@ -379,32 +397,35 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  unsigned long long s0[kSlots];
-  read_bucket<P>(bucket_ptr(t, b[0]), s0);
-  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+  uint32_t m0, e0, hint;
+  unsigned long long cur0 = 0;  // the lowest matching slot's value
   {
-    const uint32_t m = match_mask(s0, key);
-    if (m) {
-      const int j = __builtin_ctz(m);
-      return update_slot(bucket_ptr(t, b[0]) + j, s0[j], kv) ? R_UPDATED : R_PENDING;
-    }
+    unsigned long long s0[P::kS];
+    read_bucket<P>(bucket_ptr<P>(t, b[0]), s0);
+    hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+    m0 = match_mask(s0, key);
+    e0 = empty_mask(s0);
+#pragma unroll
+    for (int q = 0; q < P::kS; ++q)
+      if (m0 && q == __builtin_ctz(m0)) cur0 = s0[q];
   }
+  if (m0) return update_slot(bucket_ptr<P>(t, b[0]) + __builtin_ctz(m0), cur0, kv) ? R_UPDATED : R_PENDING;
   if (hint & ovf_bit(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
-      unsigned long long s[kSlots];
-      read_bucket<P>(bucket_ptr(t, b[i]), s);
+      unsigned long long s[P::kS];
+      read_bucket<P>(bucket_ptr<P>(t, b[i]), s);
       const uint32_t m = match_mask(s, key);
       if (m) {
         const int j = __builtin_ctz(m);
-        return update_slot(bucket_ptr(t, b[i]) + j, s[j], kv) ? R_UPDATED : R_PENDING;
+        return update_slot(bucket_ptr<P>(t, b[i]) + j, s[j], kv) ? R_UPDATED : R_PENDING;
       }
     }
   }
   if (hint & kStashBit) return R_PENDING;  // Stage A' runs under the lock
-  uint32_t e = empty_mask(s0);
-  unsigned long long* cb = bucket_ptr(t, b[0]);
+  uint32_t e = e0;
+  unsigned long long* cb = bucket_ptr<P>(t, b[0]);
   while (e) {
     const int j = __builtin_ctz(e);
     const unsigned long long seen = cas64(cb + j, 0ull, kv);
@@ -415,11 +436,13 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
   return R_PENDING;
 }
 
-// Stage A on one bucket with coherent reads (all eight slots requested at
-// once, then scanned): 1 updated, 0 absent, -1 raced.
+// Stage A on one bucket (N slots: a bucket's S, or the 8 of a stash bucket)
+// with coherent reads (all N slots requested at once, then scanned), among
+// its first nslots slots: 1 updated, 0 absent, -1 raced.
+template <int N>
 __device__ __forceinline__ int locked_update(unsigned long long* bk, int nslots, uint32_t key,
                                              unsigned long long kv) {
-  unsigned long long s[kSlots];
+  unsigned long long s[N];
   load_bucket_sync(bk, s);
   uint32_t m = match_mask(s, key) & ((1u << nslots) - 1u);
   if (m) {
@@ -468,30 +491,46 @@ __device__ __forceinline__ bool finish_move(unsigned long long* src, unsigned lo
 template <class P>
 __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val, uint32_t c,
                                  unsigned long long pre_victim, uint32_t pre_alt) {
+  constexpr int S = P::kS;
+  constexpr int NV = kMaxKicks < S ? kMaxKicks : S;  // the kicks' victims: slots 0 .. NV-1
   const unsigned long long kv = kv_pack(key, val);
-  unsigned long long* cb = bucket_ptr(t, c);
+  unsigned long long* cb = bucket_ptr<P>(t, c);
   unsigned long long* sb = stash_ptr(t, c);
-  unsigned long long pre[kSlots];
+  // Buckets are reduced to what the protocol needs as soon as they arrive
+  // (match and empty masks, the matching slot, the victims), so that no more
+  // than the two buckets of one round are live in registers.
+  uint32_t pre_empty = 0;  // empty slots of pre_alt, read with the first coherent round
   bool have_pre = false;
   for (int attempt = 0; attempt < 256; ++attempt) {
     bool retry = false;
     // one coherent round: the canonical bucket and its hint bits.  Only
     // holders of this lock move keys whose H0 is c out of c, or push to
     // stash[c], so the hint read here is exact for this key.
-    unsigned long long cs[kSlots];
-    load_bucket_sync(cb, cs);
-    const uint32_t hint = meta_bits(ld_sync32(t.meta + c / kHintsPerWord), c);
-    if (attempt == 0 && pre_alt != ~0u) {
-      read_bucket<P>(bucket_ptr(t, pre_alt), pre);
-      have_pre = true;
-    }
+    uint32_t m, e;
+    unsigned long long cur = 0;  // the lowest matching slot's value
+    unsigned long long vic[NV];
+    uint32_t hint;
     {
-      const uint32_t m = match_mask(cs, key);
-      if (m) {
-        const int j = __builtin_ctz(m);
-        if (update_slot(cb + j, cs[j], kv)) return R_UPDATED;
-        continue;
+      unsigned long long cs[S];
+      load_bucket_sync(cb, cs);
+      hint = meta_bits(ld_sync32(t.meta + c / kHintsPerWord), c);
+      if (attempt == 0 && pre_alt != ~0u) {
+        unsigned long long pre[S];
+        read_bucket<P>(bucket_ptr<P>(t, pre_alt), pre);
+        pre_empty = empty_mask(pre);
+        have_pre = true;
       }
+      m = match_mask(cs, key);
+      e = empty_mask(cs);
+#pragma unroll
+      for (int q = 0; q < S; ++q)
+        if (m && q == __builtin_ctz(m)) cur = cs[q];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) vic[q] = cs[q];
+    }
+    if (m) {
+      if (update_slot(cb + __builtin_ctz(m), cur, kv)) return R_UPDATED;
+      continue;
     }
     if (hint & ovf_bit(key)) {
       uint32_t b[P::NH];
@@ -500,7 +539,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
 #pragma unroll
       for (int i = 1; i < P::NH; ++i) {
         if (retry || is_dup<P>(b, i)) continue;
-        const int r = locked_update(bucket_ptr(t, b[i]), kSlots, key, kv);
+        const int r = locked_update<S>(bucket_ptr<P>(t, b[i]), S, key, kv);
         if (r > 0) return R_UPDATED;
         if (r < 0) retry = true;
       }
@@ -511,39 +550,41 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       cnt = ld_sync32(t.stash_count + c);
       if (cnt > kStashSlots) cnt = kStashSlots;
       if (cnt) {
-        const int r = locked_update(sb, static_cast<int>(cnt), key, kv);
+        const int r = locked_update<kStashSlots>(sb, static_cast<int>(cnt), key, kv);
         if (r > 0) return R_UPDATED;
         if (r < 0) continue;
       }
     }
-    uint32_t e = empty_mask(cs);
     while (e) {
       const int j = __builtin_ctz(e);
       const unsigned long long seen = cas64(cb + j, 0ull, kv);
       if (seen == 0ull) return R_INSERTED;
-      cs[j] = seen;
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+        if (q == j) vic[q] = seen;
       if (kv_key(seen) == key) { retry = true; break; }
       e &= e - 1;
     }
     if (retry) continue;
+#pragma unroll
     for (int kick = 0; kick < kMaxKicks; ++kick) {
-      const int vs = kick % kSlots;
-      const unsigned long long victim = cs[vs];
-      if (victim == 0ull) continue;
+      const int vs = kick % S;
+      const unsigned long long victim = vic[vs];
+      if (retry || victim == 0ull) continue;
       const uint32_t vkey = kv_key(victim);
       const uint32_t a = alt_bucket<P>(vkey, c, t.nb);
       if (a == c) continue;
-      unsigned long long* ab = bucket_ptr(t, a);
+      unsigned long long* ab = bucket_ptr<P>(t, a);
       // slots of the alternate bucket are only ever claimed by CAS 0 -> kv, so
       // a stale plain view can only overstate its empty slots; the CAS decides
-      unsigned long long as[kSlots];
+      uint32_t ae;
       if (kick == 0 && have_pre && victim == pre_victim && a == pre_alt) {
-#pragma unroll
-        for (int j = 0; j < kSlots; ++j) as[j] = pre[j];
+        ae = pre_empty;
       } else {
+        unsigned long long as[S];
         read_bucket<P>(ab, as);
+        ae = empty_mask(as);
       }
-      uint32_t ae = empty_mask(as);
       if (!ae) continue;
       const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
       // before the victim can leave its H0 (already set when vh0 == c and the
@@ -559,7 +600,6 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       if (d < 0) continue;
       if (finish_move(cb + vs, ab + d, victim, kv)) return R_INSERTED;
       retry = true;
-      break;
     }
     if (retry) continue;
     if (cnt < kStashSlots) {
@@ -579,20 +619,20 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
 template <class P>
 __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
   const uint32_t c = bucket_of<P>(0, key, t.nb);
-  // plain read of H0 in the same round as the first lock attempt: names the
-  // first victim, whose alternate bucket is then fetched with the coherent reload
-  unsigned long long ps[kSlots];
-  read_bucket<P>(bucket_ptr(t, c), ps);
+  // plain read of H0's slot 0 in the same round as the first lock attempt:
+  // names the first victim, whose alternate bucket is then fetched with the
+  // coherent reload
+  const unsigned long long ps0 = read_u64<P>(bucket_ptr<P>(t, c));
   uint8_t result = R_PENDING;
   uint32_t backoff = 1;
   while (result == R_PENDING) {
     if (try_lock(t.locks + c)) {
       uint32_t pa = ~0u;
-      if (ps[0] != 0ull) {
-        pa = alt_bucket<P>(kv_key(ps[0]), c, t.nb);
+      if (ps0 != 0ull) {
+        pa = alt_bucket<P>(kv_key(ps0), c, t.nb);
         if (pa == c) pa = ~0u;
       }
-      result = locked_insert<P>(t, key, val, c, ps[0], pa);
+      result = locked_insert<P>(t, key, val, c, ps0, pa);
       unlock(t.locks + c);
     } else {
       for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
@@ -614,8 +654,8 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
   {
-    unsigned long long s[kSlots];
-    read_bucket<P>(bucket_ptr(t, b[0]), s);
+    unsigned long long s[P::kS];
+    read_bucket<P>(bucket_ptr<P>(t, b[0]), s);
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
@@ -630,8 +670,8 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
-      unsigned long long s[kSlots];
-      read_bucket<P>(bucket_ptr(t, b[i]), s);
+      unsigned long long s[P::kS];
+      read_bucket<P>(bucket_ptr<P>(t, b[i]), s);
       const uint32_t m = match_mask(s, key);
       if (m) {
         val = kv_val(s[__builtin_ctz(m)]);
@@ -673,7 +713,7 @@ __device__ __forceinline__ int erase_slot(unsigned long long* p, unsigned long l
 // lowest matching slot of a bucket: one 64-byte read, then the CAS
 template <class P>
 __device__ __forceinline__ int erase_bucket(unsigned long long* bk, uint32_t key) {
-  unsigned long long s[kSlots];
+  unsigned long long s[P::kS];
   read_bucket<P>(bk, s);
   const uint32_t m = match_mask(s, key);
   if (!m) return 0;
@@ -700,10 +740,10 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
   const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
-  removed_tab += erase_bucket<P>(bucket_ptr(t, b[0]), key);
+  removed_tab += erase_bucket<P>(bucket_ptr<P>(t, b[0]), key);
   if (hint & ovf_bit(key)) {
 #pragma unroll
-    for (int i = 1; i < P::NH; ++i) removed_tab += erase_bucket<P>(bucket_ptr(t, b[i]), key);
+    for (int i = 1; i < P::NH; ++i) removed_tab += erase_bucket<P>(bucket_ptr<P>(t, b[i]), key);
   }
   if (hint & kStashBit) {
     uint32_t cnt = read_u32<P>(t.stash_count + b[0]);

@@ -67,15 +67,26 @@ inline unsigned fill_grid(size_t n) {
   return static_cast<unsigned>(g ? g : 1);
 }
 
-#define HIVE_POLICY_SWITCH(policy, BODY)                                                  \
-  switch (policy) {                                                                       \
-    case HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::PolicyDefault2; BODY; } break;       \
-    case HIVE_POLICY_MURMURCITY: { using P = ::hive::PolicyMurmurCity; BODY; } break;       \
-    case HIVE_POLICY_TRIPLE: { using P = ::hive::PolicyTriple; BODY; } break;               \
-    case HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::PolicyMurmurCityBit; BODY; } break; \
-    case HIVE_POLICY_LOOKUP2: { using P = ::hive::PolicyLookup2; BODY; } break;             \
-    case HIVE_POLICY_LOOKUP3: { using P = ::hive::PolicyLookup3; BODY; } break;             \
-    default: return ::hive::fail(HIVE_EINVAL, "unknown policy");                          \
+// A table's kind: its hash policy and bucket width.  HIVE_POLICY_SWITCH
+// runs BODY with P bound to the matching policy type (kS slots per bucket).
+inline int table_kind(const hive_table* t) { return t->policy + (t->v.S == 16 ? 8 : 0); }
+inline int table_kind(int policy, uint32_t slots_per_bucket) { return policy + (slots_per_bucket == 16 ? 8 : 0); }
+
+#define HIVE_POLICY_SWITCH(kind, BODY)                                                                      \
+  switch (kind) {                                                                                         \
+    case HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::PolicyDefault2; BODY; } break;                         \
+    case HIVE_POLICY_MURMURCITY: { using P = ::hive::PolicyMurmurCity; BODY; } break;                         \
+    case HIVE_POLICY_TRIPLE: { using P = ::hive::PolicyTriple; BODY; } break;                                 \
+    case HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::PolicyMurmurCityBit; BODY; } break;                   \
+    case HIVE_POLICY_LOOKUP2: { using P = ::hive::PolicyLookup2; BODY; } break;                               \
+    case HIVE_POLICY_LOOKUP3: { using P = ::hive::PolicyLookup3; BODY; } break;                               \
+    case 8 + HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::Wide16<::hive::PolicyDefault2>; BODY; } break;     \
+    case 8 + HIVE_POLICY_MURMURCITY: { using P = ::hive::Wide16<::hive::PolicyMurmurCity>; BODY; } break;     \
+    case 8 + HIVE_POLICY_TRIPLE: { using P = ::hive::Wide16<::hive::PolicyTriple>; BODY; } break;             \
+    case 8 + HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::Wide16<::hive::PolicyMurmurCityBit>; BODY; } break; \
+    case 8 + HIVE_POLICY_LOOKUP2: { using P = ::hive::Wide16<::hive::PolicyLookup2>; BODY; } break;           \
+    case 8 + HIVE_POLICY_LOOKUP3: { using P = ::hive::Wide16<::hive::PolicyLookup3>; BODY; } break;           \
+    default: return ::hive::fail(HIVE_EINVAL, "unknown policy");                                            \
   }
 
 template <class Src, class Out>
@@ -85,13 +96,13 @@ hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase
   const uint32_t blk = t->launch[OP_INSERT].block_threads;
   if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
   if (t->launch[OP_INSERT].combine_duplicates) {
-    HIVE_POLICY_SWITCH(t->policy, (k_insert_combined<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
+    HIVE_POLICY_SWITCH(table_kind(t), (k_insert_combined<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                     phase, clk)));
   } else if (t->launch[OP_INSERT].inline_slow_path) {
-    HIVE_POLICY_SWITCH(t->policy, (k_insert<P, Src, Out, true><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
+    HIVE_POLICY_SWITCH(table_kind(t), (k_insert<P, Src, Out, true><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                  phase, clk)));
   } else {
-    HIVE_POLICY_SWITCH(t->policy, (k_insert<P, Src, Out, false><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
+    HIVE_POLICY_SWITCH(table_kind(t), (k_insert<P, Src, Out, false><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                   phase, clk)));
   }
   HIP_CHECK_LAUNCH();
@@ -105,13 +116,13 @@ hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase
   const uint32_t lanes = t->launch[OP_LOOKUP].lanes_per_key > 1 ? t->launch[OP_LOOKUP].lanes_per_key : 1;
   if (!fits_one_launch(n * lanes, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
   if (lanes == 8) {
-    HIVE_POLICY_SWITCH(t->policy, (k_lookup_coop<P, Src, Out, 8><<<grid_for(n * 8, blk), blk, 0, s>>>(t->v, src, n, out,
+    HIVE_POLICY_SWITCH(table_kind(t), (k_lookup_coop<P, Src, Out, 8><<<grid_for(n * 8, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                    phase)));
   } else if (lanes == 4) {
-    HIVE_POLICY_SWITCH(t->policy, (k_lookup_coop<P, Src, Out, 4><<<grid_for(n * 4, blk), blk, 0, s>>>(t->v, src, n, out,
+    HIVE_POLICY_SWITCH(table_kind(t), (k_lookup_coop<P, Src, Out, 4><<<grid_for(n * 4, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                    phase)));
   } else {
-    HIVE_POLICY_SWITCH(t->policy, (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+    HIVE_POLICY_SWITCH(table_kind(t), (k_lookup<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
   }
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
@@ -122,7 +133,7 @@ hive_status_t launch_erase(hive_table* t, Src src, size_t n, Out out, int phase,
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_DELETE].block_threads;
   if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
-  HIVE_POLICY_SWITCH(t->policy, (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
+  HIVE_POLICY_SWITCH(table_kind(t), (k_erase<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out, phase)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }
@@ -260,16 +271,19 @@ inline void free_view(TableView& v) {
   v = TableView{};
 }
 
-inline size_t view_bytes(uint32_t nb) {
-  return static_cast<size_t>(nb) * kSlots * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
+inline size_t view_bytes(uint32_t nb, uint32_t slots_per_bucket) {
+  return static_cast<size_t>(nb) * slots_per_bucket * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
          static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_words() * 8;
 }
 
-inline hive_status_t alloc_view(TableView& v, uint32_t nb) {
+inline bool valid_width(uint64_t slots_per_bucket) { return slots_per_bucket == 8 || slots_per_bucket == 16; }
+
+inline hive_status_t alloc_view(TableView& v, uint32_t nb, uint32_t slots_per_bucket) {
   v = TableView{};
   v.nb = nb;
+  v.S = slots_per_bucket;
   hipError_t e = hipSuccess;
-  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kSlots * 8)) != hipSuccess ||
+  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * slots_per_bucket * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
       (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
@@ -286,7 +300,7 @@ inline hive_status_t alloc_view(TableView& v, uint32_t nb) {
 // zero buckets, stash pointers, hint bits, locks and counters (stash slots at or
 // above a bucket's count are never read, so the stash body is left as is).
 inline hive_status_t clear_view(const TableView& v, hipStream_t s) {
-  const size_t ns = static_cast<size_t>(v.nb) * kSlots;
+  const size_t ns = static_cast<size_t>(v.nb) * v.S;
   k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
   k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
   k_fill_u32<<<fill_grid(meta_words(v.nb)), 256, 0, s>>>(v.meta, meta_words(v.nb), 0u);

@@ -362,8 +362,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
 
 // ------------------------------------------------------ cooperative lookup
 // L lanes per key (L = 4 or 8, a group of consecutive lanes of one
-// wavefront): lane r of a group loads slots [r*8/L, (r+1)*8/L) of each probed
-// 64-byte bucket, and one wavefront ballot gives every group the lowest
+// wavefront): lane r of a group loads slots [r*S/L, (r+1)*S/L) of each probed
+// bucket of S slots, and one wavefront ballot gives every group the lowest
 // matching slot.  The probe sequence and result are those of lookup_one: H0,
 // the other candidates if H0's overflow bit is set, then stash[H0] if its
 // stash bit is set (stash entries below its count, one per lane of an
@@ -375,22 +375,24 @@ __device__ __forceinline__ uint32_t group_bits(bool pred) {
   return static_cast<uint32_t>(b >> base) & ((1u << L) - 1u);
 }
 
-// this lane's part of a probed region: kSlots / L consecutive slots from slot0
-template <int L>
-__device__ __forceinline__ void load_part(const unsigned long long* p, unsigned long long (&v)[kSlots / L]) {
-  if constexpr (L == 8) {
+// this lane's part of a probed region: W consecutive slots from p
+template <int W>
+__device__ __forceinline__ void load_part(const unsigned long long* p, unsigned long long (&v)[W]) {
+  if constexpr (W == 1) {
     v[0] = p[0];
   } else {
-    const uint4 q = *reinterpret_cast<const uint4*>(p);
-    v[0] = (static_cast<unsigned long long>(q.y) << 32) | q.x;
-    v[1] = (static_cast<unsigned long long>(q.w) << 32) | q.z;
+#pragma unroll
+    for (int h = 0; h < W / 2; ++h) {
+      const uint4 q = reinterpret_cast<const uint4*>(p)[h];
+      v[2 * h] = (static_cast<unsigned long long>(q.y) << 32) | q.x;
+      v[2 * h + 1] = (static_cast<unsigned long long>(q.w) << 32) | q.z;
+    }
   }
 }
 
 // lowest slot of the group's region holding key: its value to every lane
-template <int L>
-__device__ __forceinline__ bool group_match(const unsigned long long (&v)[kSlots / L], uint32_t key, uint32_t& val) {
-  constexpr int W = kSlots / L;
+template <int L, int W>
+__device__ __forceinline__ bool group_match(const unsigned long long (&v)[W], uint32_t key, uint32_t& val) {
   int mine = -1;
 #pragma unroll
   for (int j = W - 1; j >= 0; --j)
@@ -409,7 +411,8 @@ __device__ __forceinline__ bool group_match(const unsigned long long (&v)[kSlots
 
 template <class P, class Src, class Out, int L>
 __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src, size_t n, Out out, int phase) {
-  constexpr int W = kSlots / L;
+  constexpr int W = P::kS / L;         // bucket slots per lane
+  constexpr int WS = kStashSlots / L;  // stash slots per lane
   const size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / L;
   const uint32_t r = threadIdx.x & (L - 1);
   if (i >= n) return;  // whole groups: n ops cover whole groups of lanes
@@ -428,8 +431,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
 #pragma unroll
     for (int h = 0; h < P::NH; ++h) b[h] = bucket_of<P>(h, key, t.nb);
     unsigned long long v[W];
-    load_part<L>(bucket_ptr(t, b[0]) + r * W, v);
-    if (group_match<L>(v, key, val)) {
+    load_part<W>(bucket_ptr<P>(t, b[0]) + r * W, v);
+    if (group_match<L, W>(v, key, val)) {
       res = R_FOUND;
     } else {
       const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
@@ -437,18 +440,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
 #pragma unroll
         for (int h = 1; h < P::NH; ++h) {
           if (res == R_FOUND || is_dup<P>(b, h)) continue;
-          load_part<L>(bucket_ptr(t, b[h]) + r * W, v);
-          if (group_match<L>(v, key, val)) res = R_FOUND;
+          load_part<W>(bucket_ptr<P>(t, b[h]) + r * W, v);
+          if (group_match<L, W>(v, key, val)) res = R_FOUND;
         }
       }
       if (res != R_FOUND && (hint & kStashBit)) {
         uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
         if (cnt > kStashSlots) cnt = kStashSlots;
-        load_part<L>(stash_ptr(t, b[0]) + r * W, v);
+        unsigned long long vs[WS];
+        load_part<WS>(stash_ptr(t, b[0]) + r * WS, vs);
 #pragma unroll
-        for (int j = 0; j < W; ++j)
-          if (r * W + j >= cnt) v[j] = 0ull;  // past the count: not an entry
-        if (group_match<L>(v, key, val)) res = R_FOUND;
+        for (int j = 0; j < WS; ++j)
+          if (r * WS + j >= cnt) vs[j] = 0ull;  // past the count: not an entry
+        if (group_match<L, WS>(vs, key, val)) res = R_FOUND;
       }
     }
   }
@@ -472,40 +476,42 @@ __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_
 // ---------------------------------------------------------------- rehash
 // One thread per source entry, inserted into dst with the same fast/slow
 // protocol.  A wavefront takes one slot of a row of 64 consecutive source
-// buckets; region 0 holds the bucket slots, region 1 the stash slots.  The 8
-// keys of one source bucket share H0 in the source and land in the same two
-// destination buckets, where lanes running at the same time would race for
-// the same empty slots.  So a row's 8 slots go to wavefronts kRehashChunk
+// buckets; region 0 holds the SS bucket slots of the source (SS = the
+// source's width, which may differ from dst's), region 1 the 8 stash slots.
+// The keys of one source bucket share H0 in the source and land in the same
+// two destination buckets, where lanes running at the same time would race
+// for the same empty slots.  So a row's slots go to wavefronts kRehashChunk
 // apart (rows are taken in chunks of kRehashChunk, slot by slot: about one
 // chip's worth of resident wavefronts between them), and a chunk's source
-// lines stay cached for its 8 passes.
+// lines stay cached for its passes.
 constexpr size_t kRehashRows = 64;      // source buckets per wavefront
 constexpr size_t kRehashChunk = 8192;   // rows per chunk
-static_assert(kStashSlots == kSlots, "a region covers the bucket or the stash slots");
 __host__ __device__ inline size_t rehash_rows(uint32_t nb) { return (static_cast<size_t>(nb) + kRehashRows - 1) / kRehashRows; }
-// threads of a k_rehash launch over a source of nb buckets
-inline size_t rehash_threads(uint32_t nb) { return 2 * rehash_rows(nb) * kSlots * kRehashRows; }
-template <class P>
+// threads of a k_rehash launch over a source of nb buckets of ss slots
+inline size_t rehash_threads(uint32_t nb, uint32_t ss) {
+  return rehash_rows(nb) * (ss + kStashSlots) * kRehashRows;
+}
+template <class P, int SS>
 __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView src, unsigned long long* failed) {
   __shared__ unsigned long long queue[kMaxBlock];
   __shared__ uint32_t qn;
   if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t rows = rehash_rows(src.nb), per_region = rows * kSlots;
+  const size_t rows = rehash_rows(src.nb), region0 = rows * SS;
   const size_t w = i / kRehashRows;
   unsigned long long kv = 0;
-  if (w < 2 * per_region) {
-    const bool in_stash = w >= per_region;
-    const size_t wr = in_stash ? w - per_region : w;
-    const size_t chunk = wr / (kRehashChunk * kSlots), within = wr % (kRehashChunk * kSlots);
+  if (w < region0 + rows * kStashSlots) {
+    const bool in_stash = w >= region0;
+    const size_t wr = in_stash ? w - region0 : w;
+    const size_t ns = in_stash ? kStashSlots : SS;  // slots per row of this region
+    const size_t chunk = wr / (kRehashChunk * ns), within = wr % (kRehashChunk * ns);
     const size_t crow = chunk * kRehashChunk, nrow = rows - crow < kRehashChunk ? rows - crow : kRehashChunk;
     const uint32_t slot = static_cast<uint32_t>(within / nrow);
     const size_t b = (crow + within % nrow) * kRehashRows + i % kRehashRows;
     if (b < src.nb) {
-      const size_t e = b * kSlots + slot;
-      if (!in_stash) kv = src.slots[e];
-      else if (slot < src.stash_count[b]) kv = src.stash[e];
+      if (!in_stash) kv = src.slots[b * SS + slot];
+      else if (slot < src.stash_count[b]) kv = src.stash[b * kStashSlots + slot];
     }
   }
   int dtab = 0, dstash = 0;
@@ -540,15 +546,16 @@ static __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 
 // exact occupancy (include/hive_kernels.cuh:303-325, counting kv != 0)
 static __global__ void k_count_exact(TableView t, unsigned long long* out) {
-  const size_t ntab = static_cast<size_t>(t.nb) * kSlots;
+  const size_t ntab = static_cast<size_t>(t.nb) * t.S, nst = static_cast<size_t>(t.nb) * kStashSlots;
   unsigned long long ct = 0, cs = 0;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < ntab;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     ct += (t.slots[i] != 0ull);
-    const uint32_t b = static_cast<uint32_t>(i / kStashSlots);
-    const uint32_t j = static_cast<uint32_t>(i % kStashSlots);
-    uint32_t cnt = t.stash_count[b];
-    cs += (j < cnt && t.stash[i] != 0ull);
+    if (i < nst) {
+      const uint32_t b = static_cast<uint32_t>(i / kStashSlots);
+      const uint32_t j = static_cast<uint32_t>(i % kStashSlots);
+      cs += (j < t.stash_count[b] && t.stash[i] != 0ull);
+    }
   }
   int a = static_cast<int>(ct), c = static_cast<int>(cs);
   a = wave_sum(a);
@@ -565,14 +572,14 @@ static __global__ void k_count_exact(TableView t, unsigned long long* out) {
 template <class P>
 __global__ void k_rebuild_meta(TableView t) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= static_cast<size_t>(t.nb) * kSlots) return;
-  const uint32_t b = static_cast<uint32_t>(i / kSlots);
+  if (i >= static_cast<size_t>(t.nb) * P::kS) return;
+  const uint32_t b = static_cast<uint32_t>(i / P::kS);
   const unsigned long long kv = t.slots[i];
   if (kv != 0ull) {
     const uint32_t h0 = bucket_of<P>(0, kv_key(kv), t.nb);
     if (h0 != b) meta_set(t, h0, ovf_bit(kv_key(kv)));
   }
-  if (i % kSlots == 0 && t.stash_count[b] > 0) meta_set(t, b, kStashBit);
+  if (i % P::kS == 0 && t.stash_count[b] > 0) meta_set(t, b, kStashBit);
 }
 
 // drop every stash bit, keep the overflow bits (stash drain)
@@ -607,11 +614,11 @@ static __global__ void k_stash_compact(TableView t, unsigned long long* out, uns
   }
 }
 
-// live bucket entries of slots [0, nb*8), appended at *pos with one atomic
+// live bucket entries of slots [0, nb*S), appended at *pos with one atomic
 // per wavefront
 static __global__ void k_dump_slots(TableView t, unsigned long long* out, unsigned long long* pos) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t n = static_cast<size_t>(t.nb) * kSlots;
+  const size_t n = static_cast<size_t>(t.nb) * t.S;
   const unsigned long long kv = i < n ? t.slots[i] : 0ull;
   const unsigned long long live = __ballot(kv != 0ull);
   if (live == 0ull) return;  // uniform across the wavefront
@@ -647,8 +654,8 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
     const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
     v[0] = 1;
     v[1] = 1;
-    unsigned long long s[kSlots];
-    load_bucket(bucket_ptr(t, b[0]), s);
+    unsigned long long s[P::kS];
+    load_bucket(bucket_ptr<P>(t, b[0]), s);
     bool found = match_mask(s, key) != 0;
     if (!found) v[3] = 1;
 #pragma unroll
@@ -657,7 +664,7 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
       ++v[0];
       if (found) continue;
       ++v[1];
-      load_bucket(bucket_ptr(t, b[j]), s);
+      load_bucket(bucket_ptr<P>(t, b[j]), s);
       found = match_mask(s, key) != 0;
     }
     if (!found && (hint & kStashBit)) v[2] = 1;
@@ -677,17 +684,19 @@ __device__ __forceinline__ uint32_t ceiling_index(size_t i, uint32_t seed) {
   return mix_murmur(static_cast<uint32_t>(i) ^ seed) ^ mix_hash2(static_cast<uint32_t>(i >> 32) + seed);
 }
 
-// n random 64-byte reads (one per lane), same load shape as a bucket probe
+// n random S * 8-byte reads (one per lane), same load shape as a bucket
+// probe (S = 8: 64 bytes, S = 16: a 128-byte line)
+template <int S>
 static __global__ __launch_bounds__(256) void k_ceiling_gather(const unsigned long long* buf, uint32_t nbuckets, size_t n,
                                                         uint32_t seed, uint32_t* sink) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t b = ceiling_index(i, seed) % nbuckets;
-  unsigned long long s[kSlots];
-  load_bucket(buf + static_cast<size_t>(b) * kSlots, s);
+  unsigned long long s[S];
+  load_bucket(buf + static_cast<size_t>(b) * S, s);
   unsigned long long acc = 0;
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) acc ^= s[j];
+  for (int j = 0; j < S; ++j) acc ^= s[j];
   if (static_cast<uint32_t>(acc) == 0x9e3779b9u && sink) sink[blockIdx.x] = static_cast<uint32_t>(acc >> 32);
 }
 

@@ -51,7 +51,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
   auto grow_target = [&](uint64_t live, uint64_t add, uint64_t cap) {
     double want = static_cast<double>(cap) * growth;
     while (static_cast<double>(live + add) > max_load_factor * want) want *= growth;
-    return static_cast<uint64_t>(std::ceil(want / kSlots));  // buckets
+    return static_cast<uint64_t>(std::ceil(want / t->v.S));  // buckets
   };
   auto launch_batch = [&](size_t b, size_t lo, size_t m, hipStream_t q) -> hive_status_t {
     hive_status_t rr = launch_insert(t, SrcNative{keys + lo, vals ? vals + lo : nullptr, nullptr}, m,
@@ -73,7 +73,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       if (static_cast<double>(st.table_live + st.stash_live + m) > max_load_factor * static_cast<double>(st.capacity)) {
         const uint64_t nb = grow_target(st.table_live + st.stash_live, m, st.capacity);
         if (nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "growth beyond 2^32-1 buckets");
-        if ((r = hive_resize(t, nb * kSlots, stream)) != HIVE_OK) return r;
+        if ((r = hive_resize(t, nb * t->v.S, stream)) != HIVE_OK) return r;
         ++rep.resizes;
       }
       if ((r = launch_batch(b, lo, m, s)) != HIVE_OK) return r;
@@ -98,7 +98,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
         if (nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "growth beyond 2^32-1 buckets");
         steps[b].grow = static_cast<int>(target_nb.size());
         target_nb.push_back(static_cast<uint32_t>(nb));
-        cap = nb * kSlots;
+        cap = nb * t->v.S;
       }
       live += steps[b].n;
     }
@@ -112,7 +112,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       }
     } guard{&views};
     for (size_t i = 0; i < views.size(); ++i) {
-      if ((r = alloc_view(views[i], target_nb[i])) != HIVE_OK) return r;
+      if ((r = alloc_view(views[i], target_nb[i], t->v.S)) != HIVE_OK) return r;
       if ((r = clear_view(views[i], s)) != HIVE_OK) return r;
     }
     DevArena ar;
@@ -132,9 +132,9 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       for (size_t b = 0; b < nbatch && rc == HIVE_OK; ++b) {
         if (steps[b].grow >= 0) {
           const TableView nv = views[steps[b].grow];
-          const size_t nsrc = rehash_threads(t->v.nb);
-          HIVE_POLICY_SWITCH(t->policy, (k_rehash<P><<<grid_for(nsrc, kDefaultBlock), kDefaultBlock, 0, cs>>>(
-                                            nv, t->v, d_fail)));
+          const size_t nsrc = rehash_threads(t->v.nb, t->v.S);
+          HIVE_POLICY_SWITCH(table_kind(t), (k_rehash<P, P::kS><<<grid_for(nsrc, kDefaultBlock), kDefaultBlock, 0,
+                                                                  cs>>>(nv, t->v, d_fail)));
           t->v = nv;
         }
         rc = launch_batch(b, steps[b].lo, steps[b].n, cs);
@@ -164,13 +164,13 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       TableView old = orig;
       free_view(old);
       guard.keep_last = true;  // the handle now owns the last target table
-      t->bytes = view_bytes(t->v.nb) + 16;
+      t->bytes = view_bytes(t->v.nb, t->v.S) + 16;
     }
     rep.resizes = static_cast<uint32_t>(views.size());
     rep.rehash_failed = failed;
     if (failed) return fail(HIVE_ECAPACITY, "a rehash inside the sequence could not place every entry");
   }
-  rep.final_slots = static_cast<uint64_t>(t->v.nb) * kSlots;
+  rep.final_slots = static_cast<uint64_t>(t->v.nb) * t->v.S;
   if (report) *report = rep;
   return HIVE_OK;
 }

@@ -138,7 +138,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   hive_serve_report_t rep{};
   rep.requests = nreq;
   if (nreq == 0) {
-    rep.final_slots = static_cast<uint64_t>(t->v.nb) * kSlots;
+    rep.final_slots = static_cast<uint64_t>(t->v.nb) * t->v.S;
     if (report) *report = rep;
     return HIVE_OK;
   }
@@ -270,13 +270,13 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
           // grow in place if free HBM allows it
           double want = static_cast<double>(capacity) * c.growth;
           while (static_cast<double>(table_ub + ins) > c.max_load_factor * want) want *= c.growth;
-          const uint64_t new_nb = static_cast<uint64_t>(std::ceil(want / kSlots));
+          const uint64_t new_nb = static_cast<uint64_t>(std::ceil(want / t->v.S));
           size_t free_b = 0, total_b = 0;
           HIP_TRY(hipMemGetInfo(&free_b, &total_b));
           if (new_nb <= 0xFFFFFFFFull &&
-              free_b >= view_bytes(static_cast<uint32_t>(new_nb)) + c.hbm_reserve_bytes) {
+              free_b >= view_bytes(static_cast<uint32_t>(new_nb), t->v.S) + c.hbm_reserve_bytes) {
             const Clock::time_point tr = Clock::now();
-            if ((r = hive_resize(t, new_nb * kSlots, stream)) != HIVE_OK) return r;
+            if ((r = hive_resize(t, new_nb * t->v.S, stream)) != HIVE_OK) return r;
             if ((r = refresh()) != HIVE_OK) return r;
             rep.resize_ms += std::chrono::duration<double, std::milli>(Clock::now() - tr).count();
             ++rep.resizes;

@@ -360,9 +360,9 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
     if (h_offsets[nbatches] > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "persistent stream: more than 2^32-1 ops");
     const void* kfn = nullptr;
     if (t->stream_inline) {
-      HIVE_POLICY_SWITCH(t->policy, (kfn = reinterpret_cast<const void*>(k_stream<P, true>)));
+      HIVE_POLICY_SWITCH(table_kind(t), (kfn = reinterpret_cast<const void*>(k_stream<P, true>)));
     } else {
-      HIVE_POLICY_SWITCH(t->policy, (kfn = reinterpret_cast<const void*>(k_stream<P, false>)));
+      HIVE_POLICY_SWITCH(table_kind(t), (kfn = reinterpret_cast<const void*>(k_stream<P, false>)));
     }
     int occ = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kStreamBlock, t->stream_inline ? 0 : kMaxTileOps * 4));
@@ -414,9 +414,9 @@ hive_status_t hive_mixed_stream(hive_table_t* t, const uint8_t* types, const uin
                     static_cast<unsigned long long>(kWaitLimitSeconds * 1e3 * khz)};
     k_stamp<<<1, 64, 0, s>>>(ts);
     if (t->stream_inline) {
-      HIVE_POLICY_SWITCH(t->policy, (k_stream<P, true><<<grid, kStreamBlock, 0, s>>>(t->v, args)));
+      HIVE_POLICY_SWITCH(table_kind(t), (k_stream<P, true><<<grid, kStreamBlock, 0, s>>>(t->v, args)));
     } else {
-      HIVE_POLICY_SWITCH(t->policy, (k_stream<P, false><<<grid, kStreamBlock, max_tile * 4, s>>>(t->v, args)));
+      HIVE_POLICY_SWITCH(table_kind(t), (k_stream<P, false><<<grid, kStreamBlock, max_tile * 4, s>>>(t->v, args)));
     }
     HIP_CHECK_LAUNCH();
     unsigned int err = 0;
@@ -440,7 +440,7 @@ namespace {
 
 hive_status_t copy_view(const TableView& dst, const TableView& src, hipStream_t s) {
   const size_t nb = src.nb;
-  HIP_TRY(hipMemcpyAsync(dst.slots, src.slots, nb * kSlots * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dst.slots, src.slots, nb * src.S * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.stash, src.stash, nb * kStashSlots * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.stash_count, src.stash_count, nb * 4, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.meta, src.meta, meta_words(src.nb) * 4, hipMemcpyDeviceToDevice, s));
@@ -467,7 +467,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   if ((r = ensure_partition(t, std::max<uint64_t>(maxn, windows[2]))) != HIVE_OK) return r;
   if ((r = ensure_window(t, windows[2])) != HIVE_OK) return r;
   hive_table tmp = *t;  // same policy and device, scratch copy of the table state
-  if ((r = alloc_view(tmp.v, t->v.nb)) != HIVE_OK) return r;
+  if ((r = alloc_view(tmp.v, t->v.nb, t->v.S)) != HIVE_OK) return r;
   struct ViewGuard {
     TableView* v;
     ~ViewGuard() { free_view(*v); }
@@ -617,7 +617,7 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   DevArena ar;
   if (n_insert) {
     hive_table tmp = *t;  // scratch copy of the table state, same policy and device
-    if ((r = alloc_view(tmp.v, t->v.nb)) != HIVE_OK) return r;
+    if ((r = alloc_view(tmp.v, t->v.nb, t->v.S)) != HIVE_OK) return r;
     struct ViewGuard {
       TableView* v;
       ~ViewGuard() { free_view(*v); }

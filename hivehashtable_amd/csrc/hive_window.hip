@@ -646,13 +646,13 @@ hive_status_t enqueue_window(hive_table* t, const uint8_t* types, const uint32_t
   ca.nts = nts;
   // grid-stride loops over the window's ops
   const unsigned grid = std::max(1u, std::min(ntiles, static_cast<uint32_t>(std::max(ncu, 1)) * 8u));
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_note<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
+  HIVE_POLICY_SWITCH(table_kind(t), (k_chain_note<P><<<grid, kWinBlock, 0, s>>>(t->v, ca)));
   HIP_CHECK_LAUNCH();
   k_chain_resolve<<<grid, kWinBlock, 0, s>>>(ca);
   HIP_CHECK_LAUNCH();
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, false)));
+  HIVE_POLICY_SWITCH(table_kind(t), (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, false)));
   HIP_CHECK_LAUNCH();
-  HIVE_POLICY_SWITCH(t->policy, (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, true)));
+  HIVE_POLICY_SWITCH(table_kind(t), (k_chain_apply<P><<<grid, kWinBlock, 0, s>>>(t->v, ca, true)));
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }

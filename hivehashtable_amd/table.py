@@ -50,16 +50,18 @@ def _ptr(t):
 
 
 class HiveTable:
-    """A device-resident Hive hash table (8-slot buckets, 8-slot stash per bucket).
+    """A device-resident Hive hash table (8- or 16-slot buckets, 8-slot stash per bucket).
 
-    ``num_slots`` is the reference's ``table_size``: num_buckets = num_slots // 8.
+    ``num_slots`` is the reference's ``table_size``: num_buckets =
+    num_slots // slots_per_bucket (8: the reference's bucket; 16: one 128-byte
+    line per bucket).
     """
 
-    def __init__(self, num_slots, policy="Default2Hash"):
+    def __init__(self, num_slots, policy="Default2Hash", slots_per_bucket=8):
         self.L = lib()
         self.policy = POLICIES[policy] if isinstance(policy, str) else int(policy)
         h = C.c_void_p()
-        check(self.L.hive_create(int(num_slots), self.policy, C.byref(h)), "hive_create")
+        check(self.L.hive_create_ex(int(num_slots), self.policy, int(slots_per_bucket), C.byref(h)), "hive_create_ex")
         self.h = h
 
     def close(self):
@@ -76,6 +78,15 @@ class HiveTable:
     @property
     def num_buckets(self):
         return int(self.L.hive_num_buckets(self.h))
+
+    @property
+    def slots_per_bucket(self):
+        return int(self.stats()["slots_per_bucket"])
+
+    def set_slots_per_bucket(self, slots_per_bucket, stream=None):
+        """Rehash into buckets of the given width (8 or 16) at the same slot capacity."""
+        check(self.L.hive_set_slots_per_bucket(self.h, int(slots_per_bucket), _stream_ptr(stream)),
+              "hive_set_slots_per_bucket")
 
     # ---------------------------------------------------------- bulk ops
     def clear(self, stream=None):
@@ -249,7 +260,7 @@ class HiveTable:
     def dump(self, cap=None, stream=None):
         """Every live entry as packed kv (int64 device tensor, unordered)."""
         torch = _torch()
-        cap = 2 * self.num_buckets * 8 if cap is None else int(cap)
+        cap = self.num_buckets * (self.slots_per_bucket + 8) if cap is None else int(cap)
         out = torch.empty(max(cap, 1), dtype=torch.int64, device="cuda")
         n = C.c_size_t(0)
         check(self.L.hive_dump(self.h, _ptr(out), cap, C.byref(n), _stream_ptr(stream)), "hive_dump")
@@ -281,7 +292,7 @@ class HiveTable:
 
     def export_raw(self, stream=None):
         nb = self.num_buckets
-        slots = np.empty(nb * 8, np.uint64)
+        slots = np.empty(nb * self.slots_per_bucket, np.uint64)
         stash = np.empty(nb * 8, np.uint64)
         cnt = np.empty(nb, np.uint32)
         check(self.L.hive_export_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
@@ -293,7 +304,7 @@ class HiveTable:
         stash = np.ascontiguousarray(stash, np.uint64)
         cnt = np.ascontiguousarray(stash_count, np.uint32)
         nb = self.num_buckets
-        if slots.size != nb * 8 or stash.size != nb * 8 or cnt.size != nb:
+        if slots.size != nb * self.slots_per_bucket or stash.size != nb * 8 or cnt.size != nb:
             raise ValueError("raw arrays do not match the table size")
         check(self.L.hive_import_raw(self.h, slots.ctypes.data_as(C.c_void_p), stash.ctypes.data_as(C.c_void_p),
                                      cnt.ctypes.data_as(C.c_void_p), _stream_ptr(stream)), "hive_import_raw")

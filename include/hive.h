@@ -8,8 +8,10 @@
  * interfaces each entry point replaces.
  *
  * Table layout (device memory owned by the handle):
- *   slots[nb * 8]   u64 packed kv = (uint64)value << 32 | key, 0 = empty
- *                   (8 slots x 8 B = one 64-byte bucket, densely packed)
+ *   slots[nb * S]   u64 packed kv = (uint64)value << 32 | key, 0 = empty;
+ *                   S = 8 slots per bucket (one 64-byte bucket, densely
+ *                   packed: the reference's bucket) or S = 16 (a bucket
+ *                   owns its 128-byte line), chosen per table
  *   stash[nb * 8]   u64, one 8-slot overflow bucket per canonical bucket
  *   stash_count[nb] u32 monotonic push pointer of each stash bucket
  *   meta[nb/16]     u32, 2 hint bits per bucket: overflow (a key whose H0 is
@@ -76,8 +78,8 @@ typedef struct hive_table hive_table_t;
 
 typedef struct {
   uint64_t num_buckets;
-  uint64_t slots_per_bucket; /* 8 */
-  uint64_t capacity;         /* num_buckets * 8 */
+  uint64_t slots_per_bucket; /* 8 or 16 */
+  uint64_t capacity;         /* num_buckets * slots_per_bucket */
   uint64_t table_live;       /* entries stored in buckets */
   uint64_t stash_live;       /* entries stored in the stash */
   double load_factor;        /* table_live / capacity, as count_occupancy_kernel (include/hive_kernels.cuh:303-325) */
@@ -112,6 +114,13 @@ typedef struct {
  * (src/hive_dispatch.cu:630-631).  The handle keeps the table resident on the
  * current device until hive_destroy. */
 hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out);
+/* hive_create with a bucket width: slots_per_bucket 8 (the reference's
+ * bucket; what hive_create makes) or 16 (128-byte buckets: one line per
+ * probe, fewer keys leave their canonical bucket).  num_buckets =
+ * num_slots / slots_per_bucket.  Same protocol (include/hive_hash_insert.cuh:
+ * 898-1042 with SLOTS = the width; the cuckoo stage still tries 8 victims,
+ * slots 0..7) and the same 8-slot stash per bucket. */
+hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_bucket, hive_table_t** out);
 hive_status_t hive_destroy(hive_table_t* t);
 hive_status_t hive_clear(hive_table_t* t, void* stream);
 hive_status_t hive_policy_from_name(const char* name, int* policy);
@@ -345,6 +354,12 @@ hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* keys, const 
  * hive_stash_reinsert: drain the stash and insert the drained entries back
  *   (with their values) through the normal insert path.  Synchronizes. */
 hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream);
+/* hive_set_slots_per_bucket: rehash the table into buckets of the given
+ * width (8 or 16) at the same slot capacity, as hive_resize does (no
+ * reference counterpart: the reference's width is fixed at build time,
+ * HIVE_BUCKET_SLOTS).  HIVE_ECAPACITY (table unchanged) if an entry could
+ * not be placed.  Synchronizes. */
+hive_status_t hive_set_slots_per_bucket(hive_table_t* t, uint32_t slots_per_bucket, void* stream);
 /* hive_dump: every live entry (bucket slots, then stash) as packed kv into
  *   out_kv (device, cap entries), in no particular order -- the content
  *   multiset, for checks and persistence (the reference has no export; its
@@ -359,7 +374,7 @@ hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* s
  *   (include/hive_kernels.cuh:303-325, src/hive_dispatch.cu:166-177).
  * hive_count_exact scans the table instead of reading the counters.
  * hive_export_raw / hive_import_raw copy the raw layout to / from host
- *   memory (arrays sized num_buckets*8, num_buckets*8, num_buckets). */
+ *   memory (arrays sized num_buckets*slots_per_bucket, num_buckets*8, num_buckets). */
 hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream);
 hive_status_t hive_count_exact(hive_table_t* t, uint64_t* table_live, uint64_t* stash_live, void* stream);
 /* Probe statistics for a key set against the current table (no reference
@@ -409,6 +424,10 @@ hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t
  * lanes cooperating on each 64-byte bucket -- the lane-per-key against
  * wavefront-per-bucket layout measurement.  For lanes_per_bucket > 1 the
  * sink receives at most one u32 (sink[0]). */
+/* hive_ceiling_gather_bytes: the gather with bucket_bytes-sized reads (64:
+ *   the 8-slot bucket, 128: the 16-slot bucket, one line). */
+hive_status_t hive_ceiling_gather_bytes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
+                                        uint32_t bucket_bytes, uint32_t seed, uint32_t* sink, void* stream);
 hive_status_t hive_ceiling_gather_lanes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
                                         uint32_t lanes_per_bucket, uint32_t seed, uint32_t* sink, void* stream);
 

@@ -233,13 +233,20 @@ static uint32_t kv_key(uint64_t kv) { return (uint32_t)kv; }          /* :303-30
 static uint32_t kv_val(uint64_t kv) { return (uint32_t)(kv >> 32); }  /* :308-311 */
 static uint64_t kv_pack(uint32_t k, uint32_t v) { return ((uint64_t)v << 32) | k; } /* :298-301 */
 
-ho_table_t* ho_create(uint32_t nb, int policy) {
-  if (nb == 0) return NULL;
+ho_table_t* ho_create(uint32_t nb, int policy) { return ho_create_ex(nb, policy, HO_SLOTS); }
+
+/* A table of nb buckets of S slots (8: the reference bucket; 16: the
+ * 128-byte bucket of libhive's wide layout).  The protocol is the same with
+ * SLOTS = S: Stage B takes the lowest empty of S slots, Stage C tries victim
+ * slots kick % S for kick < HO_MAX_KICKS, the stash keeps 8 slots. */
+ho_table_t* ho_create_ex(uint32_t nb, int policy, uint32_t S) {
+  if (nb == 0 || (S != 8 && S != 16)) return NULL;
   ho_table_t* t = (ho_table_t*)calloc(1, sizeof(ho_table_t));
   if (!t) return NULL;
   t->nb = nb;
+  t->S = S;
   t->policy = policy;
-  t->slots = (uint64_t*)calloc((size_t)nb * HO_SLOTS, sizeof(uint64_t));
+  t->slots = (uint64_t*)calloc((size_t)nb * t->S, sizeof(uint64_t));
   t->stash = (uint64_t*)calloc((size_t)nb * HO_STASH_SLOTS, sizeof(uint64_t));
   t->stash_count = (uint32_t*)calloc(nb, sizeof(uint32_t));
   if (!t->slots || !t->stash || !t->stash_count) {
@@ -260,19 +267,19 @@ void ho_destroy(ho_table_t* t) {
 /* src/hive_memory.cu:203-218 resets buckets and masks; the stash is reset too
  * here (the reference leaves it, see SURVEY F-notes on hiveHashTableClear). */
 void ho_clear(ho_table_t* t) {
-  memset(t->slots, 0, (size_t)t->nb * HO_SLOTS * sizeof(uint64_t));
+  memset(t->slots, 0, (size_t)t->nb * t->S * sizeof(uint64_t));
   memset(t->stash, 0, (size_t)t->nb * HO_STASH_SLOTS * sizeof(uint64_t));
   memset(t->stash_count, 0, (size_t)t->nb * sizeof(uint32_t));
 }
 
 void ho_import(ho_table_t* t, const uint64_t* slots, const uint64_t* stash, const uint32_t* stash_count) {
-  memcpy(t->slots, slots, (size_t)t->nb * HO_SLOTS * sizeof(uint64_t));
+  memcpy(t->slots, slots, (size_t)t->nb * t->S * sizeof(uint64_t));
   memcpy(t->stash, stash, (size_t)t->nb * HO_STASH_SLOTS * sizeof(uint64_t));
   memcpy(t->stash_count, stash_count, (size_t)t->nb * sizeof(uint32_t));
 }
 
 void ho_export(const ho_table_t* t, uint64_t* slots, uint64_t* stash, uint32_t* stash_count) {
-  memcpy(slots, t->slots, (size_t)t->nb * HO_SLOTS * sizeof(uint64_t));
+  memcpy(slots, t->slots, (size_t)t->nb * t->S * sizeof(uint64_t));
   memcpy(stash, t->stash, (size_t)t->nb * HO_STASH_SLOTS * sizeof(uint64_t));
   memcpy(stash_count, t->stash_count, (size_t)t->nb * sizeof(uint32_t));
 }
@@ -284,8 +291,8 @@ static uint32_t stash_limit(const ho_table_t* t, uint32_t b) {
 
 /* lowest slot of bucket b holding key, or -1.  include/hive_hash_insert.cuh:96-113
  * (lowest lane, then x before y == lowest slot index). */
-static int find_in_bucket(const uint64_t* bucket, uint32_t key) {
-  for (int s = 0; s < HO_SLOTS; ++s)
+static int find_in_bucket(const uint64_t* bucket, uint32_t nslots, uint32_t key) {
+  for (int s = 0; s < (int)nslots; ++s)
     if (kv_key(bucket[s]) == key && bucket[s] != 0) return s;
   return -1;
 }
@@ -298,8 +305,8 @@ static int find_in_stash(const ho_table_t* t, uint32_t b, uint32_t key) {
   return -1;
 }
 
-static int lowest_empty(const uint64_t* bucket) {
-  for (int s = 0; s < HO_SLOTS; ++s)
+static int lowest_empty(const uint64_t* bucket, uint32_t nslots) {
+  for (int s = 0; s < (int)nslots; ++s)
     if (bucket[s] == 0) return s;
   return -1;
 }
@@ -313,8 +320,8 @@ int ho_insert(ho_table_t* t, uint32_t key, uint32_t value) {
   /* Stage A: every candidate bucket, in policy order */
   for (int i = 0; i < nh; ++i) {
     uint32_t b = ho_bucket(t->policy, i, key, t->nb);
-    uint64_t* bk = t->slots + (size_t)b * HO_SLOTS;
-    int s = find_in_bucket(bk, key);
+    uint64_t* bk = t->slots + (size_t)b * t->S;
+    int s = find_in_bucket(bk, t->S, key);
     if (s >= 0) {
       bk[s] = kv;
       return HO_R_UPDATED;
@@ -329,10 +336,10 @@ int ho_insert(ho_table_t* t, uint32_t key, uint32_t value) {
       return HO_R_UPDATED;
     }
   }
-  uint64_t* cb = t->slots + (size_t)c * HO_SLOTS;
+  uint64_t* cb = t->slots + (size_t)c * t->S;
   /* Stage B: lowest empty slot of the canonical bucket */
   {
-    int s = lowest_empty(cb);
+    int s = lowest_empty(cb, t->S);
     if (s >= 0) {
       cb[s] = kv;
       return HO_R_INSERTED;
@@ -341,13 +348,13 @@ int ho_insert(ho_table_t* t, uint32_t key, uint32_t value) {
   /* Stage C: kick k tries victim slot k % SLOTS; the victim moves to the
    * lowest free slot of its alternate bucket. */
   for (int kick = 0; kick < HO_MAX_KICKS; ++kick) {
-    int vs = kick % HO_SLOTS;
+    int vs = kick % t->S;
     uint64_t victim = cb[vs];
     if (victim == 0) continue;
     uint32_t a = ho_alt_bucket(t->policy, kv_key(victim), c, t->nb);
     if (a == c) continue;
-    uint64_t* ab = t->slots + (size_t)a * HO_SLOTS;
-    int d = lowest_empty(ab);
+    uint64_t* ab = t->slots + (size_t)a * t->S;
+    int d = lowest_empty(ab, t->S);
     if (d < 0) continue;
     ab[d] = victim;
     cb[vs] = kv;
@@ -369,8 +376,8 @@ int ho_lookup(const ho_table_t* t, uint32_t key, uint32_t* value) {
   const int nh = ho_num_hashes(t->policy);
   for (int i = 0; i < nh; ++i) {
     uint32_t b = ho_bucket(t->policy, i, key, t->nb);
-    const uint64_t* bk = t->slots + (size_t)b * HO_SLOTS;
-    int s = find_in_bucket(bk, key);
+    const uint64_t* bk = t->slots + (size_t)b * t->S;
+    int s = find_in_bucket(bk, t->S, key);
     if (s >= 0) {
       if (value) *value = kv_val(bk[s]);
       return HO_R_FOUND;
@@ -393,8 +400,8 @@ int ho_erase(ho_table_t* t, uint32_t key) {
   int removed = 0;
   for (int i = 0; i < nh; ++i) {
     uint32_t b = ho_bucket(t->policy, i, key, t->nb);
-    uint64_t* bk = t->slots + (size_t)b * HO_SLOTS;
-    int s = find_in_bucket(bk, key);
+    uint64_t* bk = t->slots + (size_t)b * t->S;
+    int s = find_in_bucket(bk, t->S, key);
     if (s >= 0) {
       bk[s] = 0;
       removed++;
@@ -486,9 +493,9 @@ void ho_mixed_sequential(ho_table_t* t, const uint8_t* types, const uint32_t* ke
  * multiset, every key found with its value.  Returns 0, or -1 if an entry
  * could not be placed (the table is then left unchanged). */
 int ho_resize(ho_table_t* t, uint32_t new_nb) {
-  ho_table_t* n = ho_create(new_nb, t->policy);
+  ho_table_t* n = ho_create_ex(new_nb, t->policy, t->S);
   if (!n) return -1;
-  for (size_t i = 0; i < (size_t)t->nb * HO_SLOTS; ++i) {
+  for (size_t i = 0; i < (size_t)t->nb * t->S; ++i) {
     uint64_t kv = t->slots[i];
     if (kv == 0) continue;
     int r = ho_insert(n, kv_key(kv), kv_val(kv));
@@ -535,7 +542,7 @@ size_t ho_drain_stash(ho_table_t* t, uint64_t* out, size_t cap) {
 
 size_t ho_dump(const ho_table_t* t, uint64_t* out, size_t cap) {
   size_t k = 0;
-  for (size_t i = 0; i < (size_t)t->nb * HO_SLOTS; ++i)
+  for (size_t i = 0; i < (size_t)t->nb * t->S; ++i)
     if (t->slots[i] != 0) {
       if (out && k < cap) out[k] = t->slots[i];
       k++;
@@ -584,8 +591,8 @@ int ho_check_invariants(const ho_table_t* t, char* msg, size_t msglen) {
   size_t k = 0;
   int bad = 0;
   for (uint32_t b = 0; b < t->nb && !bad; ++b) {
-    for (int s = 0; s < HO_SLOTS; ++s) {
-      uint64_t kv = t->slots[(size_t)b * HO_SLOTS + s];
+    for (int s = 0; s < (int)t->S; ++s) {
+      uint64_t kv = t->slots[(size_t)b * t->S + s];
       if (kv == 0) continue;
       uint32_t key = kv_key(kv);
       if (key == 0) { bad = 1; if (msg) snprintf(msg, msglen, "bucket %u slot %d: key 0 with value", b, s); break; }
@@ -632,7 +639,7 @@ void ho_lookup_probe_count(const ho_table_t* t, const uint32_t* keys, size_t n,
       for (int j2 = 0; j2 < i; ++j2) dup |= (seen[j2] == b);
       if (dup) continue; /* a bucket equal to an earlier candidate is not re-read */
       br++;
-      if (find_in_bucket(t->slots + (size_t)b * HO_SLOTS, key) >= 0) found = 1;
+      if (find_in_bucket(t->slots + (size_t)b * t->S, t->S, key) >= 0) found = 1;
     }
     if (!found) {
       uint32_t c = ho_bucket(t->policy, 0, key, t->nb);

@@ -6,7 +6,7 @@
  *
  * Table state uses exactly the device layout of libhive.so, so a raw device
  * export can be loaded here and queried:
- *   slots[nb * 8]       packed kv = (uint64)value << 32 | key, 0 = empty
+ *   slots[nb * S]       packed kv = (uint64)value << 32 | key, 0 = empty (S = 8 or 16)
  *   stash[nb * 8]       overflow stash, one 8-slot bucket per canonical bucket
  *   stash_count[nb]     monotonic push pointer of each stash bucket
  *
@@ -48,6 +48,7 @@ enum { HO_OP_INSERT = 0, HO_OP_LOOKUP = 1, HO_OP_DELETE = 2, HO_OP_ATOMIC_INC = 
 
 typedef struct {
   uint32_t nb;
+  uint32_t S; /* slots per bucket: 8 (HO_SLOTS, the reference) or 16 */
   int policy;
   uint64_t* slots;
   uint64_t* stash;
@@ -75,6 +76,7 @@ uint32_t ho_bucket(int policy, int i, uint32_t key, uint32_t nb);
 uint32_t ho_alt_bucket(int policy, uint32_t key, uint32_t cur, uint32_t nb);
 
 ho_table_t* ho_create(uint32_t nb, int policy);
+ho_table_t* ho_create_ex(uint32_t nb, int policy, uint32_t slots_per_bucket);
 void ho_destroy(ho_table_t* t);
 void ho_clear(ho_table_t* t);
 void ho_import(ho_table_t* t, const uint64_t* slots, const uint64_t* stash, const uint32_t* stash_count);

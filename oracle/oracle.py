@@ -64,6 +64,8 @@ def lib():
     L.ho_alt_bucket.argtypes = [C.c_int, u32, u32, u32]
     L.ho_create.restype = p
     L.ho_create.argtypes = [u32, C.c_int]
+    L.ho_create_ex.restype = p
+    L.ho_create_ex.argtypes = [u32, C.c_int, u32]
     L.ho_destroy.argtypes = [p]
     L.ho_clear.argtypes = [p]
     L.ho_import.argtypes = [p, p, p, p]
@@ -142,11 +144,15 @@ def bucket(policy, i, key, nb):
 
 
 class RawState:
-    """The device layout of one table, as numpy arrays."""
+    """The device layout of one table, as numpy arrays (S slots per bucket)."""
 
-    def __init__(self, nb, slots=None, stash=None, stash_count=None):
+    def __init__(self, nb, slots=None, stash=None, stash_count=None, slots_per_bucket=None):
         self.nb = nb
-        self.slots = np.zeros(nb * SLOTS, np.uint64) if slots is None else np.ascontiguousarray(slots, np.uint64)
+        if slots_per_bucket is None:
+            slots_per_bucket = SLOTS if slots is None else len(slots) // nb
+        self.S = slots_per_bucket
+        self.slots = (np.zeros(nb * self.S, np.uint64) if slots is None
+                      else np.ascontiguousarray(slots, np.uint64))
         self.stash = np.zeros(nb * STASH_SLOTS, np.uint64) if stash is None else np.ascontiguousarray(stash, np.uint64)
         self.stash_count = (np.zeros(nb, np.uint32) if stash_count is None
                             else np.ascontiguousarray(stash_count, np.uint32))
@@ -162,13 +168,14 @@ class RawState:
 
 
 class Oracle:
-    """Sequential single-operation model (hive_oracle.c)."""
+    """Sequential single-operation model (hive_oracle.c), S slots per bucket."""
 
-    def __init__(self, nb, policy="Default2Hash"):
+    def __init__(self, nb, policy="Default2Hash", slots_per_bucket=SLOTS):
         self.L = lib()
         self.nb = nb
+        self.S = slots_per_bucket
         self.policy = policy_id(policy)
-        self.h = self.L.ho_create(nb, self.policy)
+        self.h = self.L.ho_create_ex(nb, self.policy, self.S)
         if not self.h:
             raise MemoryError("ho_create failed")
 
@@ -239,15 +246,16 @@ class Oracle:
         return r, buf.value.decode()
 
     def export(self):
-        st = RawState(self.nb)
+        st = RawState(self.nb, slots_per_bucket=self.S)
         self.L.ho_export(self.h, _ptr(st.slots), _ptr(st.stash), _ptr(st.stash_count))
         return st
 
     def load(self, raw):
-        if raw.nb != self.nb:
+        if raw.nb != self.nb or raw.S != self.S:
             self.L.ho_destroy(self.h)
-            self.h = self.L.ho_create(raw.nb, self.policy)
+            self.h = self.L.ho_create_ex(raw.nb, self.policy, raw.S)
             self.nb = raw.nb
+            self.S = raw.S
         self.L.ho_import(self.h, _ptr(raw.slots), _ptr(raw.stash), _ptr(raw.stash_count))
 
     def probe_count(self, keys):
@@ -308,6 +316,6 @@ class CpuTable:
         return out, st, secs
 
     def export(self):
-        st = RawState(self.nb)
+        st = RawState(self.nb, slots_per_bucket=SLOTS)  # the CPU path keeps the reference's 8-slot bucket
         self.L.hcpu_export(self.h, _ptr(st.slots), _ptr(st.stash), _ptr(st.stash_count))
         return st

@@ -420,3 +420,88 @@ def test_resize_and_drain(O):
     assert o.resize(8) == -1 and o.nb == 2 * nb  # cannot shrink below content; unchanged
     n_st = o.count_stash()
     assert o.drain_stash() == n_st and o.count_stash() == 0
+
+
+# ------------------------------------------- 16-slot buckets (the wide layout)
+# The same protocol with SLOTS = 16: Stage B takes the lowest empty of 16
+# slots; Stage C's kick k still evicts slot k % 16 for k < 8 (MAX_EVICT = 8);
+# the stash keeps 8 slots per canonical bucket.
+def test_wide_lowest_slot_fills_sixteen_before_kicking(O):
+    nb = 64
+    c = 5
+    ks = keys_with_h0("Default2Hash", nb, c, 17, h1_filter=lambda h1: h1 != c)
+    o = O.Oracle(nb, slots_per_bucket=16)
+    assert (o.insert(ks[:16]) == O.R_INSERTED).all()
+    raw = o.export()
+    assert raw.slots.size == nb * 16
+    assert raw.slots.reshape(nb, 16)[c].tolist() == [(k + 1) << 32 | k for k in ks[:16]]
+    # the 17th key: kick 0 moves slot 0 to its alternate, the new key takes slot 0
+    assert o.insert([ks[16]])[0] == O.R_INSERTED
+    s = o.export().slots.reshape(nb, 16)
+    _, alt = hashref.buckets("Default2Hash", np.array([ks[0]], np.uint32), nb)
+    assert s[c, 0] & 0xFFFFFFFF == ks[16]
+    assert ks[0] in (s[int(alt[0])] & 0xFFFFFFFF).tolist()
+    assert (s[c, 1:] & 0xFFFFFFFF).tolist() == ks[1:16]
+
+
+def test_wide_kicks_try_only_slots_zero_to_seven(O):
+    # slots 0..7 hold keys whose H1 == H0 (no kick can move them); slots 8..15
+    # hold movable keys, which MAX_EVICT = 8 kicks never reach: the 17th key
+    # goes to the stash
+    nb = 32
+    c = 3
+    fixed = keys_with_h0("Default2Hash", nb, c, 8, h1_filter=lambda h1: h1 == c)
+    movable = keys_with_h0("Default2Hash", nb, c, 9, start=fixed[-1] + 1, h1_filter=lambda h1: h1 != c)
+    o = O.Oracle(nb, slots_per_bucket=16)
+    assert (o.insert(fixed + movable[:8]) == O.R_INSERTED).all()
+    assert o.insert([movable[8]])[0] == O.R_STASHED
+    raw = o.export()
+    assert raw.stash_count[c] == 1 and raw.stash[c * 8] & 0xFFFFFFFF == movable[8]
+    assert (raw.slots.reshape(nb, 16)[c] & 0xFFFFFFFF).tolist() == fixed + movable[:8]
+
+
+def test_wide_map_model_brute_force(O):
+    rng = np.random.default_rng(11)
+    for policy in ("Default2Hash", "MurmurCityHash", "TripleHash"):
+        o = O.Oracle(64, policy, slots_per_bucket=16)
+        model = {}
+        universe = rng.integers(1, 5000, size=600).astype(np.uint32)
+        for step in range(3000):
+            k = int(universe[rng.integers(0, universe.size)])
+            op = rng.integers(0, 3)
+            if op == 0:
+                v = int(rng.integers(0, 2**32))
+                r = o.insert([k], [v])[0]
+                assert r != O.R_FAILED
+                if k in model:
+                    assert r == O.R_UPDATED
+                model[k] = v
+            elif op == 1:
+                vals, st = o.lookup([k])
+                assert (st[0] == O.R_FOUND and vals[0] == model[k]) if k in model else st[0] == O.R_NOT_FOUND
+            else:
+                r = o.erase([k])[0]
+                assert r == (O.R_ERASED if k in model else O.R_NOT_FOUND)
+                model.pop(k, None)
+        r, msg = o.check_invariants()
+        assert r == 0, msg
+        assert o.count_live() == len(model)
+
+
+def test_wide_and_reference_widths_hold_the_same_content(O):
+    # the width changes placement, never the content: same key set, values,
+    # lookups, and fewer keys outside their canonical bucket at 16 slots
+    keys = np.arange(1, 30001, dtype=np.uint32) * np.uint32(2654435761)
+    vals = keys ^ np.uint32(0x5A5A5A5A)
+    slots = 1 << 15
+    o8 = O.Oracle(slots // 8)
+    o16 = O.Oracle(slots // 16, slots_per_bucket=16)
+    s8, s16 = o8.insert(keys, vals), o16.insert(keys, vals)
+    assert not (s8 == O.R_FAILED).any() and not (s16 == O.R_FAILED).any()
+    assert np.array_equal(o8.dump(), o16.dump())
+    lk = np.concatenate([keys[::3], keys[::7] + np.uint32(1)])
+    assert all(np.array_equal(a, b) for a, b in zip(o8.lookup(lk), o16.lookup(lk)))
+    away8 = o8.probe_count(keys)[0] - keys.size
+    away16 = o16.probe_count(keys)[0] - keys.size
+    assert away16 < away8
+    assert o16.resize(slots // 16 * 2) == 0 and np.array_equal(o16.dump(), o8.dump())
