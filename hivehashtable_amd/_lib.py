@@ -17,6 +17,8 @@ R_FAILED, R_INSERTED, R_UPDATED, R_STASHED, R_FOUND, R_NOT_FOUND, R_ERASED, R_IN
 OP_INSERT, OP_LOOKUP, OP_DELETE, OP_ATOMIC_INC, OP_NONE = range(5)
 STREAM_LAUNCHES, STREAM_GRAPH, STREAM_PERSISTENT, STREAM_WINDOWED = range(4)
 SEQUENCE_PLAIN, SEQUENCE_GRAPH = range(2)
+LAYOUTS = {"bucket8": 0, "bucket16": 1, "inline8": 2}  # hive_layout_t
+
 POLICIES = {
     "Default2Hash": 0,
     "MurmurCityHash": 1,
@@ -44,6 +46,7 @@ class Stats(C.Structure):
         ("device_bytes", C.c_uint64),
         ("policy", C.c_int),
         ("slow_path_ops", C.c_uint64),
+        ("layout", C.c_int),
     ]
 
 
@@ -111,8 +114,8 @@ def lib():
     st, p, sz, u32, u64, i32 = C.c_int, C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint64, C.c_int
     proto = {
         "hive_create": [u64, i32, C.POINTER(p)],
-        "hive_create_ex": [u64, i32, u32, C.POINTER(p)],
-        "hive_set_slots_per_bucket": [p, u32, p],
+        "hive_create_ex": [u64, i32, i32, C.POINTER(p)],
+        "hive_set_layout": [p, i32, p],
         "hive_ceiling_gather_bytes": [p, u64, u64, u32, u32, p, p],
         "hive_destroy": [p],
         "hive_clear": [p, p],

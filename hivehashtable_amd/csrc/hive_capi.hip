@@ -41,7 +41,7 @@ hive_status_t read_counters(hive_table* t, hipStream_t s, uint64_t* tab, uint64_
 hive_status_t count_exact_view(hive_table* t, const TableView& v, hipStream_t s, uint64_t* tab, uint64_t* st) {
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
   const size_t ns = static_cast<size_t>(v.nb) * v.S;
-  k_count_exact<<<fill_grid(ns), 256, 0, s>>>(v, t->scratch);
+  HIVE_POLICY_SWITCH(table_kind(t->policy, layout_of(v)), (k_count_exact<P><<<fill_grid(ns), 256, 0, s>>>(v, t->scratch)));
   HIP_CHECK_LAUNCH();
   unsigned long long h[2];
   HIP_TRY(hipMemcpyAsync(h, t->scratch, 16, hipMemcpyDeviceToHost, s));
@@ -98,28 +98,36 @@ hive_status_t noalloc_setup(int dev) {
 // Rehash every live entry (buckets, then stash) into a new table of
 // num_slots / width buckets of `width` slots, with the insert protocol; swap
 // it in, or keep the old table on HIVE_ECAPACITY.
-hive_status_t relayout(hive_table* t, uint64_t num_slots, uint32_t width, hipStream_t s) {
+hive_status_t relayout(hive_table* t, uint64_t num_slots, int layout, hipStream_t s) {
   hive_status_t r;
   if ((r = set_device(t)) != HIVE_OK) return r;
-  const uint64_t nb = num_slots / width;
+  const uint64_t nb = num_slots / layout_slots(layout);
   if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "new size must give 1 .. 2^32-1 buckets");
   TableView nv;
-  if ((r = alloc_view(nv, static_cast<uint32_t>(nb), width)) != HIVE_OK) return r;
+  if ((r = alloc_view(nv, static_cast<uint32_t>(nb), layout)) != HIVE_OK) return r;
   if ((r = clear_view(nv, s)) != HIVE_OK) {
     free_view(nv);
     return r;
   }
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
-  const size_t nsrc = rehash_threads(t->v.nb, t->v.S);
-  const uint32_t blk = kDefaultBlock;
-  const bool src16 = t->v.S == 16;
-  HIVE_POLICY_SWITCH(table_kind(t->policy, width), (src16 ? k_rehash<P, 16><<<grid_for(nsrc, blk), blk, 0, s>>>(
-                                                                nv, t->v, t->scratch)
-                                                          : k_rehash<P, 8><<<grid_for(nsrc, blk), blk, 0, s>>>(
-                                                                nv, t->v, t->scratch)));
-  if (hipGetLastError() != hipSuccess) {
+  hive_status_t rr = HIVE_OK;
+  switch (table_kind(t->policy, layout)) {
+#define HIVE_RELAYOUT_CASE(K, T) \
+    case K: rr = launch_rehash<T>(nv, t->v, t->scratch, s); break;
+    HIVE_RELAYOUT_CASE(0, PolicyDefault2) HIVE_RELAYOUT_CASE(1, PolicyMurmurCity) HIVE_RELAYOUT_CASE(2, PolicyTriple)
+    HIVE_RELAYOUT_CASE(3, PolicyMurmurCityBit) HIVE_RELAYOUT_CASE(4, PolicyLookup2) HIVE_RELAYOUT_CASE(5, PolicyLookup3)
+    HIVE_RELAYOUT_CASE(8, Wide16<PolicyDefault2>) HIVE_RELAYOUT_CASE(9, Wide16<PolicyMurmurCity>)
+    HIVE_RELAYOUT_CASE(10, Wide16<PolicyTriple>) HIVE_RELAYOUT_CASE(11, Wide16<PolicyMurmurCityBit>)
+    HIVE_RELAYOUT_CASE(12, Wide16<PolicyLookup2>) HIVE_RELAYOUT_CASE(13, Wide16<PolicyLookup3>)
+    HIVE_RELAYOUT_CASE(16, Inline8<PolicyDefault2>) HIVE_RELAYOUT_CASE(17, Inline8<PolicyMurmurCity>)
+    HIVE_RELAYOUT_CASE(18, Inline8<PolicyTriple>) HIVE_RELAYOUT_CASE(19, Inline8<PolicyMurmurCityBit>)
+    HIVE_RELAYOUT_CASE(20, Inline8<PolicyLookup2>) HIVE_RELAYOUT_CASE(21, Inline8<PolicyLookup3>)
+#undef HIVE_RELAYOUT_CASE
+    default: rr = fail(HIVE_EINVAL, "unknown policy or layout");
+  }
+  if (rr != HIVE_OK) {
     free_view(nv);
-    return fail(HIVE_EDEVICE, "rehash launch");
+    return rr;
   }
   unsigned long long failed = 0;
   if (hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -133,7 +141,7 @@ hive_status_t relayout(hive_table* t, uint64_t num_slots, uint32_t width, hipStr
   }
   free_view(t->v);
   t->v = nv;
-  t->bytes = view_bytes(static_cast<uint32_t>(nb), width) + 16;
+  t->bytes = view_bytes(static_cast<uint32_t>(nb), layout) + 16;
   return HIVE_OK;
 }
 
@@ -168,15 +176,15 @@ hive_status_t hive_policy_from_name(const char* name, int* policy) {
 }
 
 hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out) {
-  return hive_create_ex(num_slots, policy, kRefSlots, out);
+  return hive_create_ex(num_slots, policy, HIVE_LAYOUT_BUCKET8, out);
 }
 
-hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_bucket, hive_table_t** out) {
+hive_status_t hive_create_ex(uint64_t num_slots, int policy, int layout, hive_table_t** out) {
   if (!out) return fail(HIVE_EINVAL, "null out");
   *out = nullptr;
   if (policy < 0 || policy > HIVE_POLICY_LOOKUP3) return fail(HIVE_EINVAL, "unknown policy");
-  if (!valid_width(slots_per_bucket)) return fail(HIVE_EINVAL, "slots_per_bucket must be 8 or 16");
-  const uint64_t nb = num_slots / slots_per_bucket;
+  if (!valid_layout(layout)) return fail(HIVE_EINVAL, "unknown layout");
+  const uint64_t nb = num_slots / layout_slots(layout);
   if (nb == 0 || nb > 0xFFFFFFFFull) return fail(HIVE_EINVAL, "num_slots must give 1 .. 2^32-1 buckets");
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -188,7 +196,7 @@ hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_
   }
   t->device = dev;
   t->policy = policy;
-  hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb), slots_per_bucket);
+  hive_status_t r = alloc_view(t->v, static_cast<uint32_t>(nb), layout);
   if (r != HIVE_OK) {
     delete t;
     return r;
@@ -198,7 +206,7 @@ hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_
     delete t;
     return fail(HIVE_ENOMEM, "hipMalloc scratch");
   }
-  t->bytes = view_bytes(static_cast<uint32_t>(nb), slots_per_bucket) + 16;
+  t->bytes = view_bytes(static_cast<uint32_t>(nb), layout) + 16;
   if ((r = clear_view(t->v, nullptr)) != HIVE_OK || hipStreamSynchronize(nullptr) != hipSuccess) {
     free_view(t->v);
     (void)hipFree(t->scratch);
@@ -311,15 +319,15 @@ hive_status_t hive_mixed_ops(hive_table_t* t, const hive_operation_t* ops, size_
 hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK) return r;
-  return relayout(t, new_num_slots, t->v.S, static_cast<hipStream_t>(stream));
+  return relayout(t, new_num_slots, layout_of(t->v), static_cast<hipStream_t>(stream));
 }
 
-hive_status_t hive_set_slots_per_bucket(hive_table_t* t, uint32_t slots_per_bucket, void* stream) {
+hive_status_t hive_set_layout(hive_table_t* t, int layout, void* stream) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK) return r;
-  if (!valid_width(slots_per_bucket)) return fail(HIVE_EINVAL, "slots_per_bucket must be 8 or 16");
-  if (slots_per_bucket == t->v.S) return HIVE_OK;
-  return relayout(t, static_cast<uint64_t>(t->v.nb) * t->v.S, slots_per_bucket, static_cast<hipStream_t>(stream));
+  if (!valid_layout(layout)) return fail(HIVE_EINVAL, "unknown layout");
+  if (layout == layout_of(t->v)) return HIVE_OK;
+  return relayout(t, static_cast<uint64_t>(t->v.nb) * t->v.S, layout, static_cast<hipStream_t>(stream));
 }
 
 hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n_out, void* stream) {
@@ -327,7 +335,7 @@ hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, si
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 16, s));
-  k_stash_count_live<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, t->scratch);
+  HIVE_POLICY_SWITCH(table_kind(t), (k_stash_count_live<P><<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, t->scratch)));
   HIP_CHECK_LAUNCH();
   unsigned long long total = 0;
   HIP_TRY(hipMemcpyAsync(&total, t->scratch, 8, hipMemcpyDeviceToHost, s));
@@ -337,12 +345,14 @@ hive_status_t hive_stash_drain(hive_table_t* t, uint64_t* out_kv, size_t cap, si
   if (total) {
     if (!out_kv) return fail(HIVE_EINVAL, "null out_kv");
     HIP_TRY(hipMemsetAsync(t->scratch, 0, 8, s));
-    k_stash_compact<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, reinterpret_cast<unsigned long long*>(out_kv),
-                                                            t->scratch);
+    auto* o = reinterpret_cast<unsigned long long*>(out_kv);
+    HIVE_POLICY_SWITCH(table_kind(t), (k_stash_compact<P><<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, o, t->scratch)));
     HIP_CHECK_LAUNCH();
   }
-  k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.stash_count, t->v.nb, 0u);
-  k_clear_stash_bits<<<grid_for(meta_words(t->v.nb), 256), 256, 0, s>>>(t->v);
+  {
+    const size_t nr = t->v.layout == kLayoutInline ? t->v.nb : meta_words(t->v.nb);
+    HIVE_POLICY_SWITCH(table_kind(t), (k_reset_stash<P><<<grid_for(nr, 256), 256, 0, s>>>(t->v)));
+  }
   HIP_CHECK_LAUNCH();
   uint64_t tab = 0, st = 0;
   if ((r = read_counters(t, s, &tab, &st)) != HIVE_OK) return r;
@@ -362,8 +372,9 @@ hive_status_t hive_dump(hive_table_t* t, uint64_t* out_kv, size_t cap, size_t* n
   if (!out_kv) return fail(HIVE_EINVAL, "null out_kv");
   auto* out = reinterpret_cast<unsigned long long*>(out_kv);
   HIP_TRY(hipMemsetAsync(t->scratch, 0, 8, s));
-  k_dump_slots<<<grid_for(static_cast<size_t>(t->v.nb) * t->v.S, 256), 256, 0, s>>>(t->v, out, t->scratch);
-  k_stash_compact<<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, out, t->scratch);
+  HIVE_POLICY_SWITCH(table_kind(t),
+                     (k_dump_slots<P><<<grid_for(static_cast<size_t>(t->v.nb) * t->v.S, 256), 256, 0, s>>>(t->v, out, t->scratch),
+                      k_stash_compact<P><<<grid_for(t->v.nb, 256), 256, 0, s>>>(t->v, out, t->scratch)));
   HIP_CHECK_LAUNCH();
   unsigned long long written = 0;
   HIP_TRY(hipMemcpyAsync(&written, t->scratch, 8, hipMemcpyDeviceToHost, s));
@@ -411,6 +422,7 @@ hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream) {
   out->load_factor = static_cast<double>(tab) / static_cast<double>(out->capacity);
   out->device_bytes = t->bytes;
   out->policy = t->policy;
+  out->layout = layout_of(t->v);
   return HIVE_OK;
 }
 
@@ -458,6 +470,21 @@ hive_status_t hive_export_raw(hive_table_t* t, uint64_t* h_slots, uint64_t* h_st
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t nb = t->v.nb;
+  if (t->v.layout == kLayoutInline) {  // converted on the device into the separate layout's arrays
+    DevArena ar;
+    unsigned long long *ds = nullptr, *dst = nullptr;
+    uint32_t* dc = nullptr;
+    HIP_TRY(ar.alloc(&ds, nb * kRefSlots));
+    HIP_TRY(ar.alloc(&dst, nb * kStashSlots));
+    HIP_TRY(ar.alloc(&dc, nb));
+    HIVE_POLICY_SWITCH(table_kind(t), (k_inline_to_raw<P><<<grid_for(nb, 256), 256, 0, s>>>(t->v, ds, dst, dc)));
+    HIP_CHECK_LAUNCH();
+    if (h_slots) HIP_TRY(hipMemcpyAsync(h_slots, ds, nb * kRefSlots * 8, hipMemcpyDeviceToHost, s));
+    if (h_stash) HIP_TRY(hipMemcpyAsync(h_stash, dst, nb * kStashSlots * 8, hipMemcpyDeviceToHost, s));
+    if (h_stash_count) HIP_TRY(hipMemcpyAsync(h_stash_count, dc, nb * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return HIVE_OK;
+  }
   if (h_slots) HIP_TRY(hipMemcpyAsync(h_slots, t->v.slots, nb * t->v.S * 8, hipMemcpyDeviceToHost, s));
   if (h_stash) HIP_TRY(hipMemcpyAsync(h_stash, t->v.stash, nb * kStashSlots * 8, hipMemcpyDeviceToHost, s));
   if (h_stash_count) HIP_TRY(hipMemcpyAsync(h_stash_count, t->v.stash_count, nb * 4, hipMemcpyDeviceToHost, s));
@@ -474,6 +501,29 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
   const size_t nb = t->v.nb;
   for (size_t b = 0; b < nb; ++b)
     if (h_stash_count[b] > static_cast<uint32_t>(kStashSlots)) return fail(HIVE_EINVAL, "stash count above 8");
+  if (t->v.layout == kLayoutInline) {  // converted on the device from the separate layout's arrays
+    DevArena ar;
+    unsigned long long *ds = nullptr, *dst = nullptr;
+    uint32_t* dc = nullptr;
+    HIP_TRY(ar.alloc(&ds, nb * kRefSlots));
+    HIP_TRY(ar.alloc(&dst, nb * kStashSlots));
+    HIP_TRY(ar.alloc(&dc, nb));
+    HIP_TRY(hipMemcpyAsync(ds, h_slots, nb * kRefSlots * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dc, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));
+    k_fill_u64<<<1, 64, 0, s>>>(t->scratch, 2, 0ull);
+    k_fill_u32<<<1, 64, 0, s>>>(t->v.pool_next, 1, 0u);
+    HIVE_POLICY_SWITCH(table_kind(t), (k_raw_to_inline<P><<<grid_for(nb, 256), 256, 0, s>>>(t->v, ds, dst, dc, t->scratch),
+                                       k_rebuild_meta<P><<<grid_for(nb * t->v.S, 256), 256, 0, s>>>(t->v)));
+    HIP_CHECK_LAUNCH();
+    unsigned long long failed = 0;
+    HIP_TRY(hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (failed) return fail(HIVE_ECAPACITY, "import: more stash buckets than the stash pool holds");
+    uint64_t a = 0, b = 0;
+    if ((r = count_exact_view(t, t->v, s, &a, &b)) != HIVE_OK) return r;
+    return reset_counters(t, t->v, s, a, b);
+  }
   HIP_TRY(hipMemcpyAsync(t->v.slots, h_slots, nb * t->v.S * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(t->v.stash_count, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));

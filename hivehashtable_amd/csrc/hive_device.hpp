@@ -63,16 +63,35 @@ struct Op16 {  // the reference's Operation record (include/utils.h:18-21)
   unsigned long long key;
 };
 
+// Table layouts.
+//  Separate arrays (P::kInline false): slots[nb * S]; hint bits, locks and
+//    stash counts in arrays of their own (meta packs kHintBits per bucket);
+//    stash[nb * 8], one stash bucket per canonical bucket.
+//  In-line header (P::kInline true, 8 slots): each bucket owns its 128-byte
+//    line -- slots 0..7, then a 16-byte header {hint, lock, stash count, stash
+//    ref} and padding -- so a probe's one line request brings the bucket's
+//    hint, lock and stash fields with it.  The hint word has kInlineClasses
+//    overflow classes and the stash bit.  Stash buckets come from a pool of
+//    pool_cap 8-slot buckets taken on a bucket's first push (stash ref = pool
+//    index + 1, 0 = none), so the stash occupies what it holds.
 struct TableView {
-  unsigned long long* slots;        // nb * S
-  unsigned long long* stash;        // nb * 8
-  uint32_t* stash_count;            // nb
-  uint32_t* meta;                   // kHintBits per bucket, kHintsPerWord buckets per word
-  uint32_t* locks;                  // nb
+  unsigned long long* slots;        // nb * S (separate) or nb * 16 (in-line header)
+  unsigned long long* stash;        // nb * 8 (separate) or pool_cap * 8 (in-line header)
+  uint32_t* stash_count;            // nb (separate layout)
+  uint32_t* meta;                   // kHintBits per bucket, kHintsPerWord buckets per word (separate layout)
+  uint32_t* locks;                  // nb (separate layout)
   unsigned long long* counters;     // [3][kCounterLanes][kCounterStride]: table, stash, slow-path ops
+  uint32_t* pool_next;              // stash buckets taken from the pool (in-line header layout)
   uint32_t nb;
-  uint32_t S;                       // slots per bucket (8 or 16), for code not instantiated per width
+  uint32_t S;                       // slots per bucket (8 or 16), for code not instantiated per layout
+  uint32_t layout;                  // 0 separate arrays, 1 in-line header
+  uint32_t pool_cap;                // stash buckets in the pool (in-line header layout)
 };
+constexpr uint32_t kLayoutSeparate = 0, kLayoutInline = 1;
+constexpr uint32_t kInlineClasses = 31;             // overflow classes of an in-line hint word
+constexpr uint32_t kInlineStashBit = 1u << 31;
+constexpr int kInlineStride = 16;                   // u64 per in-line bucket line
+constexpr int kInlineHeader = 8;                    // u64 index of the header in the line
 
 // ------------------------------------------------------------------ hashes
 // include/hash.hpp:39-47
@@ -165,6 +184,7 @@ __device__ __forceinline__ uint32_t mix_crc32(uint32_t key) {
 struct PolicyBase {
   static constexpr bool kCoherentReads = false;
   static constexpr int kS = kRefSlots;  // slots per bucket
+  static constexpr bool kInline = false;  // bucket header in the bucket's line
 };
 // include/HashPolicies.cuh:10-65
 struct PolicyDefault2 : PolicyBase {
@@ -203,6 +223,18 @@ template <class P>
 struct Wide16 : P {
   static constexpr int kS = 16;
 };
+
+// The same hashes over 8-slot buckets with the in-line header.
+template <class P>
+struct Inline8 : P {
+  static constexpr bool kInline = true;
+};
+
+// u64 words from one bucket to the next
+template <class P>
+__host__ __device__ constexpr int bucket_stride() {
+  return P::kInline ? kInlineStride : P::kS;
+}
 
 // The same hashes with every table read a device-coherent load: for code that
 // runs with no kernel boundary after other XCDs' writes (the persistent stream
@@ -369,10 +401,94 @@ __device__ __forceinline__ uint32_t ovf_bit(uint32_t key) {
 
 template <class P>
 __device__ __forceinline__ unsigned long long* bucket_ptr(const TableView& t, uint32_t b) {
-  return t.slots + static_cast<size_t>(b) * P::kS;
+  return t.slots + static_cast<size_t>(b) * bucket_stride<P>();
 }
-__device__ __forceinline__ unsigned long long* stash_ptr(const TableView& t, uint32_t b) {
-  return t.stash + static_cast<size_t>(b) * kStashSlots;
+
+// ---- bucket fields through the layout
+// Hdr: a bucket's hint bits, stash count and stash ref as one value.  The
+// separate layout fills hint only (count is read on demand); the in-line
+// layout reads the four header words with one 16-byte load from the line.
+struct Hdr {
+  uint32_t hint, count, ref;
+};
+template <class P>
+__device__ __forceinline__ uint32_t* header_ptr(const TableView& t, uint32_t b) {
+  return reinterpret_cast<uint32_t*>(t.slots + static_cast<size_t>(b) * kInlineStride + kInlineHeader);
+}
+template <class P>
+__device__ __forceinline__ uint32_t stash_bit() {
+  return P::kInline ? kInlineStashBit : kStashBit;
+}
+// a key's overflow class bit: a multiply-shift of a mix independent of the
+// bucket hashes onto [0, classes)
+template <class P>
+__device__ __forceinline__ uint32_t ovf_bit_of(uint32_t key) {
+  if constexpr (P::kInline) {
+    const uint32_t x = fmix32(key ^ 0x2545F491u);
+    return 1u << static_cast<uint32_t>((static_cast<unsigned long long>(x) * kInlineClasses) >> 32);
+  } else {
+    return ovf_bit(key);
+  }
+}
+template <class P>
+__device__ __forceinline__ uint32_t* lock_ptr(const TableView& t, uint32_t c) {
+  if constexpr (P::kInline) return header_ptr<P>(t, c) + 1;
+  else return t.locks + c;
+}
+template <class P>
+__device__ __forceinline__ uint32_t* count_ptr(const TableView& t, uint32_t c) {
+  if constexpr (P::kInline) return header_ptr<P>(t, c) + 2;
+  else return t.stash_count + c;
+}
+// the bucket's header (table reads outside locked sections: plain, or
+// coherent under CoherentReads)
+template <class P>
+__device__ __forceinline__ Hdr read_hdr(const TableView& t, uint32_t b) {
+  if constexpr (P::kInline) {
+    const uint32_t* h = header_ptr<P>(t, b);
+    if constexpr (P::kCoherentReads) {
+      const u64x2 v = *reinterpret_cast<const volatile u64x2*>(h);
+      return Hdr{static_cast<uint32_t>(v.x), static_cast<uint32_t>(v.y), static_cast<uint32_t>(v.y >> 32)};
+    } else {
+      const uint4 v = *reinterpret_cast<const uint4*>(h);
+      return Hdr{v.x, v.z, v.w};
+    }
+  } else {
+    return Hdr{meta_bits(meta_word<P>(t, b), b), 0u, 0u};
+  }
+}
+// coherent header read (locked sections)
+template <class P>
+__device__ __forceinline__ Hdr read_hdr_sync(const TableView& t, uint32_t b) {
+  if constexpr (P::kInline) {
+    const u64x2 v = *reinterpret_cast<const volatile u64x2*>(header_ptr<P>(t, b));
+    return Hdr{static_cast<uint32_t>(v.x), static_cast<uint32_t>(v.y), static_cast<uint32_t>(v.y >> 32)};
+  } else {
+    return Hdr{meta_bits(ld_sync32(t.meta + b / kHintsPerWord), b), 0u, 0u};
+  }
+}
+template <class P>
+__device__ __forceinline__ void hint_set(const TableView& t, uint32_t b, uint32_t bits) {
+  if constexpr (P::kInline)
+    __hip_atomic_fetch_or(header_ptr<P>(t, b), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    meta_set(t, b, bits);
+}
+// entries pushed to bucket c's stash (at most 8), plain or coherent
+template <class P, bool Sync = false>
+__device__ __forceinline__ uint32_t stash_count_of(const TableView& t, uint32_t c, const Hdr& h) {
+  uint32_t n;
+  if constexpr (P::kInline) n = h.count;
+  else if constexpr (Sync) n = ld_sync32(t.stash_count + c);
+  else n = read_u32<P>(t.stash_count + c);
+  return n > kStashSlots ? kStashSlots : n;
+}
+// bucket c's stash bucket (in the in-line layout: h.ref != 0, which holds
+// whenever the stash bit or a nonzero count was read)
+template <class P>
+__device__ __forceinline__ unsigned long long* stash_of(const TableView& t, uint32_t c, const Hdr& h) {
+  if constexpr (P::kInline) return t.stash + static_cast<size_t>(h.ref - 1u) * kStashSlots;
+  else return t.stash + static_cast<size_t>(c) * kStashSlots;
 }
 
 template <class P>
@@ -402,7 +518,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
   {
     unsigned long long s0[P::kS];
     read_bucket<P>(bucket_ptr<P>(t, b[0]), s0);
-    hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+    hint = read_hdr<P>(t, b[0]).hint;  // the same round (in the in-line layout, the same line)
     m0 = match_mask(s0, key);
     e0 = empty_mask(s0);
 #pragma unroll
@@ -410,7 +526,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
       if (m0 && q == __builtin_ctz(m0)) cur0 = s0[q];
   }
   if (m0) return update_slot(bucket_ptr<P>(t, b[0]) + __builtin_ctz(m0), cur0, kv) ? R_UPDATED : R_PENDING;
-  if (hint & ovf_bit(key)) {
+  if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
@@ -423,7 +539,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
       }
     }
   }
-  if (hint & kStashBit) return R_PENDING;  // Stage A' runs under the lock
+  if (hint & stash_bit<P>()) return R_PENDING;  // Stage A' runs under the lock
   uint32_t e = e0;
   unsigned long long* cb = bucket_ptr<P>(t, b[0]);
   while (e) {
@@ -495,7 +611,6 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
   constexpr int NV = kMaxKicks < S ? kMaxKicks : S;  // the kicks' victims: slots 0 .. NV-1
   const unsigned long long kv = kv_pack(key, val);
   unsigned long long* cb = bucket_ptr<P>(t, c);
-  unsigned long long* sb = stash_ptr(t, c);
   // Buckets are reduced to what the protocol needs as soon as they arrive
   // (match and empty masks, the matching slot, the victims), so that no more
   // than the two buckets of one round are live in registers.
@@ -509,11 +624,11 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     uint32_t m, e;
     unsigned long long cur = 0;  // the lowest matching slot's value
     unsigned long long vic[NV];
-    uint32_t hint;
+    Hdr h;
     {
       unsigned long long cs[S];
       load_bucket_sync(cb, cs);
-      hint = meta_bits(ld_sync32(t.meta + c / kHintsPerWord), c);
+      h = read_hdr_sync<P>(t, c);
       if (attempt == 0 && pre_alt != ~0u) {
         unsigned long long pre[S];
         read_bucket<P>(bucket_ptr<P>(t, pre_alt), pre);
@@ -528,11 +643,12 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
 #pragma unroll
       for (int q = 0; q < NV; ++q) vic[q] = cs[q];
     }
+    const uint32_t hint = h.hint;
     if (m) {
       if (update_slot(cb + __builtin_ctz(m), cur, kv)) return R_UPDATED;
       continue;
     }
-    if (hint & ovf_bit(key)) {
+    if (hint & ovf_bit_of<P>(key)) {
       uint32_t b[P::NH];
 #pragma unroll
       for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
@@ -546,11 +662,10 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     }
     if (retry) continue;
     uint32_t cnt = 0;
-    if (hint & kStashBit) {
-      cnt = ld_sync32(t.stash_count + c);
-      if (cnt > kStashSlots) cnt = kStashSlots;
+    if (hint & stash_bit<P>()) {
+      cnt = stash_count_of<P, true>(t, c, h);
       if (cnt) {
-        const int r = locked_update<kStashSlots>(sb, static_cast<int>(cnt), key, kv);
+        const int r = locked_update<kStashSlots>(stash_of<P>(t, c, h), static_cast<int>(cnt), key, kv);
         if (r > 0) return R_UPDATED;
         if (r < 0) continue;
       }
@@ -589,8 +704,8 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       const uint32_t vh0 = bucket_of<P>(0, vkey, t.nb);
       // before the victim can leave its H0 (already set when vh0 == c and the
       // coherent hint read above saw it)
-      const uint32_t vbit = ovf_bit(vkey);
-      if (a != vh0 && !(vh0 == c && (hint & vbit))) meta_set(t, vh0, vbit);
+      const uint32_t vbit = ovf_bit_of<P>(vkey);
+      if (a != vh0 && !(vh0 == c && (hint & vbit))) hint_set<P>(t, vh0, vbit);
       int d = -1;
       while (ae) {
         const int j = __builtin_ctz(ae);
@@ -603,9 +718,17 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
     }
     if (retry) continue;
     if (cnt < kStashSlots) {
-      __hip_atomic_store(sb + cnt, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(t.stash_count + c, cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      meta_set(t, c, kStashBit);
+      if constexpr (P::kInline) {
+        if (h.ref == 0u) {  // the bucket's first push takes a stash bucket from the pool
+          const uint32_t idx = __hip_atomic_fetch_add(t.pool_next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (idx >= t.pool_cap) return R_FAILED;  // pool exhausted
+          h.ref = idx + 1u;
+          __hip_atomic_store(header_ptr<P>(t, c) + 3, h.ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __hip_atomic_store(stash_of<P>(t, c, h) + cnt, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(count_ptr<P>(t, c), cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      hint_set<P>(t, c, stash_bit<P>());
       return R_STASHED;
     }
     return R_FAILED;
@@ -626,14 +749,14 @@ __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
   uint8_t result = R_PENDING;
   uint32_t backoff = 1;
   while (result == R_PENDING) {
-    if (try_lock(t.locks + c)) {
+    if (try_lock(lock_ptr<P>(t, c))) {
       uint32_t pa = ~0u;
       if (ps0 != 0ull) {
         pa = alt_bucket<P>(kv_key(ps0), c, t.nb);
         if (pa == c) pa = ~0u;
       }
       result = locked_insert<P>(t, key, val, c, ps0, pa);
-      unlock(t.locks + c);
+      unlock(lock_ptr<P>(t, c));
     } else {
       for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
       backoff = backoff < 64 ? backoff * 2 : 64;
@@ -653,20 +776,24 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
+  Hdr h{0u, 0u, 0u};
   {
     unsigned long long s[P::kS];
     read_bucket<P>(bucket_ptr<P>(t, b[0]), s);
+    if constexpr (P::kInline) h = read_hdr<P>(t, b[0]);  // the same line: no request of its own
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
   }
-  // the hint word is read only after an H0 miss: requesting it with H0 costs
-  // every lookup one more memory request, which the request-rate-bound kernel
-  // pays for more than the round trip it saves on misses (measured)
-  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
-  if (hint & ovf_bit(key)) {
+  // separate layout: the hint word is read only after an H0 miss: requesting
+  // it with H0 costs every lookup one more memory request, which the
+  // request-rate-bound kernel pays for more than the round trip it saves on
+  // misses (measured)
+  if constexpr (!P::kInline) h = read_hdr<P>(t, b[0]);
+  const uint32_t hint = h.hint;
+  if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {
       if (is_dup<P>(b, i)) continue;
@@ -679,10 +806,9 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
       }
     }
   }
-  if (hint & kStashBit) {
-    uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
-    if (cnt > kStashSlots) cnt = kStashSlots;
-    const unsigned long long* sb = stash_ptr(t, b[0]);
+  if (hint & stash_bit<P>()) {
+    const uint32_t cnt = stash_count_of<P>(t, b[0], h);
+    const unsigned long long* sb = stash_of<P>(t, b[0], h);
     for (uint32_t j = 0; j < cnt; ++j) {
       const unsigned long long cur = read_u64<P>(sb + j);
       if (cur != 0ull && kv_key(cur) == key) {
@@ -739,16 +865,16 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+  const Hdr h = read_hdr<P>(t, b[0]);
+  const uint32_t hint = h.hint;
   removed_tab += erase_bucket<P>(bucket_ptr<P>(t, b[0]), key);
-  if (hint & ovf_bit(key)) {
+  if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) removed_tab += erase_bucket<P>(bucket_ptr<P>(t, b[i]), key);
   }
-  if (hint & kStashBit) {
-    uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
-    if (cnt > kStashSlots) cnt = kStashSlots;
-    removed_stash = erase_stash<P>(stash_ptr(t, b[0]), cnt, key);
+  if (hint & stash_bit<P>()) {
+    const uint32_t cnt = stash_count_of<P>(t, b[0], h);
+    removed_stash = erase_stash<P>(stash_of<P>(t, b[0], h), cnt, key);
   }
   return (removed_tab + removed_stash) ? R_ERASED : R_NOT_FOUND;
 }

@@ -67,26 +67,35 @@ inline unsigned fill_grid(size_t n) {
   return static_cast<unsigned>(g ? g : 1);
 }
 
-// A table's kind: its hash policy and bucket width.  HIVE_POLICY_SWITCH
-// runs BODY with P bound to the matching policy type (kS slots per bucket).
-inline int table_kind(const hive_table* t) { return t->policy + (t->v.S == 16 ? 8 : 0); }
-inline int table_kind(int policy, uint32_t slots_per_bucket) { return policy + (slots_per_bucket == 16 ? 8 : 0); }
+// A table's kind: its hash policy and layout (HIVE_LAYOUT_*).
+// HIVE_POLICY_SWITCH runs BODY with P bound to the matching policy type.
+inline int table_kind(int policy, uint32_t layout) { return policy + 8 * static_cast<int>(layout); }
+inline int table_kind(const hive_table* t) {
+  return table_kind(t->policy, t->v.layout == kLayoutInline ? HIVE_LAYOUT_INLINE8 : (t->v.S == 16 ? HIVE_LAYOUT_BUCKET16
+                                                                                                : HIVE_LAYOUT_BUCKET8));
+}
 
-#define HIVE_POLICY_SWITCH(kind, BODY)                                                                      \
-  switch (kind) {                                                                                         \
-    case HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::PolicyDefault2; BODY; } break;                         \
-    case HIVE_POLICY_MURMURCITY: { using P = ::hive::PolicyMurmurCity; BODY; } break;                         \
-    case HIVE_POLICY_TRIPLE: { using P = ::hive::PolicyTriple; BODY; } break;                                 \
-    case HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::PolicyMurmurCityBit; BODY; } break;                   \
-    case HIVE_POLICY_LOOKUP2: { using P = ::hive::PolicyLookup2; BODY; } break;                               \
-    case HIVE_POLICY_LOOKUP3: { using P = ::hive::PolicyLookup3; BODY; } break;                               \
-    case 8 + HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::Wide16<::hive::PolicyDefault2>; BODY; } break;     \
-    case 8 + HIVE_POLICY_MURMURCITY: { using P = ::hive::Wide16<::hive::PolicyMurmurCity>; BODY; } break;     \
-    case 8 + HIVE_POLICY_TRIPLE: { using P = ::hive::Wide16<::hive::PolicyTriple>; BODY; } break;             \
-    case 8 + HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::Wide16<::hive::PolicyMurmurCityBit>; BODY; } break; \
-    case 8 + HIVE_POLICY_LOOKUP2: { using P = ::hive::Wide16<::hive::PolicyLookup2>; BODY; } break;           \
-    case 8 + HIVE_POLICY_LOOKUP3: { using P = ::hive::Wide16<::hive::PolicyLookup3>; BODY; } break;           \
-    default: return ::hive::fail(HIVE_EINVAL, "unknown policy");                                            \
+#define HIVE_POLICY_SWITCH(kind, BODY)                                                                            \
+  switch (kind) {                                                                                                 \
+    case 0 + HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::PolicyDefault2; BODY; } break;                         \
+    case 0 + HIVE_POLICY_MURMURCITY: { using P = ::hive::PolicyMurmurCity; BODY; } break;                         \
+    case 0 + HIVE_POLICY_TRIPLE: { using P = ::hive::PolicyTriple; BODY; } break;                                 \
+    case 0 + HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::PolicyMurmurCityBit; BODY; } break;                   \
+    case 0 + HIVE_POLICY_LOOKUP2: { using P = ::hive::PolicyLookup2; BODY; } break;                               \
+    case 0 + HIVE_POLICY_LOOKUP3: { using P = ::hive::PolicyLookup3; BODY; } break;                               \
+    case 8 + HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::Wide16<::hive::PolicyDefault2>; BODY; } break;         \
+    case 8 + HIVE_POLICY_MURMURCITY: { using P = ::hive::Wide16<::hive::PolicyMurmurCity>; BODY; } break;         \
+    case 8 + HIVE_POLICY_TRIPLE: { using P = ::hive::Wide16<::hive::PolicyTriple>; BODY; } break;                 \
+    case 8 + HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::Wide16<::hive::PolicyMurmurCityBit>; BODY; } break;   \
+    case 8 + HIVE_POLICY_LOOKUP2: { using P = ::hive::Wide16<::hive::PolicyLookup2>; BODY; } break;               \
+    case 8 + HIVE_POLICY_LOOKUP3: { using P = ::hive::Wide16<::hive::PolicyLookup3>; BODY; } break;               \
+    case 16 + HIVE_POLICY_DEFAULT2HASH: { using P = ::hive::Inline8<::hive::PolicyDefault2>; BODY; } break;       \
+    case 16 + HIVE_POLICY_MURMURCITY: { using P = ::hive::Inline8<::hive::PolicyMurmurCity>; BODY; } break;       \
+    case 16 + HIVE_POLICY_TRIPLE: { using P = ::hive::Inline8<::hive::PolicyTriple>; BODY; } break;               \
+    case 16 + HIVE_POLICY_MURMURCITYBIT: { using P = ::hive::Inline8<::hive::PolicyMurmurCityBit>; BODY; } break; \
+    case 16 + HIVE_POLICY_LOOKUP2: { using P = ::hive::Inline8<::hive::PolicyLookup2>; BODY; } break;             \
+    case 16 + HIVE_POLICY_LOOKUP3: { using P = ::hive::Inline8<::hive::PolicyLookup3>; BODY; } break;             \
+    default: return ::hive::fail(HIVE_EINVAL, "unknown policy or layout");                                        \
   }
 
 template <class Src, class Out>
@@ -271,24 +280,52 @@ inline void free_view(TableView& v) {
   v = TableView{};
 }
 
-inline size_t view_bytes(uint32_t nb, uint32_t slots_per_bucket) {
-  return static_cast<size_t>(nb) * slots_per_bucket * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
-         static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_words() * 8;
+// ---- table views by layout (HIVE_LAYOUT_*)
+inline bool valid_layout(int layout) {
+  return layout == HIVE_LAYOUT_BUCKET8 || layout == HIVE_LAYOUT_BUCKET16 || layout == HIVE_LAYOUT_INLINE8;
+}
+inline uint32_t layout_slots(int layout) { return layout == HIVE_LAYOUT_BUCKET16 ? 16u : 8u; }
+inline int layout_of(const TableView& v) {
+  return v.layout == kLayoutInline ? HIVE_LAYOUT_INLINE8 : (v.S == 16 ? HIVE_LAYOUT_BUCKET16 : HIVE_LAYOUT_BUCKET8);
+}
+// stash buckets of an in-line table's pool: a quarter of the buckets
+inline uint32_t pool_buckets(uint32_t nb) { return nb / 4 + 64; }
+// counters, then (in-line layout) the pool cursor on a line of its own
+inline size_t counter_bytes() { return counter_words() * 8 + 128; }
+
+inline size_t view_bytes(uint32_t nb, int layout) {
+  if (layout == HIVE_LAYOUT_INLINE8)
+    return static_cast<size_t>(nb) * kInlineStride * 8 + static_cast<size_t>(pool_buckets(nb)) * kStashSlots * 8 +
+           counter_bytes();
+  return static_cast<size_t>(nb) * layout_slots(layout) * 8 + static_cast<size_t>(nb) * kStashSlots * 8 +
+         static_cast<size_t>(nb) * 4 * 2 + meta_words(nb) * 4 + counter_bytes();
 }
 
-inline bool valid_width(uint64_t slots_per_bucket) { return slots_per_bucket == 8 || slots_per_bucket == 16; }
-
-inline hive_status_t alloc_view(TableView& v, uint32_t nb, uint32_t slots_per_bucket) {
+inline hive_status_t alloc_view(TableView& v, uint32_t nb, int layout) {
   v = TableView{};
   v.nb = nb;
-  v.S = slots_per_bucket;
+  v.S = layout_slots(layout);
+  v.layout = layout == HIVE_LAYOUT_INLINE8 ? kLayoutInline : kLayoutSeparate;
   hipError_t e = hipSuccess;
-  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * slots_per_bucket * 8)) != hipSuccess ||
+  if (layout == HIVE_LAYOUT_INLINE8) {
+    v.pool_cap = pool_buckets(nb);
+    if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kInlineStride * 8)) != hipSuccess ||
+        (e = hipMalloc(&v.stash, static_cast<size_t>(v.pool_cap) * kStashSlots * 8)) != hipSuccess ||
+        (e = hipMalloc(&v.counters, counter_bytes())) != hipSuccess) {
+      free_view(v);
+      (void)hipGetLastError();
+      return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
+                  std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    v.pool_next = reinterpret_cast<uint32_t*>(v.counters + counter_words());
+    return HIVE_OK;
+  }
+  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * v.S * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
       (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
       (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
       (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.counters, counter_words() * 8)) != hipSuccess) {
+      (e = hipMalloc(&v.counters, counter_bytes())) != hipSuccess) {
     free_view(v);
     (void)hipGetLastError();
     return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
@@ -297,19 +334,42 @@ inline hive_status_t alloc_view(TableView& v, uint32_t nb, uint32_t slots_per_bu
   return HIVE_OK;
 }
 
-// zero buckets, stash pointers, hint bits, locks and counters (stash slots at or
-// above a bucket's count are never read, so the stash body is left as is).
+// zero buckets (with their in-line headers), stash pointers, hint bits, locks
+// and counters (stash slots at or above a bucket's count are never read, so
+// the stash body is left as is).
 inline hive_status_t clear_view(const TableView& v, hipStream_t s) {
+  const size_t cw = counter_bytes() / 8;
+  k_fill_u64<<<fill_grid(cw), 256, 0, s>>>(v.counters, cw, 0ull);
+  if (v.layout == kLayoutInline) {
+    const size_t ns = static_cast<size_t>(v.nb) * kInlineStride;
+    k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
+    HIP_CHECK_LAUNCH();
+    return HIVE_OK;
+  }
   const size_t ns = static_cast<size_t>(v.nb) * v.S;
   k_fill_u64<<<fill_grid(ns), 256, 0, s>>>(v.slots, ns, 0ull);
   k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.stash_count, v.nb, 0u);
   k_fill_u32<<<fill_grid(meta_words(v.nb)), 256, 0, s>>>(v.meta, meta_words(v.nb), 0u);
   k_fill_u32<<<fill_grid(v.nb), 256, 0, s>>>(v.locks, v.nb, 0u);
-  k_fill_u64<<<fill_grid(counter_words()), 256, 0, s>>>(v.counters, counter_words(), 0ull);
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }
 
+
+// rehash every live entry of src into dst (k_rehash, instantiated for dst's
+// policy P and src's layout); failures counted in *failed
+template <class P>
+hive_status_t launch_rehash(const TableView& dst, const TableView& src, unsigned long long* failed, hipStream_t s) {
+  const size_t n = rehash_threads(src.nb, src.S);
+  const unsigned g = grid_for(n, kDefaultBlock);
+  switch (layout_of(src)) {
+    case HIVE_LAYOUT_BUCKET16: k_rehash<P, LayoutOf<16, false>><<<g, kDefaultBlock, 0, s>>>(dst, src, failed); break;
+    case HIVE_LAYOUT_INLINE8: k_rehash<P, LayoutOf<8, true>><<<g, kDefaultBlock, 0, s>>>(dst, src, failed); break;
+    default: k_rehash<P, LayoutOf<8, false>><<<g, kDefaultBlock, 0, s>>>(dst, src, failed); break;
+  }
+  HIP_CHECK_LAUNCH();
+  return HIVE_OK;
+}
 
 inline hive_status_t check_table(const hive_table* t) {
   if (!t) return fail(HIVE_EINVAL, "null table");

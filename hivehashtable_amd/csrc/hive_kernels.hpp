@@ -435,8 +435,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
     if (group_match<L, W>(v, key, val)) {
       res = R_FOUND;
     } else {
-      const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
-      if (hint & ovf_bit(key)) {
+      const Hdr hd = read_hdr<P>(t, b[0]);
+      const uint32_t hint = hd.hint;
+      if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll
         for (int h = 1; h < P::NH; ++h) {
           if (res == R_FOUND || is_dup<P>(b, h)) continue;
@@ -444,11 +445,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_coop(TableView t, Src src,
           if (group_match<L, W>(v, key, val)) res = R_FOUND;
         }
       }
-      if (res != R_FOUND && (hint & kStashBit)) {
-        uint32_t cnt = read_u32<P>(t.stash_count + b[0]);
-        if (cnt > kStashSlots) cnt = kStashSlots;
+      if (res != R_FOUND && (hint & stash_bit<P>())) {
+        const uint32_t cnt = stash_count_of<P>(t, b[0], hd);
         unsigned long long vs[WS];
-        load_part<WS>(stash_ptr(t, b[0]) + r * WS, vs);
+        load_part<WS>(stash_of<P>(t, b[0], hd) + r * WS, vs);
 #pragma unroll
         for (int j = 0; j < WS; ++j)
           if (r * WS + j >= cnt) vs[j] = 0ull;  // past the count: not an entry
@@ -476,8 +476,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_
 // ---------------------------------------------------------------- rehash
 // One thread per source entry, inserted into dst with the same fast/slow
 // protocol.  A wavefront takes one slot of a row of 64 consecutive source
-// buckets; region 0 holds the SS bucket slots of the source (SS = the
-// source's width, which may differ from dst's), region 1 the 8 stash slots.
+// buckets; region 0 holds the bucket slots of the source (PS: the source's
+// layout, which may differ from dst's), region 1 its 8 stash slots.
 // The keys of one source bucket share H0 in the source and land in the same
 // two destination buckets, where lanes running at the same time would race
 // for the same empty slots.  So a row's slots go to wavefronts kRehashChunk
@@ -491,8 +491,15 @@ __host__ __device__ inline size_t rehash_rows(uint32_t nb) { return (static_cast
 inline size_t rehash_threads(uint32_t nb, uint32_t ss) {
   return rehash_rows(nb) * (ss + kStashSlots) * kRehashRows;
 }
-template <class P, int SS>
+// the layout part of a policy (no hashes): a rehash's source
+template <int S, bool Inl>
+struct LayoutOf : PolicyBase {
+  static constexpr int kS = S;
+  static constexpr bool kInline = Inl;
+};
+template <class P, class PS>
 __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView src, unsigned long long* failed) {
+  constexpr int SS = PS::kS;
   __shared__ unsigned long long queue[kMaxBlock];
   __shared__ uint32_t qn;
   if (threadIdx.x == 0) qn = 0;
@@ -510,8 +517,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
     const uint32_t slot = static_cast<uint32_t>(within / nrow);
     const size_t b = (crow + within % nrow) * kRehashRows + i % kRehashRows;
     if (b < src.nb) {
-      if (!in_stash) kv = src.slots[b * SS + slot];
-      else if (slot < src.stash_count[b]) kv = src.stash[b * kStashSlots + slot];
+      if (!in_stash) {
+        kv = bucket_ptr<PS>(src, static_cast<uint32_t>(b))[slot];
+      } else {
+        const Hdr h = read_hdr<PS>(src, static_cast<uint32_t>(b));
+        if (slot < stash_count_of<PS>(src, static_cast<uint32_t>(b), h))
+          kv = stash_of<PS>(src, static_cast<uint32_t>(b), h)[slot];
+      }
     }
   }
   int dtab = 0, dstash = 0;
@@ -544,17 +556,23 @@ static __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// exact occupancy (include/hive_kernels.cuh:303-325, counting kv != 0)
-static __global__ void k_count_exact(TableView t, unsigned long long* out) {
-  const size_t ntab = static_cast<size_t>(t.nb) * t.S, nst = static_cast<size_t>(t.nb) * kStashSlots;
+// exact occupancy (include/hive_kernels.cuh:303-325, counting kv != 0): one
+// thread per bucket slot; the first of each bucket also counts its stash
+template <class P>
+__global__ void k_count_exact(TableView t, unsigned long long* out) {
+  const size_t ntab = static_cast<size_t>(t.nb) * P::kS;
   unsigned long long ct = 0, cs = 0;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < ntab;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    ct += (t.slots[i] != 0ull);
-    if (i < nst) {
-      const uint32_t b = static_cast<uint32_t>(i / kStashSlots);
-      const uint32_t j = static_cast<uint32_t>(i % kStashSlots);
-      cs += (j < t.stash_count[b] && t.stash[i] != 0ull);
+    const uint32_t b = static_cast<uint32_t>(i / P::kS), j = static_cast<uint32_t>(i % P::kS);
+    ct += (bucket_ptr<P>(t, b)[j] != 0ull);
+    if (j == 0) {
+      const Hdr h = read_hdr<P>(t, b);
+      const uint32_t n = stash_count_of<P>(t, b, h);
+      if (n) {
+        const unsigned long long* sb = stash_of<P>(t, b, h);
+        for (uint32_t q = 0; q < n; ++q) cs += (sb[q] != 0ull);
+      }
     }
   }
   int a = static_cast<int>(ct), c = static_cast<int>(cs);
@@ -568,58 +586,119 @@ static __global__ void k_count_exact(TableView t, unsigned long long* out) {
 
 // Rebuild the hint bits from a table's content (after import): stash bit from
 // each stash count, overflow bit of H0(key) for every entry stored outside
-// its H0.  meta must be zeroed first.  One thread per slot.
+// its H0.  The hint bits must be zero first.  One thread per slot.
 template <class P>
 __global__ void k_rebuild_meta(TableView t) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= static_cast<size_t>(t.nb) * P::kS) return;
   const uint32_t b = static_cast<uint32_t>(i / P::kS);
-  const unsigned long long kv = t.slots[i];
+  const unsigned long long kv = bucket_ptr<P>(t, b)[i % P::kS];
   if (kv != 0ull) {
     const uint32_t h0 = bucket_of<P>(0, kv_key(kv), t.nb);
-    if (h0 != b) meta_set(t, h0, ovf_bit(kv_key(kv)));
+    if (h0 != b) hint_set<P>(t, h0, ovf_bit_of<P>(kv_key(kv)));
   }
-  if (i % P::kS == 0 && t.stash_count[b] > 0) meta_set(t, b, kStashBit);
+  if (i % P::kS == 0 && stash_count_of<P>(t, b, read_hdr<P>(t, b)) > 0) hint_set<P>(t, b, stash_bit<P>());
 }
 
-// drop every stash bit, keep the overflow bits (stash drain)
-static __global__ void k_clear_stash_bits(TableView t) {
-  uint32_t keep = 0;
-  for (uint32_t j = 0; j < kHintsPerWord; ++j) keep |= kOverflowMask << (j * kHintBits);
+// drop every stash bit, keep the overflow bits, reset every stash (stash
+// drain): one thread per hint word (separate layout) or per bucket (in-line)
+template <class P>
+__global__ void k_reset_stash(TableView t) {
   const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w < (static_cast<size_t>(t.nb) + kHintsPerWord - 1) / kHintsPerWord) t.meta[w] &= keep;
+  if constexpr (P::kInline) {
+    if (w >= t.nb) return;
+    uint32_t* h = header_ptr<P>(t, static_cast<uint32_t>(w));
+    h[0] &= ~kInlineStashBit;
+    h[2] = 0u;
+    h[3] = 0u;
+    if (w == 0) *t.pool_next = 0u;
+  } else {
+    uint32_t keep = 0;
+    for (uint32_t j = 0; j < kHintsPerWord; ++j) keep |= kOverflowMask << (j * kHintBits);
+    if (w < (static_cast<size_t>(t.nb) + kHintsPerWord - 1) / kHintsPerWord) t.meta[w] &= keep;
+    for (size_t b = w * kHintsPerWord; b < (w + 1) * kHintsPerWord && b < t.nb; ++b) t.stash_count[b] = 0u;
+  }
 }
 
 // stash drain (include/hive_stash_table.cuh:33-64; holes are skipped)
-static __global__ void k_stash_count_live(TableView t, unsigned long long* total) {
+template <class P>
+__global__ void k_stash_count_live(TableView t, unsigned long long* total) {
   const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int c = 0;
   if (b < t.nb) {
-    uint32_t cnt = t.stash_count[b];
-    if (cnt > kStashSlots) cnt = kStashSlots;
-    for (uint32_t j = 0; j < cnt; ++j) c += (t.stash[b * kStashSlots + j] != 0ull);
+    const Hdr h = read_hdr<P>(t, static_cast<uint32_t>(b));
+    const uint32_t cnt = stash_count_of<P>(t, static_cast<uint32_t>(b), h);
+    if (cnt) {
+      const unsigned long long* sb = stash_of<P>(t, static_cast<uint32_t>(b), h);
+      for (uint32_t j = 0; j < cnt; ++j) c += (sb[j] != 0ull);
+    }
   }
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, static_cast<unsigned long long>(c));
 }
 
-static __global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned long long* pos) {
+template <class P>
+__global__ void k_stash_compact(TableView t, unsigned long long* out, unsigned long long* pos) {
   const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= t.nb) return;
-  uint32_t cnt = t.stash_count[b];
-  if (cnt > kStashSlots) cnt = kStashSlots;
+  const Hdr h = read_hdr<P>(t, static_cast<uint32_t>(b));
+  const uint32_t cnt = stash_count_of<P>(t, static_cast<uint32_t>(b), h);
+  if (!cnt) return;
+  const unsigned long long* sb = stash_of<P>(t, static_cast<uint32_t>(b), h);
   for (uint32_t j = 0; j < cnt; ++j) {
-    const unsigned long long kv = t.stash[b * kStashSlots + j];
+    const unsigned long long kv = sb[j];
     if (kv != 0ull) out[atomicAdd(pos, 1ull)] = kv;
   }
 }
 
-// live bucket entries of slots [0, nb*S), appended at *pos with one atomic
-// per wavefront
-static __global__ void k_dump_slots(TableView t, unsigned long long* out, unsigned long long* pos) {
+// The in-line layout in the separate layout's raw arrays (export) and back
+// (import): slots[nb * 8], stash[nb * 8], stash_count[nb]; one thread per
+// bucket.  Import takes a pool stash bucket for every bucket with a count
+// and leaves the hint words zero (k_rebuild_meta follows).
+template <class P>
+__global__ void k_inline_to_raw(TableView t, unsigned long long* slots, unsigned long long* stash, uint32_t* count) {
+  const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= t.nb) return;
+  const unsigned long long* bk = bucket_ptr<P>(t, static_cast<uint32_t>(b));
+  for (int j = 0; j < P::kS; ++j) slots[b * P::kS + j] = bk[j];
+  const Hdr h = read_hdr<P>(t, static_cast<uint32_t>(b));
+  const uint32_t n = stash_count_of<P>(t, static_cast<uint32_t>(b), h);
+  count[b] = h.count;
+  const unsigned long long* sb = n ? stash_of<P>(t, static_cast<uint32_t>(b), h) : nullptr;
+  for (int j = 0; j < kStashSlots; ++j) stash[b * kStashSlots + j] = (static_cast<uint32_t>(j) < n) ? sb[j] : 0ull;
+}
+template <class P>
+__global__ void k_raw_to_inline(TableView t, const unsigned long long* slots, const unsigned long long* stash,
+                                const uint32_t* count, unsigned long long* failed) {
+  const size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= t.nb) return;
+  unsigned long long* bk = bucket_ptr<P>(t, static_cast<uint32_t>(b));
+  for (int j = 0; j < P::kS; ++j) bk[j] = slots[b * P::kS + j];
+  uint32_t* h = header_ptr<P>(t, static_cast<uint32_t>(b));
+  h[0] = 0u;
+  h[1] = 0u;
+  h[2] = 0u;
+  h[3] = 0u;
+  const uint32_t n = count[b];
+  if (n == 0u) return;
+  const uint32_t idx = atomicAdd(t.pool_next, 1u);
+  if (idx >= t.pool_cap) {
+    atomicAdd(failed, 1ull);
+    return;
+  }
+  h[2] = n;
+  h[3] = idx + 1u;
+  unsigned long long* sb = t.stash + static_cast<size_t>(idx) * kStashSlots;
+  for (int j = 0; j < kStashSlots; ++j) sb[j] = stash[b * kStashSlots + j];
+}
+
+// live bucket entries of the nb * S bucket slots, appended at *pos with one
+// atomic per wavefront
+template <class P>
+__global__ void k_dump_slots(TableView t, unsigned long long* out, unsigned long long* pos) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t n = static_cast<size_t>(t.nb) * t.S;
-  const unsigned long long kv = i < n ? t.slots[i] : 0ull;
+  const size_t n = static_cast<size_t>(t.nb) * P::kS;
+  const unsigned long long kv = i < n ? bucket_ptr<P>(t, static_cast<uint32_t>(i / P::kS))[i % P::kS] : 0ull;
   const unsigned long long live = __ballot(kv != 0ull);
   if (live == 0ull) return;  // uniform across the wavefront
   const int lane = static_cast<int>(threadIdx.x & 63);
@@ -651,7 +730,7 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
     uint32_t b[P::NH];
 #pragma unroll
     for (int j = 0; j < P::NH; ++j) b[j] = bucket_of<P>(j, key, t.nb);
-    const uint32_t hint = meta_bits(meta_word<P>(t, b[0]), b[0]);
+    const uint32_t hint = read_hdr<P>(t, b[0]).hint;
     v[0] = 1;
     v[1] = 1;
     unsigned long long s[P::kS];
@@ -660,14 +739,14 @@ __global__ __launch_bounds__(256) void k_probe_stats(TableView t, const uint32_t
     if (!found) v[3] = 1;
 #pragma unroll
     for (int j = 1; j < P::NH; ++j) {
-      if (!(hint & ovf_bit(key)) || is_dup<P>(b, j)) continue;
+      if (!(hint & ovf_bit_of<P>(key)) || is_dup<P>(b, j)) continue;
       ++v[0];
       if (found) continue;
       ++v[1];
       load_bucket(bucket_ptr<P>(t, b[j]), s);
       found = match_mask(s, key) != 0;
     }
-    if (!found && (hint & kStashBit)) v[2] = 1;
+    if (!found && (hint & stash_bit<P>())) v[2] = 1;
   }
 #pragma unroll
   for (int f = 0; f < 4; ++f) {

@@ -112,7 +112,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       }
     } guard{&views};
     for (size_t i = 0; i < views.size(); ++i) {
-      if ((r = alloc_view(views[i], target_nb[i], t->v.S)) != HIVE_OK) return r;
+      if ((r = alloc_view(views[i], target_nb[i], layout_of(t->v))) != HIVE_OK) return r;
       if ((r = clear_view(views[i], s)) != HIVE_OK) return r;
     }
     DevArena ar;
@@ -132,9 +132,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       for (size_t b = 0; b < nbatch && rc == HIVE_OK; ++b) {
         if (steps[b].grow >= 0) {
           const TableView nv = views[steps[b].grow];
-          const size_t nsrc = rehash_threads(t->v.nb, t->v.S);
-          HIVE_POLICY_SWITCH(table_kind(t), (k_rehash<P, P::kS><<<grid_for(nsrc, kDefaultBlock), kDefaultBlock, 0,
-                                                                  cs>>>(nv, t->v, d_fail)));
+          HIVE_POLICY_SWITCH(table_kind(t), (rc = launch_rehash<P>(nv, t->v, d_fail, cs)));
           t->v = nv;
         }
         rc = launch_batch(b, steps[b].lo, steps[b].n, cs);
@@ -164,7 +162,7 @@ extern "C" hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* k
       TableView old = orig;
       free_view(old);
       guard.keep_last = true;  // the handle now owns the last target table
-      t->bytes = view_bytes(t->v.nb, t->v.S) + 16;
+      t->bytes = view_bytes(t->v.nb, layout_of(t->v)) + 16;
     }
     rep.resizes = static_cast<uint32_t>(views.size());
     rep.rehash_failed = failed;

@@ -274,7 +274,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
           size_t free_b = 0, total_b = 0;
           HIP_TRY(hipMemGetInfo(&free_b, &total_b));
           if (new_nb <= 0xFFFFFFFFull &&
-              free_b >= view_bytes(static_cast<uint32_t>(new_nb), t->v.S) + c.hbm_reserve_bytes) {
+              free_b >= view_bytes(static_cast<uint32_t>(new_nb), layout_of(t->v)) + c.hbm_reserve_bytes) {
             const Clock::time_point tr = Clock::now();
             if ((r = hive_resize(t, new_nb * t->v.S, stream)) != HIVE_OK) return r;
             if ((r = refresh()) != HIVE_OK) return r;

@@ -440,11 +440,17 @@ namespace {
 
 hive_status_t copy_view(const TableView& dst, const TableView& src, hipStream_t s) {
   const size_t nb = src.nb;
+  HIP_TRY(hipMemcpyAsync(dst.counters, src.counters, counter_bytes(), hipMemcpyDeviceToDevice, s));
+  if (src.layout == kLayoutInline) {
+    HIP_TRY(hipMemcpyAsync(dst.slots, src.slots, nb * kInlineStride * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst.stash, src.stash, static_cast<size_t>(src.pool_cap) * kStashSlots * 8,
+                           hipMemcpyDeviceToDevice, s));
+    return HIVE_OK;
+  }
   HIP_TRY(hipMemcpyAsync(dst.slots, src.slots, nb * src.S * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.stash, src.stash, nb * kStashSlots * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.stash_count, src.stash_count, nb * 4, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(dst.meta, src.meta, meta_words(src.nb) * 4, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dst.counters, src.counters, counter_words() * 8, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemsetAsync(dst.locks, 0, nb * 4, s));
   return HIVE_OK;
 }
@@ -467,7 +473,7 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   if ((r = ensure_partition(t, std::max<uint64_t>(maxn, windows[2]))) != HIVE_OK) return r;
   if ((r = ensure_window(t, windows[2])) != HIVE_OK) return r;
   hive_table tmp = *t;  // same policy and device, scratch copy of the table state
-  if ((r = alloc_view(tmp.v, t->v.nb, t->v.S)) != HIVE_OK) return r;
+  if ((r = alloc_view(tmp.v, t->v.nb, layout_of(t->v))) != HIVE_OK) return r;
   struct ViewGuard {
     TableView* v;
     ~ViewGuard() { free_view(*v); }
@@ -617,7 +623,7 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   DevArena ar;
   if (n_insert) {
     hive_table tmp = *t;  // scratch copy of the table state, same policy and device
-    if ((r = alloc_view(tmp.v, t->v.nb, t->v.S)) != HIVE_OK) return r;
+    if ((r = alloc_view(tmp.v, t->v.nb, layout_of(t->v))) != HIVE_OK) return r;
     struct ViewGuard {
       TableView* v;
       ~ViewGuard() { free_view(*v); }

@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BulkTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, POLICIES, check, lib,
+from ._lib import (BulkTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, LAYOUTS, POLICIES,
+                   check, lib,
                    OP_INSERT,
                    OP_LOOKUP, OP_DELETE)
 
@@ -49,19 +50,24 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
-class HiveTable:
-    """A device-resident Hive hash table (8- or 16-slot buckets, 8-slot stash per bucket).
+def layout_id(layout):
+    return LAYOUTS[layout] if isinstance(layout, str) else int(layout)
 
-    ``num_slots`` is the reference's ``table_size``: num_buckets =
-    num_slots // slots_per_bucket (8: the reference's bucket; 16: one 128-byte
-    line per bucket).
+
+class HiveTable:
+    """A device-resident Hive hash table with an 8-slot stash per bucket.
+
+    ``num_slots`` is the reference's ``table_size``: num_buckets = num_slots
+    // slots per bucket.  ``layout``: "bucket8" (8-slot 64-byte buckets, the
+    reference's bucket), "bucket16" (16-slot 128-byte buckets) or "inline8"
+    (8 slots and an in-line header per 128-byte line, pooled stash).
     """
 
-    def __init__(self, num_slots, policy="Default2Hash", slots_per_bucket=8):
+    def __init__(self, num_slots, policy="Default2Hash", layout="bucket8"):
         self.L = lib()
         self.policy = POLICIES[policy] if isinstance(policy, str) else int(policy)
         h = C.c_void_p()
-        check(self.L.hive_create_ex(int(num_slots), self.policy, int(slots_per_bucket), C.byref(h)), "hive_create_ex")
+        check(self.L.hive_create_ex(int(num_slots), self.policy, layout_id(layout), C.byref(h)), "hive_create_ex")
         self.h = h
 
     def close(self):
@@ -83,10 +89,13 @@ class HiveTable:
     def slots_per_bucket(self):
         return int(self.stats()["slots_per_bucket"])
 
-    def set_slots_per_bucket(self, slots_per_bucket, stream=None):
-        """Rehash into buckets of the given width (8 or 16) at the same slot capacity."""
-        check(self.L.hive_set_slots_per_bucket(self.h, int(slots_per_bucket), _stream_ptr(stream)),
-              "hive_set_slots_per_bucket")
+    @property
+    def layout(self):
+        return int(self.stats()["layout"])
+
+    def set_layout(self, layout, stream=None):
+        """Rehash into another layout at the same slot capacity."""
+        check(self.L.hive_set_layout(self.h, layout_id(layout), _stream_ptr(stream)), "hive_set_layout")
 
     # ---------------------------------------------------------- bulk ops
     def clear(self, stream=None):

@@ -7,17 +7,23 @@
  * process.  Citations are to the reference (SabbirPulok/HiveHashTable)
  * interfaces each entry point replaces.
  *
- * Table layout (device memory owned by the handle):
+ * Table layouts (device memory owned by the handle), chosen per table:
+ * HIVE_LAYOUT_BUCKET8 (the reference's bucket) and HIVE_LAYOUT_BUCKET16:
  *   slots[nb * S]   u64 packed kv = (uint64)value << 32 | key, 0 = empty;
- *                   S = 8 slots per bucket (one 64-byte bucket, densely
- *                   packed: the reference's bucket) or S = 16 (a bucket
- *                   owns its 128-byte line), chosen per table
+ *                   S = 8 slots (one dense 64-byte bucket) or 16 (a bucket
+ *                   owns its 128-byte line)
  *   stash[nb * 8]   u64, one 8-slot overflow bucket per canonical bucket
  *   stash_count[nb] u32 monotonic push pointer of each stash bucket
  *   meta[nb/16]     u32, 2 hint bits per bucket: overflow (a key whose H0 is
  *                   this bucket may sit in another candidate bucket) and
  *                   stash (stash_count > 0)
  *   locks[nb]       u32 per-canonical-bucket lock of the insert slow path
+ * HIVE_LAYOUT_INLINE8: 8 slots per bucket, each bucket in its own 128-byte
+ *   line with an in-line header after the slots: the hint word (31 overflow
+ *   classes and the stash bit), the lock, the stash count and the index of
+ *   the bucket's stash in a pool of nb / 4 + 64 8-slot stash buckets taken
+ *   on first push (an insert that needs a stash bucket when the pool is
+ *   exhausted fails).  Same protocol and results as HIVE_LAYOUT_BUCKET8.
  * Key 0 is the empty sentinel and is rejected by every operation
  * (include/utils.h:40-47 of the reference).
  */
@@ -39,6 +45,13 @@ typedef enum {
   HIVE_ECAPACITY = 4,  /* resize target cannot hold the live entries */
   HIVE_ERANGE = 5      /* output buffer too small */
 } hive_status_t;
+
+/* Table layouts (the table layout comment above). */
+typedef enum {
+  HIVE_LAYOUT_BUCKET8 = 0,   /* 8-slot 64-byte buckets, separate hint / lock / stash arrays (the reference's bucket) */
+  HIVE_LAYOUT_BUCKET16 = 1,  /* 16-slot 128-byte buckets, separate arrays */
+  HIVE_LAYOUT_INLINE8 = 2    /* 8 slots + in-line header per 128-byte line, pooled stash */
+} hive_layout_t;
 
 /* Hash policies: include/HashPolicies.cuh:10-65.  The string names accepted
  * by hive_policy_from_name are the reference's (src/hive_dispatch.cu:829-864). */
@@ -86,6 +99,7 @@ typedef struct {
   uint64_t device_bytes;     /* device memory held by the handle */
   int policy;
   uint64_t slow_path_ops;    /* inserts that took the locked slow path since the last clear/import */
+  int layout;                /* HIVE_LAYOUT_* */
 } hive_stats_t;
 
 /* Launch shape of the plain bulk kernels.  0 = library default / autotuned. */
@@ -114,13 +128,12 @@ typedef struct {
  * (src/hive_dispatch.cu:630-631).  The handle keeps the table resident on the
  * current device until hive_destroy. */
 hive_status_t hive_create(uint64_t num_slots, int policy, hive_table_t** out);
-/* hive_create with a bucket width: slots_per_bucket 8 (the reference's
- * bucket; what hive_create makes) or 16 (128-byte buckets: one line per
- * probe, fewer keys leave their canonical bucket).  num_buckets =
- * num_slots / slots_per_bucket.  Same protocol (include/hive_hash_insert.cuh:
- * 898-1042 with SLOTS = the width; the cuckoo stage still tries 8 victims,
- * slots 0..7) and the same 8-slot stash per bucket. */
-hive_status_t hive_create_ex(uint64_t num_slots, int policy, uint32_t slots_per_bucket, hive_table_t** out);
+/* hive_create with a layout (HIVE_LAYOUT_*; hive_create makes
+ * HIVE_LAYOUT_BUCKET8).  num_buckets = num_slots / slots per bucket.  The
+ * protocol is include/hive_hash_insert.cuh:898-1042 with SLOTS = the
+ * layout's slots per bucket (the cuckoo stage tries 8 victims, slots 0..7)
+ * and an 8-slot stash per bucket. */
+hive_status_t hive_create_ex(uint64_t num_slots, int policy, int layout, hive_table_t** out);
 hive_status_t hive_destroy(hive_table_t* t);
 hive_status_t hive_clear(hive_table_t* t, void* stream);
 hive_status_t hive_policy_from_name(const char* name, int* policy);
@@ -354,12 +367,12 @@ hive_status_t hive_insert_sequence(hive_table_t* t, const uint32_t* keys, const 
  * hive_stash_reinsert: drain the stash and insert the drained entries back
  *   (with their values) through the normal insert path.  Synchronizes. */
 hive_status_t hive_resize(hive_table_t* t, uint64_t new_num_slots, void* stream);
-/* hive_set_slots_per_bucket: rehash the table into buckets of the given
- * width (8 or 16) at the same slot capacity, as hive_resize does (no
- * reference counterpart: the reference's width is fixed at build time,
- * HIVE_BUCKET_SLOTS).  HIVE_ECAPACITY (table unchanged) if an entry could
- * not be placed.  Synchronizes. */
-hive_status_t hive_set_slots_per_bucket(hive_table_t* t, uint32_t slots_per_bucket, void* stream);
+/* hive_set_layout: rehash the table into another layout (HIVE_LAYOUT_*) at
+ * the same slot capacity, as hive_resize does (no reference counterpart: the
+ * reference's bucket is fixed at build time, HIVE_BUCKET_SLOTS).
+ * HIVE_ECAPACITY (table unchanged) if an entry could not be placed.
+ * Synchronizes. */
+hive_status_t hive_set_layout(hive_table_t* t, int layout, void* stream);
 /* hive_dump: every live entry (bucket slots, then stash) as packed kv into
  *   out_kv (device, cap entries), in no particular order -- the content
  *   multiset, for checks and persistence (the reference has no export; its
@@ -374,7 +387,9 @@ hive_status_t hive_stash_reinsert(hive_table_t* t, size_t* n_reinserted, void* s
  *   (include/hive_kernels.cuh:303-325, src/hive_dispatch.cu:166-177).
  * hive_count_exact scans the table instead of reading the counters.
  * hive_export_raw / hive_import_raw copy the raw layout to / from host
- *   memory (arrays sized num_buckets*slots_per_bucket, num_buckets*8, num_buckets). */
+ *   memory (arrays sized num_buckets*slots_per_bucket, num_buckets*8,
+ *   num_buckets: the separate layouts' slots, stash and stash counts; an
+ *   in-line table is converted to and from them). */
 hive_status_t hive_stats(hive_table_t* t, hive_stats_t* out, void* stream);
 hive_status_t hive_count_exact(hive_table_t* t, uint64_t* table_live, uint64_t* stash_live, void* stream);
 /* Probe statistics for a key set against the current table (no reference
@@ -425,7 +440,7 @@ hive_status_t hive_ceiling_cas(uint64_t* buf, uint64_t footprint_bytes, uint64_t
  * wavefront-per-bucket layout measurement.  For lanes_per_bucket > 1 the
  * sink receives at most one u32 (sink[0]). */
 /* hive_ceiling_gather_bytes: the gather with bucket_bytes-sized reads (64:
- *   the 8-slot bucket, 128: the 16-slot bucket, one line). */
+ *   the 8-slot bucket, 128: a 128-byte line). */
 hive_status_t hive_ceiling_gather_bytes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,
                                         uint32_t bucket_bytes, uint32_t seed, uint32_t* sink, void* stream);
 hive_status_t hive_ceiling_gather_lanes(const uint64_t* buf, uint64_t footprint_bytes, uint64_t n,

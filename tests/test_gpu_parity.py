@@ -34,6 +34,8 @@ def np_(t, dtype=None):
 
 
 OK = (1, 2, 3)  # INSERTED, UPDATED, STASHED
+LAYOUT_NAMES = ["bucket8", "bucket16", "inline8"]
+SPB = {"bucket8": 8, "bucket16": 16, "inline8": 8}  # slots per bucket of each layout
 
 
 def placed(st):
@@ -54,14 +56,15 @@ def check_gpu_state(tbl, O, policy):
     return raw, chk
 
 
-@pytest.mark.parametrize("width", [8, 16])
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
 @pytest.mark.parametrize("policy", ["Default2Hash", "MurmurCityHash"])
-def test_cfg1_insert_then_lookup(H, O, policy, width):
-    print(f"[case] cfg1 2^20 slots LF 0.8 {policy} width {width}", flush=True)
+def test_cfg1_insert_then_lookup(H, O, policy, layout):
+    width = SPB[layout]
+    print(f"[case] cfg1 2^20 slots LF 0.8 {policy} layout {layout}", flush=True)
     slots, keys, vals, lk, present = W.cfg1()
-    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
+    o = O.Oracle(slots // width, policy, layout=layout)
     so = o.insert(keys, vals)
-    t = H.HiveTable(slots, policy, slots_per_bucket=width)
+    t = H.HiveTable(slots, policy, layout=layout)
     sg = np_(t.insert(keys, vals))
     assert np.array_equal(placed(sg), placed(so))
     assert not (sg == 2).any()  # distinct keys: no updates
@@ -79,21 +82,22 @@ def test_cfg1_insert_then_lookup(H, O, policy, width):
     assert (a, b) == (st["table_live"], st["stash_live"]) and a + b == placed(sg).sum()
 
 
-@pytest.mark.parametrize("width", [8, 16])
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
 @pytest.mark.parametrize("policy", ["Default2Hash", "MurmurCityHash", "TripleHash", "MurmurCityBitHash",
                                     "Lookup2Hash", "Lookup3Hash"])
-def test_lookup_and_erase_on_oracle_built_table(H, O, policy, width):
+def test_lookup_and_erase_on_oracle_built_table(H, O, policy, layout):
     # LF 0.97 so the stash is populated; lookups and deletes move nothing,
     # so results and the raw state after deletes must be identical.
-    print(f"[case] oracle-built table {policy} width {width}", flush=True)
+    width = SPB[layout]
+    print(f"[case] oracle-built table {policy} layout {layout}", flush=True)
     nb = 1 << 12
     keys = W.distinct_keys(int(nb * width * 0.97), 11)
     vals = W.values(keys.size, 12)
-    o = O.Oracle(nb, policy, slots_per_bucket=width)
+    o = O.Oracle(nb, policy, layout=layout)
     so = o.insert(keys, vals)
     assert (so == 3).any()
     raw = o.export()
-    t = H.HiveTable(nb * width, policy, slots_per_bucket=width)
+    t = H.HiveTable(nb * width, policy, layout=layout)
     t.import_raw(raw.slots, raw.stash, raw.stash_count)
     probe = np.concatenate([keys, W.distinct_keys(5000, 13, exclude=keys), np.array([0], np.uint32)])
     ov, os_ = o.lookup(probe)
@@ -116,14 +120,15 @@ def test_lookup_and_erase_on_oracle_built_table(H, O, policy, width):
     assert np.array_equal(after.live_kv(), want.live_kv())
 
 
-@pytest.mark.parametrize("width", [8, 16])
-def test_insert_at_full_load_keeps_invariants(H, O, width):
-    print(f"[case] LF 1.0 insert, 2^15 slots, width {width}", flush=True)
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_insert_at_full_load_keeps_invariants(H, O, layout):
+    width = SPB[layout]
+    print(f"[case] LF 1.0 insert, 2^15 slots, layout {layout}", flush=True)
     slots = 1 << 15
     keys = W.distinct_keys(slots, 21)
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    o = O.Oracle(slots // width, layout=layout)
     so = o.insert(keys)
-    t = H.HiveTable(slots, slots_per_bucket=width)
+    t = H.HiveTable(slots, layout=layout)
     sg = np_(t.insert(keys))
     raw, chk = check_gpu_state(t, O, "Default2Hash")
     ok = placed(sg)
@@ -160,20 +165,21 @@ def test_duplicates_in_one_batch_and_invalid_keys(H, O, combine):
     assert t.count_exact() == (3, 0)
 
 
-@pytest.mark.parametrize("combine,width", [(0, 8), (1, 8), (0, 16)])
-def test_contended_same_key_inserts_are_linearizable(H, O, combine, width):
+@pytest.mark.parametrize("combine,layout", [(0, "bucket8"), (1, "bucket8"), (0, "bucket16"), (0, "inline8")])
+def test_contended_same_key_inserts_are_linearizable(H, O, combine, layout):
     # Litmus for cross-XCD coherence of the claim/update protocol: 2^20 inserts
     # over only 4096 distinct keys, shuffled so every key is written by lanes
     # of many workgroups (and so of every XCD) in the same launch.  Exactly one
     # insert per key may report INSERTED, the table must hold each key once,
     # and its value must be one of the values written for it.
-    print(f"[case] contended same-key inserts, combine={combine} width={width}", flush=True)
+    width = SPB[layout]
+    print(f"[case] contended same-key inserts, combine={combine} layout={layout}", flush=True)
     rng = np.random.default_rng(81)
     base = W.distinct_keys(4096, 82)
     keys = np.tile(base, 256)
     rng.shuffle(keys)
     vals = (keys ^ np.uint32(0x5bd1e995)).astype(np.uint32)  # value is a function of the key
-    t = H.HiveTable(1 << 16, slots_per_bucket=width)
+    t = H.HiveTable(1 << 16, layout=layout)
     t.set_launch(0, 1024, combine_duplicates=combine)
     st = np_(t.insert(keys, vals))
     assert ((st == 1) | (st == 3)).sum() == base.size
@@ -183,7 +189,7 @@ def test_contended_same_key_inserts_are_linearizable(H, O, combine, width):
     assert np.array_equal(np.sort((kv & 0xFFFFFFFF).astype(np.uint32)), np.sort(base))
     assert np.array_equal((kv >> 32).astype(np.uint32), (kv & 0xFFFFFFFF).astype(np.uint32) ^ np.uint32(0x5bd1e995))
     # same with a table small enough that most keys take the locked slow path
-    t = H.HiveTable(1 << 9, slots_per_bucket=width)
+    t = H.HiveTable(1 << 9, layout=layout)
     t.set_launch(0, 1024, combine_duplicates=combine)
     small = base[:600]
     keys = np.tile(small, 64)
@@ -196,14 +202,16 @@ def test_contended_same_key_inserts_are_linearizable(H, O, combine, width):
     assert ((st == 1) | (st == 3)).sum() == k_in.size
 
 
-@pytest.mark.parametrize("combine,width", [(0, 8), (1, 8), (0, 16), (1, 16)])
-def test_mixed_batches_match_phased_oracle(H, O, combine, width):
-    print(f"[case] mixed batches 50/40/10, combine={combine} width={width}", flush=True)
+@pytest.mark.parametrize("combine,layout", [(0, "bucket8"), (1, "bucket8"), (0, "bucket16"), (1, "bucket16"),
+                                            (0, "inline8"), (1, "inline8")])
+def test_mixed_batches_match_phased_oracle(H, O, combine, layout):
+    width = SPB[layout]
+    print(f"[case] mixed batches 50/40/10, combine={combine} layout={layout}", flush=True)
     slots = 1 << 18
     pre = W.distinct_keys(slots // 2, 31)
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    o = O.Oracle(slots // width, layout=layout)
     o.insert(pre)
-    t = H.HiveTable(slots, slots_per_bucket=width)
+    t = H.HiveTable(slots, layout=layout)
     t.set_launch(0, 512, combine_duplicates=combine)
     t.insert(pre)
     conflicted = set()
@@ -268,13 +276,14 @@ def test_big_mixed_batches_split_by_type(H, O):
         t.close()
 
 
-@pytest.mark.parametrize("width", [8, 16])
-def test_resize_growth_sequence(H, O, width):
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_resize_growth_sequence(H, O, layout):
     # cfg3 shape at small scale: grow x2 whenever the next batch would exceed LF 0.9
-    print(f"[case] resize sequence, width {width}", flush=True)
+    width = SPB[layout]
+    print(f"[case] resize sequence, layout {layout}", flush=True)
     slots = 1 << 12
-    t = H.HiveTable(slots, slots_per_bucket=width)
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    t = H.HiveTable(slots, layout=layout)
+    o = O.Oracle(slots // width, layout=layout)
     all_keys = W.distinct_keys(64 * 512, 41)
     for i in range(64):
         batch = all_keys[i * 512:(i + 1) * 512]
@@ -370,29 +379,28 @@ def test_resize_keeps_content_at_odd_sizes(H, src_slots, n_keys, dst_slots):
     t.close()
 
 
-@pytest.mark.parametrize("width", [8, 16])
-def test_resize_and_width_change_keep_content_with_a_stash(H, O, width):
-    """A table of the given width at LF 0.93 (a populated stash) is resized
-    to an odd bucket count, then re-laid out in the other width and back:
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_resize_and_layout_change_keep_content_with_a_stash(H, O, layout):
+    """A table of the given layout at LF 0.93 (a populated stash) is resized
+    to an odd slot count, then re-laid out in each other layout and back:
     every time the content multiset is unchanged, every key is found with its
-    value and the invariants hold for the table's width."""
-    print(f"[case] resize and width change from width {width}", flush=True)
+    value and the invariants hold."""
+    print(f"[case] resize and layout change from layout {layout}", flush=True)
     slots = 1 << 20
     keys = W.distinct_keys(int(slots * 0.93), 901)
     vals = W.values(keys.size, 902)
-    t = H.HiveTable(slots, "MurmurCityHash", slots_per_bucket=width)
+    t = H.HiveTable(slots, "MurmurCityHash", layout=layout)
     st = np_(t.insert(keys, vals))
     ok = placed(st)
     before = np.sort(np_(t.dump()).view(np.uint64))
     assert t.stats()["stash_live"] > 0
-    other = 24 - width
-    for new_slots, new_width in ((1_300_000, width), (1_300_000, other), (1_300_000, width)):
-        if new_width == width and t.slots_per_bucket == width and t.num_buckets * width != new_slots:
-            t.resize(new_slots)
-        else:
-            t.set_slots_per_bucket(new_width)
-        assert t.slots_per_bucket == new_width
-        assert t.num_buckets == new_slots // new_width
+    t.resize(1_300_000)
+    others = [x for x in LAYOUT_NAMES if x != layout]
+    for step, lay in enumerate([layout] + others + [layout]):
+        if step:
+            t.set_layout(lay)
+        assert t.layout == H.LAYOUTS[lay]
+        assert t.num_buckets == 1_300_000 // SPB[lay]
         assert np.array_equal(np.sort(np_(t.dump()).view(np.uint64)), before)
         gv, gs = t.lookup(keys)
         assert np.array_equal(np_(gs) == 4, ok) and np.array_equal(np_(gv, np.uint32)[ok], vals[ok])
@@ -482,20 +490,23 @@ def test_launch_shapes_give_identical_results(H, O):
         ref = cur
 
 
-@pytest.mark.parametrize("policy,lf,width", [("Default2Hash", 0.9, 8), ("MurmurCityHash", 0.95, 8), ("Default2Hash", 1.0, 8),
-                                            ("Default2Hash", 0.9, 16), ("MurmurCityHash", 1.0, 16)])
-def test_full_size_properties(H, policy, lf, width):
+@pytest.mark.parametrize("policy,lf,layout", [("Default2Hash", 0.9, "bucket8"), ("MurmurCityHash", 0.95, "bucket8"),
+                                             ("Default2Hash", 1.0, "bucket8"), ("Default2Hash", 0.9, "bucket16"),
+                                             ("MurmurCityHash", 1.0, "bucket16"), ("Default2Hash", 0.9, "inline8"),
+                                             ("MurmurCityHash", 1.0, "inline8")])
+def test_full_size_properties(H, policy, lf, layout):
     """2^26 slots at the sweep's load factors: size-independent properties
     (every placed key found with its value, absent keys not found, exact
     counts, no key stored twice, re-insert = update)."""
     import torch
-    print(f"[case] 2^26 slots LF {lf} {policy} width {width} full size", flush=True)
+    width = SPB[layout]
+    print(f"[case] 2^26 slots LF {lf} {policy} layout {layout} full size", flush=True)
     slots = 1 << 26
     n = W.keys_for_lf(slots, lf)
     keys = W.distinct_keys(n, 44)
     vals = W.values(n, 45)
     absent = W.distinct_keys(1 << 22, 44, start=n + (n >> 6) + 64)
-    t = H.HiveTable(slots, policy, slots_per_bucket=width)
+    t = H.HiveTable(slots, policy, layout=layout)
     st = t.insert(keys, vals)
     ok = placed(np_(st))
     assert ok.mean() > 0.9999
@@ -535,15 +546,16 @@ def _clean_ops(types, keys, off):
     return (~dup) & (batch < fd)
 
 
-@pytest.mark.parametrize("width", [8, 16])
-def test_stream_modes_agree_with_phased_oracle(H, O, width):
-    print(f"[case] mixed stream: launches / graph / persistent, width {width}", flush=True)
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_stream_modes_agree_with_phased_oracle(H, O, layout):
+    width = SPB[layout]
+    print(f"[case] mixed stream: launches / graph / persistent, layout {layout}", flush=True)
     slots = 1 << 18
     pre = W.distinct_keys(slots // 2, 91)
     sizes = W.log_uniform_sizes(40, 1 << 6, 1 << 14, 92)
     sizes[5] = 0  # an empty batch
     types, keys, vals, off = W.mixed_stream(pre, 1 << 19, sizes, seed=93)
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    o = O.Oracle(slots // width, layout=layout)
     o.insert(pre)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
@@ -561,7 +573,7 @@ def test_stream_modes_agree_with_phased_oracle(H, O, width):
     for mode, inl, tile, bpc, sinl, win in shapes:
         print(f"[case] stream shape mode={mode} inline={inl} tile={tile} bpc={bpc} stream_inline={sinl} "
               f"window={win}", flush=True)
-        t = H.HiveTable(slots, slots_per_bucket=width)
+        t = H.HiveTable(slots, layout=layout)
         for op in (0, 1, 2):
             t.set_launch(op, 256, inline_slow_path=inl)
         t.set_stream_shape(tile, bpc, sinl)
@@ -601,15 +613,17 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"], 1)
 
 
-@pytest.mark.parametrize("policy,width", [("Default2Hash", 8), ("TripleHash", 8), ("Default2Hash", 16)])
-def test_windowed_stream_orders_each_key_by_batch(H, O, policy, width):
+@pytest.mark.parametrize("policy,layout", [("Default2Hash", "bucket8"), ("TripleHash", "bucket8"),
+                                           ("Default2Hash", "bucket16"), ("TripleHash", "inline8")])
+def test_windowed_stream_orders_each_key_by_batch(H, O, policy, layout):
     """Windows of many small batches in which the same keys recur in every
     batch (a hot set in all of them, cold keys from a small space): each
     key's ops must see exactly the batches before theirs (chains 63 batches
     long), against the oracle run batch by batch.  No key repeats inside a
     batch, so every op has one admissible outcome.  A few ops are NONE /
     ATOMIC_INC or carry key 0, which a window must answer like a batch."""
-    print(f"[case] windowed stream: recurring keys, {policy}, width {width}", flush=True)
+    width = SPB[layout]
+    print(f"[case] windowed stream: recurring keys, {policy}, layout {layout}", flush=True)
     slots = 1 << 14
     rng = np.random.default_rng(7)
     pre = W.distinct_keys(slots // 4, 71, bits=13)
@@ -627,7 +641,7 @@ def test_windowed_stream_orders_each_key_by_batch(H, O, policy, width):
     types[odd < 0.02] = rng.choice([3, 4], size=int((odd < 0.02).sum())).astype(np.uint8)
     keys[(odd >= 0.02) & (odd < 0.03)] = 0
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
-    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
+    o = O.Oracle(slots // width, policy, layout=layout)
     o.insert(pre)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
@@ -638,7 +652,7 @@ def test_windowed_stream_orders_each_key_by_batch(H, O, policy, width):
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
     for win, split in ((1 << 10, 1), (1 << 12, 0), (1 << 14, 1)):
         print(f"[case] windowed stream window={win} split={split}", flush=True)
-        t = H.HiveTable(slots, policy, slots_per_bucket=width)
+        t = H.HiveTable(slots, policy, layout=layout)
         t.set_launch(0, 256, split_by_type=split)
         t.set_stream_window(win, 1 << 10)
         t.insert(pre)

@@ -29,6 +29,10 @@ def np_(t, dtype=None):
     return a.view(dtype) if dtype is not None else a
 
 
+LAYOUT_NAMES = ["bucket8", "bucket16", "inline8"]
+SPB = {"bucket8": 8, "bucket16": 16, "inline8": 8}  # slots per bucket of each layout
+
+
 def norm(st):
     return np.where(st == 3, 1, st)  # STASHED is a placement detail: count it as INSERTED
 
@@ -50,16 +54,17 @@ def disjoint_requests(pre, fresh, nreq, seed, max_size=300):
     return types, keys, vals, off
 
 
-@pytest.mark.parametrize("width", [8, 16])
-def test_serve_results_same_alone_and_in_mix(H, O, width):
-    print(f"[case] serve: coalesced vs alone vs oracle, width {width}", flush=True)
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_serve_results_same_alone_and_in_mix(H, O, layout):
+    width = SPB[layout]
+    print(f"[case] serve: coalesced vs alone vs oracle, layout {layout}", flush=True)
     slots = 1 << 16
     pre = W.distinct_keys(slots // 2, 501)
     pre_v = W.values(pre.size, 502)
     fresh = W.distinct_keys(20000, 501, start=pre.size + 2)
     types, keys, vals, off = disjoint_requests(pre, fresh, 60, 503)
     nreq = off.size - 1
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    o = O.Oracle(slots // width, layout=layout)
     o.insert(pre, pre_v)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
@@ -70,7 +75,7 @@ def test_serve_results_same_alone_and_in_mix(H, O, width):
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
     results = {}
     for name, cap, inflight in (("mix", 700, 2), ("mix1", 5000, 1), ("alone", 1, 1)):
-        t = H.HiveTable(slots, slots_per_bucket=width)
+        t = H.HiveTable(slots, layout=layout)
         t.insert(pre, pre_v)
         gv, gs, lat, rep = t.serve(types, keys, vals, off, np.zeros(nreq), max_batch_ops=cap, max_inflight=inflight)
         gv, gs = np_(gv, np.uint32), np_(gs)
