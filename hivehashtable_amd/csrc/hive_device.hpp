@@ -776,22 +776,21 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  Hdr h{0u, 0u, 0u};
   {
     unsigned long long s[P::kS];
     read_bucket<P>(bucket_ptr<P>(t, b[0]), s);
-    if constexpr (P::kInline) h = read_hdr<P>(t, b[0]);  // the same line: no request of its own
     const uint32_t m = match_mask(s, key);
     if (m) {
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
   }
-  // separate layout: the hint word is read only after an H0 miss: requesting
-  // it with H0 costs every lookup one more memory request, which the
+  // the hint word is read only after an H0 miss: requesting it with H0 costs
+  // every lookup one more request (in the in-line layout, to the other half
+  // of the line the bucket read just brought into L2), which the
   // request-rate-bound kernel pays for more than the round trip it saves on
-  // misses (measured)
-  if constexpr (!P::kInline) h = read_hdr<P>(t, b[0]);
+  // misses (measured in both layouts)
+  const Hdr h = read_hdr<P>(t, b[0]);
   const uint32_t hint = h.hint;
   if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll

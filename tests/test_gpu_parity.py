@@ -62,7 +62,7 @@ def test_cfg1_insert_then_lookup(H, O, policy, layout):
     width = SPB[layout]
     print(f"[case] cfg1 2^20 slots LF 0.8 {policy} layout {layout}", flush=True)
     slots, keys, vals, lk, present = W.cfg1()
-    o = O.Oracle(slots // width, policy, layout=layout)
+    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
     so = o.insert(keys, vals)
     t = H.HiveTable(slots, policy, layout=layout)
     sg = np_(t.insert(keys, vals))
@@ -93,7 +93,7 @@ def test_lookup_and_erase_on_oracle_built_table(H, O, policy, layout):
     nb = 1 << 12
     keys = W.distinct_keys(int(nb * width * 0.97), 11)
     vals = W.values(keys.size, 12)
-    o = O.Oracle(nb, policy, layout=layout)
+    o = O.Oracle(nb, policy, slots_per_bucket=width)
     so = o.insert(keys, vals)
     assert (so == 3).any()
     raw = o.export()
@@ -126,7 +126,7 @@ def test_insert_at_full_load_keeps_invariants(H, O, layout):
     print(f"[case] LF 1.0 insert, 2^15 slots, layout {layout}", flush=True)
     slots = 1 << 15
     keys = W.distinct_keys(slots, 21)
-    o = O.Oracle(slots // width, layout=layout)
+    o = O.Oracle(slots // width, slots_per_bucket=width)
     so = o.insert(keys)
     t = H.HiveTable(slots, layout=layout)
     sg = np_(t.insert(keys))
@@ -209,7 +209,7 @@ def test_mixed_batches_match_phased_oracle(H, O, combine, layout):
     print(f"[case] mixed batches 50/40/10, combine={combine} layout={layout}", flush=True)
     slots = 1 << 18
     pre = W.distinct_keys(slots // 2, 31)
-    o = O.Oracle(slots // width, layout=layout)
+    o = O.Oracle(slots // width, slots_per_bucket=width)
     o.insert(pre)
     t = H.HiveTable(slots, layout=layout)
     t.set_launch(0, 512, combine_duplicates=combine)
@@ -283,7 +283,7 @@ def test_resize_growth_sequence(H, O, layout):
     print(f"[case] resize sequence, layout {layout}", flush=True)
     slots = 1 << 12
     t = H.HiveTable(slots, layout=layout)
-    o = O.Oracle(slots // width, layout=layout)
+    o = O.Oracle(slots // width, slots_per_bucket=width)
     all_keys = W.distinct_keys(64 * 512, 41)
     for i in range(64):
         batch = all_keys[i * 512:(i + 1) * 512]
@@ -555,7 +555,7 @@ def test_stream_modes_agree_with_phased_oracle(H, O, layout):
     sizes = W.log_uniform_sizes(40, 1 << 6, 1 << 14, 92)
     sizes[5] = 0  # an empty batch
     types, keys, vals, off = W.mixed_stream(pre, 1 << 19, sizes, seed=93)
-    o = O.Oracle(slots // width, layout=layout)
+    o = O.Oracle(slots // width, slots_per_bucket=width)
     o.insert(pre)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
@@ -641,7 +641,7 @@ def test_windowed_stream_orders_each_key_by_batch(H, O, policy, layout):
     types[odd < 0.02] = rng.choice([3, 4], size=int((odd < 0.02).sum())).astype(np.uint8)
     keys[(odd >= 0.02) & (odd < 0.03)] = 0
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
-    o = O.Oracle(slots // width, policy, layout=layout)
+    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
     o.insert(pre)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)

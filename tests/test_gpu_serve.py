@@ -64,7 +64,7 @@ def test_serve_results_same_alone_and_in_mix(H, O, layout):
     fresh = W.distinct_keys(20000, 501, start=pre.size + 2)
     types, keys, vals, off = disjoint_requests(pre, fresh, 60, 503)
     nreq = off.size - 1
-    o = O.Oracle(slots // width, layout=layout)
+    o = O.Oracle(slots // width, slots_per_bucket=width)
     o.insert(pre, pre_v)
     ov = np.empty(keys.size, np.uint32)
     os_ = np.empty(keys.size, np.uint8)
