@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
+    p.add_argument("--layout", default="auto", choices=["auto", "bucket8", "bucket16", "inline8"],
+                   help="bucket layout of the headline table (auto: timed on the workload, fastest kept)")
     p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
     p.add_argument("--mixed-batches", type=int, default=1000)
     p.add_argument("--stream-modes", default="launches,graph,persistent,windowed",
@@ -88,22 +90,23 @@ def git_sha():
         return None
 
 
-def traffic_from_profiles(lf):
-    """HBM bytes per launch of the dominant kernel, from the committed PMC summary."""
+def traffic_from_profiles(lf, layout, kernel):
+    """HBM bytes per launch of `kernel` in `layout`, from the committed PMC summary."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_bulk_2p26.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(f"lf{int(round(lf * 100)):03d}")
+        return d[layout][f"{kernel}_lf{int(round(lf * 100)):03d}"]
     except Exception:
         return None
 
 
-def request_counts_from_profiles():
-    """Memory-side requests per launch of the headline kernels (committed PMC summary)."""
+def request_counts_from_profiles(layout):
+    """Memory-side requests per launch of the headline kernels in `layout` (committed PMC summary)."""
     try:
         with open(os.path.join(ROOT, "profiles", "request_counts_bulk_2p26.json")) as f:
-            return json.load(f)
+            d = json.load(f)
+        return d.get(layout)
     except Exception:
         return None
 
@@ -505,6 +508,7 @@ def main():
     import torch
     import hivehashtable_amd as H
     from hivehashtable_amd import workloads as W
+    from hivehashtable_amd._lib import check
 
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
@@ -527,11 +531,21 @@ def main():
     d_st = torch.empty(n_ins, dtype=torch.uint8, device=dev)
     d_lv = torch.empty(n_look, dtype=torch.int32, device=dev)
     d_ls = torch.empty(n_look, dtype=torch.uint8, device=dev)
-    t = H.HiveTable(slots, a.policy)
+    t = H.HiveTable(slots, a.policy, layout="bucket8" if a.layout == "auto" else a.layout)
     if a.block:
         for op in (0, 1, 2):
             t.set_launch(op, a.block)
     stream = torch.cuda.current_stream()
+    layout_tune = None
+    if a.layout == "auto":
+        # every layout timed on this workload (insert into the empty table, then the lookups)
+        layout_tune = t.autotune_layout(d_keys, d_vals, d_lk)
+        layout_tune = {k: ({n: (round(x, 4) if x is not None else None) for n, x in v.items()}
+                           if isinstance(v, dict) else v) for k, v in layout_tune.items()}
+        log(f"layout tune: {layout_tune}")
+    layout = {v: k for k, v in H.LAYOUTS.items()}[t.layout]
+    spb = t.slots_per_bucket
+    nb = t.num_buckets
 
     def step(ev=None):
         if ev is not None:
@@ -589,34 +603,39 @@ def main():
     # algorithmic bytes (probe statistics of this exact table and key sets)
     cand, _, _ = t.probe_stats(d_keys)
     _, lreads, lstash = t.probe_stats(d_lk)
-    ins_bytes = n_ins * (4 + 4 + 1 + 4) + 64 * cand + 8 * int(placed.sum())  # +4: hint word of H0
-    look_bytes = n_look * (4 + 4 + 1) + 64 * lreads + (64 + 4) * lstash
+    bucket_bytes = 8 * spb
+    ins_bytes = n_ins * (4 + 4 + 1 + 4) + bucket_bytes * cand + 8 * int(placed.sum())  # +4: hint word of H0
+    look_bytes = n_look * (4 + 4 + 1) + bucket_bytes * lreads + (64 + 4) * lstash
     ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
     look_gbps = look_bytes / (look_ms / 1e3) / 1e9
     dominant = "insert" if ins_ms >= look_ms else "lookup"
     achieved = ins_gbps if dominant == "insert" else look_gbps
 
-    # same-footprint random-access ceilings
-    buf = torch.zeros(nb * 8, dtype=torch.int64, device=dev)
+    # same-footprint random-access ceilings (the layout's bucket array; reads of one bucket's size)
+    footprint = nb * (16 if layout == "inline8" else spb) * 8
+    buf = torch.zeros(footprint // 8, dtype=torch.int64, device=dev)
     sink = torch.zeros(1 << 20, dtype=torch.int32, device=dev)
     L = H.lib()
     import ctypes as C
     sptr = C.c_void_p(stream.cuda_stream)
     n_g = lreads
-    L.hive_ceiling_gather(C.c_void_p(buf.data_ptr()), nb * 64, n_g, 7, C.c_void_p(sink.data_ptr()), sptr)
+    def gather(seed):
+        check(L.hive_ceiling_gather_bytes(C.c_void_p(buf.data_ptr()), footprint, n_g, bucket_bytes, seed,
+                                            C.c_void_p(sink.data_ptr()), sptr), "hive_ceiling_gather_bytes")
+    gather(7)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     gms = []
     for r in range(3):
         e0.record(stream)
-        L.hive_ceiling_gather(C.c_void_p(buf.data_ptr()), nb * 64, n_g, 11 + r, C.c_void_p(sink.data_ptr()), sptr)
+        gather(11 + r)
         e1.record(stream)
         torch.cuda.synchronize()
         gms.append(e0.elapsed_time(e1))
-    gather_rate = n_g / (min(gms) / 1e3)  # random 64-byte reads per second
+    gather_rate = n_g / (min(gms) / 1e3)  # random bucket-sized reads per second
     cms = []
     for r in range(3):
         e0.record(stream)
-        L.hive_ceiling_cas(C.c_void_p(buf.data_ptr()), nb * 64, n_ins, 13 + r, sptr)
+        L.hive_ceiling_cas(C.c_void_p(buf.data_ptr()), footprint, n_ins, 13 + r, sptr)
         e1.record(stream)
         torch.cuda.synchronize()
         cms.append(e0.elapsed_time(e1))
@@ -627,7 +646,7 @@ def main():
         lms = []
         for r in range(3):
             e0.record(stream)
-            L.hive_ceiling_gather_lanes(C.c_void_p(buf.data_ptr()), nb * 64, n_g, lanes, 17 + r,
+            L.hive_ceiling_gather_lanes(C.c_void_p(buf.data_ptr()), footprint, n_g, lanes, 17 + r,
                                         C.c_void_p(sink.data_ptr()), sptr)
             e1.record(stream)
             torch.cuda.synchronize()
@@ -652,7 +671,7 @@ def main():
         threads, host = cpu_allotment()
         threads = a.cpu_threads or threads
         log(f"cpu baseline: {threads} threads ({host}), one full step, median of 5")
-        cpu = cpu_baseline(keys, vals, lk, threads, host, found, lv, a.policy, nb)
+        cpu = cpu_baseline(keys, vals, lk, threads, host, found, lv, a.policy, slots // 8)
         log(f"cpu: {cpu['value']} Mops/s, {cpu['seconds_per_run']} s per run, agree={cpu['lookups_agree_with_gpu']}")
 
     out = {
@@ -671,13 +690,14 @@ def main():
         "config": {
             "workload": (f"bulk insert {n_ins} distinct uniform u32 keys (LF {a.lf}) then bulk lookup {n_look} keys "
                          f"({a.exist:.0%} present) on a 2^{a.log2_slots}-slot table, {a.policy}"),
-            "slots": slots, "load_factor": a.lf, "exist_ratio": a.exist, "policy": a.policy,
+            "slots": slots, "load_factor": a.lf, "exist_ratio": a.exist, "policy": a.policy, "layout": layout,
             "inserts": n_ins, "lookups": n_look, "parallelism": "single GPU",
         },
         "insert_mops": round(n_ins / (ins_ms / 1e3) / 1e6, 3),
         "lookup_mops": round(n_look / (look_ms / 1e3) / 1e6, 3),
         "kernel_ms": {"clear": round(clear_ms, 4), "insert": round(ins_ms, 4), "lookup": round(look_ms, 4)},
         "verified": verified,
+        "layout_tune": layout_tune,
         "insert_slow_path_frac": round(slow_frac, 5),
         "stash_entries": tstats["stash_live"],
         "roofline": {
@@ -687,20 +707,22 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 5),
-            "traffic": traffic_from_profiles(a.lf),
+            "traffic": traffic_from_profiles(a.lf, layout, dominant),
             "algorithmic_bytes": {"insert": int(ins_bytes), "lookup": int(look_bytes)},
             "achieved_gbps": {"insert": round(ins_gbps, 2), "lookup": round(look_gbps, 2)},
         },
-        "request_roofline": request_roofline(request_counts_from_profiles(), ins_ms, look_ms, gather_rate, cas_rate),
+        "request_roofline": request_roofline(request_counts_from_profiles(layout), ins_ms, look_ms, gather_rate,
+                                             cas_rate),
         "ceiling": {
-            "random_gather_64B_per_s": round(gather_rate, 1),
+            "random_gather_per_s": round(gather_rate, 1),
+            "gather_bytes": bucket_bytes,
             "random_cas_u64_per_s": round(cas_rate, 1),
             "lookup_bucket_reads_per_s": round(look_read_rate, 1),
             "lookup_frac_of_gather_ceiling": round(look_read_rate / gather_rate, 4),
             "insert_bucket_reads_per_s": round(ins_read_rate, 1),
             "insert_frac_of_gather_ceiling": round(ins_read_rate / gather_rate, 4),
             "gather_64B_per_s_by_lanes_per_bucket": lane_rates,
-            "footprint_bytes": nb * 64,
+            "footprint_bytes": footprint,
         },
         "mixed_latency": mixed_summary(mixed),
         "mixed_stream": mixed,

@@ -71,6 +71,11 @@ class BulkTune(C.Structure):
                 ("load_factor", C.c_double), ("candidates", C.c_uint32)]
 
 
+class LayoutTune(C.Structure):
+    _fields_ = [("layout", C.c_int), ("insert_ms", C.c_double * 3), ("lookup_ms", C.c_double * 3),
+                ("candidates", C.c_uint32)]
+
+
 class ServeConfig(C.Structure):
     _fields_ = [("max_batch_ops", C.c_uint64), ("max_inflight", C.c_uint32), ("max_load_factor", C.c_double),
                 ("growth", C.c_double), ("hbm_reserve_bytes", C.c_uint64), ("stash_headroom", C.c_double),
@@ -146,6 +151,7 @@ def lib():
         "hive_set_stream_window": [p, C.c_uint64, C.c_uint64],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
         "hive_autotune_bulk": [p, p, p, sz, p, sz, C.POINTER(BulkTune), p],
+        "hive_autotune_layout": [p, p, p, sz, p, sz, C.POINTER(LayoutTune), p],
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],
         "hive_insert_sequence": [p, p, p, sz, sz, p, sz, C.c_double, C.c_double, i32, p, p,
                                  C.POINTER(SequenceReport), p],

@@ -691,3 +691,92 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   *out = best;
   return HIVE_OK;
 }
+
+// Layout choice: the sample insert, then the sample lookup, run on a copy of
+// the table in each layout; the layout with the least insert + lookup time
+// is applied to the table (its content kept, rehashed into that layout).
+extern "C" hive_status_t hive_autotune_layout(hive_table_t* t, const uint32_t* insert_keys,
+                                              const uint32_t* insert_vals, size_t n_insert,
+                                              const uint32_t* lookup_keys, size_t n_lookup, hive_layout_tune_t* out,
+                                              void* stream) {
+  hive_status_t r;
+  if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
+  if (!out || (n_insert && !insert_keys) || (n_lookup && !lookup_keys) || n_insert + n_lookup == 0)
+    return fail(HIVE_EINVAL, "layout tuning needs a non-empty sample and an output");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hive_layout_tune_t best{};
+  best.layout = layout_of(t->v);
+  double best_ms = 1e300;
+  for (int l = 0; l < 3; ++l) best.insert_ms[l] = best.lookup_ms[l] = -1.0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {
+    hipEvent_t* a;
+    hipEvent_t* b;
+    ~Events() {
+      if (*a) (void)hipEventDestroy(*a);
+      if (*b) (void)hipEventDestroy(*b);
+    }
+  } ev_guard{&e0, &e1};
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto timed = [&](auto&& launch, double* ms) -> hive_status_t {
+    HIP_TRY(hipEventRecord(e0, s));
+    hive_status_t rr = launch();
+    if (rr != HIVE_OK) return rr;
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    return HIVE_OK;
+  };
+  DevArena ar;
+  uint8_t* st8 = nullptr;
+  uint32_t* v32 = nullptr;
+  HIP_TRY(ar.alloc(&st8, std::max<size_t>(std::max(n_insert, n_lookup), 1)));
+  if (n_lookup) HIP_TRY(ar.alloc(&v32, n_lookup));
+  const int start = layout_of(t->v);
+  const int order[3] = {start, (start + 1) % 3, (start + 2) % 3};  // the current layout first: no rehash
+  for (int layout : order) {
+    r = hive_set_layout(t, layout, stream);
+    if (r == HIVE_ECAPACITY) continue;  // the content does not fit this layout: not a candidate
+    if (r != HIVE_OK) return r;
+    hive_table tmp = *t;  // scratch copy of the table state in this layout
+    if ((r = alloc_view(tmp.v, t->v.nb, layout)) != HIVE_OK) return r;
+    struct ViewGuard {
+      TableView* v;
+      ~ViewGuard() { free_view(*v); }
+    } guard{&tmp.v};
+    double ins = 1e300, look = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {  // keep the faster of two runs from the same state
+      if ((r = copy_view(tmp.v, t->v, s)) != HIVE_OK) return r;
+      double m = 0.0;
+      if (n_insert) {
+        r = timed([&] { return launch_insert(&tmp, SrcNative{insert_keys, insert_vals, nullptr}, n_insert,
+                                             OutNative{st8, nullptr}, -1, s); },
+                  &m);
+        if (r != HIVE_OK) return r;
+        ins = std::min(ins, m);
+      }
+      if (n_lookup) {
+        r = timed([&] { return launch_lookup(&tmp, SrcNative{lookup_keys, nullptr, nullptr}, n_lookup,
+                                             OutNative{st8, v32}, -1, s); },
+                  &m);
+        if (r != HIVE_OK) return r;
+        look = std::min(look, m);
+      }
+    }
+    ++best.candidates;
+    best.insert_ms[layout] = n_insert ? ins : 0.0;
+    best.lookup_ms[layout] = n_lookup ? look : 0.0;
+    const double total = best.insert_ms[layout] + best.lookup_ms[layout];
+    if (total < best_ms) {
+      best_ms = total;
+      best.layout = layout;
+    }
+  }
+  if (best.candidates == 0) return fail(HIVE_ECAPACITY, "the table's content fits no layout at this size");
+  if ((r = hive_set_layout(t, best.layout, stream)) != HIVE_OK) return r;
+  *out = best;
+  return HIVE_OK;
+}

@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BulkTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, LAYOUTS, POLICIES,
+from ._lib import (BulkTune, LayoutTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, LAYOUTS, POLICIES,
                    check, lib,
                    OP_INSERT,
                    OP_LOOKUP, OP_DELETE)
@@ -345,6 +345,25 @@ class HiveTable:
         out["lookup_ms_by_lanes"] = {1: res.lookup_ms_by_lanes[0], 4: res.lookup_ms_by_lanes[1],
                                      8: res.lookup_ms_by_lanes[2]}
         return out
+
+    def autotune_layout(self, insert_keys=None, insert_vals=None, lookup_keys=None, stream=None):
+        """Time the insert sample then the lookup sample on a copy of the table
+        in each layout and switch the table to the layout with the least total
+        time (content kept).  Returns the chosen layout's name and the times
+        by layout name (None where the content did not fit)."""
+        torch = _torch()
+        ik = _dev(insert_keys, torch.int32) if insert_keys is not None else None
+        iv = _dev(insert_vals, torch.int32) if insert_vals is not None else None
+        lk = _dev(lookup_keys, torch.int32) if lookup_keys is not None else None
+        res = LayoutTune()
+        check(self.L.hive_autotune_layout(self.h, _ptr(ik) if ik is not None else None,
+                                          _ptr(iv) if iv is not None else None, ik.numel() if ik is not None else 0,
+                                          _ptr(lk) if lk is not None else None, lk.numel() if lk is not None else 0,
+                                          C.byref(res), _stream_ptr(stream)), "hive_autotune_layout")
+        names = {v: k for k, v in LAYOUTS.items()}
+        return {"layout": names[res.layout], "candidates": res.candidates,
+                "insert_ms": {names[i]: (res.insert_ms[i] if res.insert_ms[i] >= 0 else None) for i in range(3)},
+                "lookup_ms": {names[i]: (res.lookup_ms[i] if res.lookup_ms[i] >= 0 else None) for i in range(3)}}
 
 
 def make_ops(types, keys):

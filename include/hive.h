@@ -270,6 +270,23 @@ hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, c
                                  size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup, hive_bulk_tune_t* out,
                                  void* stream);
 
+/* Bucket layout chosen by measurement (no reference counterpart: the
+ * reference has one layout).  For each layout (HIVE_LAYOUT_*) the table's
+ * content is rehashed into that layout, the insert sample is inserted into a
+ * copy of it, and the lookup sample is looked up in the result (the faster
+ * of two runs each); the layout with the least insert + lookup time is then
+ * applied to the table, its content kept.  Times are indexed by layout id;
+ * -1 marks a layout the content did not fit.  Synchronizes. */
+typedef struct {
+  int layout;            /* chosen and applied */
+  double insert_ms[3];   /* by layout id (device clock) */
+  double lookup_ms[3];
+  uint32_t candidates;
+} hive_layout_tune_t;
+hive_status_t hive_autotune_layout(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
+                                   size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup,
+                                   hive_layout_tune_t* out, void* stream);
+
 /* ---- serving: continuous batching over one resident table
  * Replaces the reference's YCSB driver (src/hashTableBenchmark.cc:434-570,
  * src/hive_dispatch.cu:317-549), which runs a pre-built workload as one

@@ -691,6 +691,33 @@ def test_bulk_autotune_leaves_table_unchanged(H, O):
     assert np.array_equal(v, ref_v) and np.array_equal(st, ref_s)
 
 
+@pytest.mark.parametrize("start", ["bucket8", "inline8"])
+def test_layout_autotune_keeps_content(H, O, start):
+    """The layout tuner times the samples in every layout and applies the
+    fastest: every layout is timed, the chosen one is the table's layout
+    afterwards, its total time is the least, and the live entries and lookup
+    results are those of before."""
+    print(f"[case] layout autotune from {start}", flush=True)
+    slots = 1 << 18
+    keys = W.distinct_keys(int(slots * 0.9), 83)
+    t = H.HiveTable(slots, layout=start)
+    t.insert(keys[: keys.size // 2])
+    before = gpu_raw(t, O).live_kv()
+    lk, _ = W.lookup_set(keys, keys.size // 2, 0.5, 84)
+    ref_v, ref_s = (np_(x) for x in t.lookup(lk))
+    res = t.autotune_layout(keys[keys.size // 2:], None, lk)
+    assert res["candidates"] == 3
+    assert t.layout == H.LAYOUTS[res["layout"]] and t.slots_per_bucket == SPB[res["layout"]]
+    assert t.stats()["capacity"] == slots
+    tot = {n: res["insert_ms"][n] + res["lookup_ms"][n] for n in LAYOUT_NAMES}
+    assert all(v > 0 for v in tot.values())
+    assert tot[res["layout"]] == min(tot.values())
+    raw, _ = check_gpu_state(t, O, "Default2Hash")
+    assert np.array_equal(raw.live_kv(), before)
+    v, st = (np_(x) for x in t.lookup(lk))
+    assert np.array_equal(v, ref_v) and np.array_equal(st, ref_s)
+
+
 def test_windowed_stream_thousands_of_tiny_batches(H, O):
     """5000 batches of 2 ops over 4 keys: windows reach their 4096-batch cap,
     and each key has a deferred op in most batches of its window, so its
