@@ -270,13 +270,10 @@ struct DevArena {
   }
 };
 
+// every array of a view lives in one allocation, starting at v.slots (one
+// hipMalloc / hipFree per view: each costs 0.1-0.5 ms, a resize's stall)
 inline void free_view(TableView& v) {
   if (v.slots) (void)hipFree(v.slots);
-  if (v.stash) (void)hipFree(v.stash);
-  if (v.stash_count) (void)hipFree(v.stash_count);
-  if (v.meta) (void)hipFree(v.meta);
-  if (v.locks) (void)hipFree(v.locks);
-  if (v.counters) (void)hipFree(v.counters);
   v = TableView{};
 }
 
@@ -306,31 +303,38 @@ inline hive_status_t alloc_view(TableView& v, uint32_t nb, int layout) {
   v.nb = nb;
   v.S = layout_slots(layout);
   v.layout = layout == HIVE_LAYOUT_INLINE8 ? kLayoutInline : kLayoutSeparate;
-  hipError_t e = hipSuccess;
-  if (layout == HIVE_LAYOUT_INLINE8) {
-    v.pool_cap = pool_buckets(nb);
-    if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * kInlineStride * 8)) != hipSuccess ||
-        (e = hipMalloc(&v.stash, static_cast<size_t>(v.pool_cap) * kStashSlots * 8)) != hipSuccess ||
-        (e = hipMalloc(&v.counters, counter_bytes())) != hipSuccess) {
-      free_view(v);
-      (void)hipGetLastError();
-      return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
-                  std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    v.pool_next = reinterpret_cast<uint32_t*>(v.counters + counter_words());
-    return HIVE_OK;
-  }
-  if ((e = hipMalloc(&v.slots, static_cast<size_t>(nb) * v.S * 8)) != hipSuccess ||
-      (e = hipMalloc(&v.stash, static_cast<size_t>(nb) * kStashSlots * 8)) != hipSuccess ||
-      (e = hipMalloc(&v.stash_count, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.meta, meta_words(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.locks, static_cast<size_t>(nb) * 4)) != hipSuccess ||
-      (e = hipMalloc(&v.counters, counter_bytes())) != hipSuccess) {
-    free_view(v);
+  const bool inl = layout == HIVE_LAYOUT_INLINE8;
+  if (inl) v.pool_cap = pool_buckets(nb);
+  // carve: slots (or in-line lines), stash (or pool), then the separate
+  // layout's per-bucket arrays, counters last; every part 256-byte aligned
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t b_slots = static_cast<size_t>(nb) * (inl ? kInlineStride : v.S) * 8;
+  const size_t b_stash = static_cast<size_t>(inl ? v.pool_cap : nb) * kStashSlots * 8;
+  const size_t b_u32 = inl ? 0 : static_cast<size_t>(nb) * 4;
+  const size_t b_meta = inl ? 0 : meta_words(nb) * 4;
+  const size_t total = up(b_slots) + up(b_stash) + 2 * up(b_u32) + up(b_meta) + up(counter_bytes());
+  char* base = nullptr;
+  const hipError_t e = hipMalloc(&base, total);
+  if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE,
-                std::string("hipMalloc: ") + hipGetErrorString(e));
+    v = TableView{};
+    return fail(e == hipErrorOutOfMemory ? HIVE_ENOMEM : HIVE_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base + off;
+    off += up(bytes);
+    return p;
+  };
+  v.slots = reinterpret_cast<unsigned long long*>(take(b_slots));
+  v.stash = reinterpret_cast<unsigned long long*>(take(b_stash));
+  if (!inl) {
+    v.stash_count = reinterpret_cast<uint32_t*>(take(b_u32));
+    v.locks = reinterpret_cast<uint32_t*>(take(b_u32));
+    v.meta = reinterpret_cast<uint32_t*>(take(b_meta));
+  }
+  v.counters = reinterpret_cast<unsigned long long*>(take(counter_bytes()));
+  if (inl) v.pool_next = reinterpret_cast<uint32_t*>(v.counters + counter_words());
   return HIVE_OK;
 }
 
@@ -360,7 +364,7 @@ inline hive_status_t clear_view(const TableView& v, hipStream_t s) {
 // policy P and src's layout); failures counted in *failed
 template <class P>
 hive_status_t launch_rehash(const TableView& dst, const TableView& src, unsigned long long* failed, hipStream_t s) {
-  const size_t n = rehash_threads(src.nb, src.S);
+  const size_t n = rehash_threads(src.nb, src.S, src.layout == kLayoutInline);
   const unsigned g = grid_for(n, kDefaultBlock);
   switch (layout_of(src)) {
     case HIVE_LAYOUT_BUCKET16: k_rehash<P, LayoutOf<16, false>><<<g, kDefaultBlock, 0, s>>>(dst, src, failed); break;

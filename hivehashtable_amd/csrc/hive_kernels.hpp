@@ -487,9 +487,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_erase(TableView t, Src src, size_
 constexpr size_t kRehashRows = 64;      // source buckets per wavefront
 constexpr size_t kRehashChunk = 8192;   // rows per chunk
 __host__ __device__ inline size_t rehash_rows(uint32_t nb) { return (static_cast<size_t>(nb) + kRehashRows - 1) / kRehashRows; }
-// threads of a k_rehash launch over a source of nb buckets of ss slots
-inline size_t rehash_threads(uint32_t nb, uint32_t ss) {
-  return rehash_rows(nb) * (ss + kStashSlots) * kRehashRows;
+// threads of a k_rehash launch over a source of nb buckets of ss slots: one
+// per bucket slot, then one per stash slot (separate layout) or one per
+// bucket (in-line layout: its header is read once, its stash entries walked)
+inline size_t rehash_threads(uint32_t nb, uint32_t ss, bool inl) {
+  return rehash_rows(nb) * (ss + (inl ? 1 : kStashSlots)) * kRehashRows;
 }
 // the layout part of a policy (no hashes): a rehash's source
 template <int S, bool Inl>
@@ -508,7 +510,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
   const size_t rows = rehash_rows(src.nb), region0 = rows * SS;
   const size_t w = i / kRehashRows;
   unsigned long long kv = 0;
-  if (w < region0 + rows * kStashSlots) {
+  int dtab = 0, dstash = 0;
+  if constexpr (PS::kInline) {
+    // in-line source: one thread per bucket walks its pooled stash (rare:
+    // entries go straight through the inline slow path)
+    if (w >= region0) {
+      const size_t b = (w - region0) * kRehashRows + i % kRehashRows;
+      if (b < src.nb) {
+        const Hdr h = read_hdr<PS>(src, static_cast<uint32_t>(b));
+        const uint32_t cnt = stash_count_of<PS>(src, static_cast<uint32_t>(b), h);
+        const unsigned long long* sb = cnt ? stash_of<PS>(src, static_cast<uint32_t>(b), h) : nullptr;
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const unsigned long long e = sb[j];
+          if (e == 0ull) continue;
+          uint8_t r = fast_insert<P>(dst, kv_key(e), kv_val(e));
+          if (r == R_PENDING) r = slow_insert<P>(dst, kv_key(e), kv_val(e));
+          dtab += (r == R_INSERTED);
+          dstash += (r == R_STASHED);
+          if (r == R_FAILED) atomicAdd(failed, 1ull);
+        }
+      }
+    }
+  }
+  if (w < region0 + (PS::kInline ? 0 : rows * kStashSlots)) {
     const bool in_stash = w >= region0;
     const size_t wr = in_stash ? w - region0 : w;
     const size_t ns = in_stash ? kStashSlots : SS;  // slots per row of this region
@@ -526,7 +550,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
       }
     }
   }
-  int dtab = 0, dstash = 0;
   if (kv != 0ull) {
     const uint8_t r = fast_insert<P>(dst, kv_key(kv), kv_val(kv));
     if (r == R_PENDING) queue[atomicAdd(&qn, 1u)] = kv;
