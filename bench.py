@@ -54,13 +54,14 @@ def parse():
     p.add_argument("--stream-modes", default="launches,graph,persistent,windowed",
                    help="stream forms the mixed leg runs (comma list)")
     p.add_argument("--no-autotune", action="store_true",
-                   help="mixed leg: fixed shapes (--stream-shape) instead of the stream autotuner (profiling runs)")
+                   help="fixed shapes instead of the autotuners: the headline lookups keep the default shape and "
+                        "the mixed leg takes --stream-shape (profiling runs)")
     p.add_argument("--stream-shape", default="512:1:0:2097152:524288",
                    help="with --no-autotune: threads per block:inline slow path:split by type:window ops:"
                         "coalesce below ops")
     p.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
-    p.add_argument("--sweep", action="store_true", help="also run the LF x exist-ratio sweep (2 policies)")
-    p.add_argument("--growth", action="store_true", help="also run the in-place growth leg (2^22 -> 2^27 slots)")
+    p.add_argument("--no-sweep", action="store_true", help="skip the LF x exist-ratio sweep (2 policies)")
+    p.add_argument("--no-growth", action="store_true", help="skip the in-place growth leg (2^22 -> 2^27 slots)")
     p.add_argument("--serve-requests", type=int, default=1000)
     p.add_argument("--serve-offered-mops", type=float, default=4000.0,
                    help="offered load of the open-loop serving runs, million ops per second")
@@ -319,7 +320,13 @@ def sweep_leg(a, H, W, torch, dev):
     """cfg2: bulk insert at LF {0.65, 0.80, 0.90, 0.95, 1.0} of 2^26 slots (seed
     42 + i), then 2^26 lookups at exist ratio {0, 25, 50, 75, 100} %, for
     Default2Hash and MurmurCityHash.  Each time is the median of 3 runs;
-    every lookup result is checked (found exactly for placed keys)."""
+    every lookup result is checked (found exactly for placed keys).
+
+    The lookup sets are drawn on the device from one pool per load factor:
+    present keys sampled with replacement from the inserted keys, absent keys
+    continuing the inserted keys' stream (absent by construction), a prefix
+    of each concatenated and permuted (torch generator seeded 2042 + i), as
+    src/hashTableBenchmark.cc:289-431 builds them from a hit and a miss set."""
     slots = 1 << 26
     stream = torch.cuda.current_stream()
     tables = {p: H.HiveTable(slots, p) for p in ("Default2Hash", "MurmurCityHash")}
@@ -331,42 +338,56 @@ def sweep_leg(a, H, W, torch, dev):
         t0 = time.time()
         n = W.keys_for_lf(slots, lf)
         keys = W.distinct_keys(n, 42 + i)
+        miss = W.distinct_keys(slots, 42 + i, start=n + (n >> 6) + 64)
         d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
         d_v = torch.from_numpy(W.values(n, 1042 + i).view(np.int32)).to(dev)
+        d_miss = torch.from_numpy(miss.view(np.int32)).to(dev)
+        g = torch.Generator(device=dev)
+        g.manual_seed(2042 + i)
+        d_hit = d_k[torch.randint(0, n, (slots,), device=dev, generator=g)]
         sets = {}
         for x in (0.0, 0.25, 0.5, 0.75, 1.0):
-            lk, present = W.lookup_set(keys, slots, x, 2042 + i, key_seed=42 + i)
-            sets[x] = (torch.from_numpy(lk.view(np.int32)).to(dev), lk, present)
+            n_hit = int(round(slots * x))
+            perm = torch.randperm(slots, device=dev, generator=g)
+            cat = torch.cat([d_hit[:n_hit], d_miss[:slots - n_hit]])
+            pres = torch.cat([torch.ones(n_hit, dtype=torch.bool, device=dev),
+                              torch.zeros(slots - n_hit, dtype=torch.bool, device=dev)])
+            sets[x] = (cat[perm].contiguous(), pres[perm].contiguous())
+        del d_hit, d_miss
+        torch.cuda.synchronize()
         log(f"sweep LF {lf}: inputs in {time.time() - t0:.1f}s")
         for pol, t in tables.items():
             def ins():
                 t.clear()
                 t.insert(d_k, d_v, status=d_st[:n])
             ins_ms = _timed(torch, stream, ins)
-            st = d_st[:n].cpu().numpy()
-            placed = np.isin(st, (1, 3))
+            placed = torch.isin(d_st[:n], torch.tensor([1, 3], dtype=torch.uint8, device=dev))
             stats = t.stats()
-            failed = keys[~placed]
+            d_failed = d_k[~placed]
             row = {"inserts": n, "insert_ms": round(ins_ms, 4), "insert_mops": round(n / ins_ms / 1e3, 1),
-                   "failed": int((~placed).sum()), "stash_entries": stats["stash_live"],
+                   "failed": int(d_failed.numel()), "stash_entries": stats["stash_live"],
                    "slow_path_frac": round(stats["slow_path_ops"] / n, 5), "lookups": {}}
-            for x, (d_lk, lk, present) in sets.items():
+            for x, (d_lk, pres) in sets.items():
                 look_ms = _timed(torch, stream, lambda: t.lookup(d_lk, vals=d_lv, status=d_ls))
-                found = d_ls.cpu().numpy() == 4
-                expect = present & ~np.isin(lk, failed) if failed.size else present
+                expect = pres & ~torch.isin(d_lk, d_failed) if d_failed.numel() else pres
                 row["lookups"][f"{x:.2f}"] = {"ms": round(look_ms, 4), "mops": round(slots / look_ms / 1e3, 1),
-                                              "verified": bool(np.array_equal(found, expect))}
-            # lookup layout at this load factor: the bulk autotuner's lane-per-key
-            # vs 4 / 8 lanes per key on the 50 % exist-ratio set (then reset)
+                                              "verified": bool(torch.equal(d_ls == 4, expect))}
+            # lookup shape at this load factor: the bulk autotuner's lane-per-key,
+            # 4 / 8 lanes per key and 2 / 4 keys per lane on the 50 % exist-ratio
+            # set (then reset)
             bt = t.autotune_bulk(None, None, sets[0.5][0])
             row["lookup_layout"] = {"chosen_lanes_per_key": bt["lookup_lanes_per_key"],
+                                    "chosen_keys_per_lane": bt["lookup_keys_per_lane"],
                                     "chosen_block_threads": bt["lookup_block_threads"],
                                     "ms_by_lanes_per_key": {str(k): round(v, 4)
-                                                            for k, v in bt["lookup_ms_by_lanes"].items()}}
+                                                            for k, v in bt["lookup_ms_by_lanes"].items()},
+                                    "ms_by_keys_per_lane": {str(k): round(v, 4)
+                                                            for k, v in bt["lookup_ms_by_keys"].items()}}
             t.set_launch(1, 256)
             out[pol][f"{lf:.2f}"] = row
             log(f"sweep {pol} LF {lf}: insert {row['insert_mops']} Mops/s, lookups "
                 f"{[v['mops'] for v in row['lookups'].values()]}, layout {row['lookup_layout']}")
+        del sets
     for t in tables.values():
         t.close()
     return out
@@ -546,6 +567,21 @@ def main():
     layout = {v: k for k, v in H.LAYOUTS.items()}[t.layout]
     spb = t.slots_per_bucket
     nb = t.num_buckets
+    shape_tune = None
+    if not a.block and not a.no_autotune:
+        # lookup shapes (block size, lanes per key, keys per lane) timed on the
+        # filled table, at the step's own load factor; the table is unchanged
+        t.clear()
+        t.insert(d_keys, d_vals, status=d_st)
+        bt = t.autotune_bulk(None, None, d_lk)
+        shape_tune = {"lookup_block_threads": bt["lookup_block_threads"],
+                      "lookup_lanes_per_key": bt["lookup_lanes_per_key"],
+                      "lookup_keys_per_lane": bt["lookup_keys_per_lane"],
+                      "lookup_ms": round(bt["lookup_ms"], 4),
+                      "lookup_ms_by_lanes_per_key": {str(k): round(v, 4) for k, v in bt["lookup_ms_by_lanes"].items()},
+                      "lookup_ms_by_keys_per_lane": {str(k): round(v, 4) for k, v in bt["lookup_ms_by_keys"].items()},
+                      "candidates": bt["candidates"]}
+        log(f"lookup shape tune: {shape_tune}")
 
     def step(ev=None):
         if ev is not None:
@@ -663,8 +699,8 @@ def main():
     serving = None
     if not a.no_serve:
         serving = serve_leg(a, H, W, torch, dev)
-    sweep = sweep_leg(a, H, W, torch, dev) if a.sweep else None
-    growth = growth_leg(a, H, W, torch, dev) if a.growth else None
+    sweep = sweep_leg(a, H, W, torch, dev) if not a.no_sweep else None
+    growth = growth_leg(a, H, W, torch, dev) if not a.no_growth else None
 
     cpu = None
     if not a.no_cpu:
@@ -698,6 +734,7 @@ def main():
         "kernel_ms": {"clear": round(clear_ms, 4), "insert": round(ins_ms, 4), "lookup": round(look_ms, 4)},
         "verified": verified,
         "layout_tune": layout_tune,
+        "lookup_shape_tune": shape_tune,
         "insert_slow_path_frac": round(slow_frac, 5),
         "stash_entries": tstats["stash_live"],
         "roofline": {

@@ -68,7 +68,8 @@ class BulkTune(C.Structure):
     _fields_ = [("insert_block_threads", C.c_uint32), ("insert_inline_slow_path", C.c_uint32),
                 ("lookup_block_threads", C.c_uint32), ("lookup_lanes_per_key", C.c_uint32),
                 ("insert_ms", C.c_double), ("lookup_ms", C.c_double), ("lookup_ms_by_lanes", C.c_double * 3),
-                ("load_factor", C.c_double), ("candidates", C.c_uint32)]
+                ("load_factor", C.c_double), ("candidates", C.c_uint32), ("lookup_keys_per_lane", C.c_uint32),
+                ("lookup_ms_by_keys", C.c_double * 3)]
 
 
 class LayoutTune(C.Structure):

@@ -544,7 +544,9 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (c.block_threads == 0) c.block_threads = kDefaultBlock;
   if (c.keys_per_lane == 0) c.keys_per_lane = 1;
   if (!valid_block(c.block_threads)) return fail(HIVE_EINVAL, "block_threads must be a multiple of 64 in 64..1024");
-  if (c.keys_per_lane != 1) return fail(HIVE_EINVAL, "keys_per_lane must be 1");
+  if (c.keys_per_lane != 1 && c.keys_per_lane != 2 && c.keys_per_lane != 4)
+    return fail(HIVE_EINVAL, "keys_per_lane must be 1, 2 or 4");
+  if (c.keys_per_lane != 1 && op_type != OP_LOOKUP) return fail(HIVE_EINVAL, "keys_per_lane > 1 is for lookups");
   if (c.inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
   if (c.combine_duplicates > 1) return fail(HIVE_EINVAL, "combine_duplicates must be 0 or 1");
   if (c.split_by_type > 1) return fail(HIVE_EINVAL, "split_by_type must be 0 or 1");
@@ -552,6 +554,8 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (c.lanes_per_key != 1 && c.lanes_per_key != 4 && c.lanes_per_key != 8)
     return fail(HIVE_EINVAL, "lanes_per_key must be 1, 4 or 8");
   if (c.lanes_per_key != 1 && op_type != OP_LOOKUP) return fail(HIVE_EINVAL, "lanes_per_key > 1 is for lookups");
+  if (c.lanes_per_key != 1 && c.keys_per_lane != 1)
+    return fail(HIVE_EINVAL, "lanes_per_key > 1 and keys_per_lane > 1 exclude each other");
   t->launch[op_type] = c;
   return HIVE_OK;
 }

@@ -467,6 +467,30 @@ __device__ __forceinline__ Hdr read_hdr_sync(const TableView& t, uint32_t b) {
     return Hdr{meta_bits(ld_sync32(t.meta + b / kHintsPerWord), b), 0u, 0u};
   }
 }
+// Two-hash policies in the in-line layout publish a locked section's hint bits
+// with the unlock.  There the hint word of bucket c is written only by holders
+// of lock(c): a kick moves a victim out of c either to its H1 (its H0 is c,
+// whose bit is set) or back to its own H0 (no bit), and stash pushes to c set
+// c's stash bit.  The hint (word 0) and the lock (word 1) share the header's
+// first 8 bytes, so one 8-byte store releases the lock and sets the bits: the
+// next holder's coherent header read sees them.  Until then, inserts of c's
+// displaced or stashed keys cannot go wrong for want of the bits: c is full
+// (kicks and pushes happen only to a full c, and slots fill but never empty
+// within an insert phase), so their fast path's claim in c fails and they
+// take lock(c).  Lookups and erases run in later kernels.  With three hashes a
+// kick from c can set the bit of another victim's H0, which that bucket's lock
+// holder does not own; those keep the separate atomic.
+template <class P>
+__host__ __device__ constexpr bool hint_at_unlock() {
+  return P::kInline && P::NH == 2;
+}
+template <class P>
+__device__ __forceinline__ void unlock_with_hint(const TableView& t, uint32_t c, uint32_t hint) {
+  wait_vmem();  // the critical section's atomics and coherent stores are performed
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(header_ptr<P>(t, c)), static_cast<unsigned long long>(hint),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // hint | lock 0 << 32
+}
+
 template <class P>
 __device__ __forceinline__ void hint_set(const TableView& t, uint32_t b, uint32_t bits) {
   if constexpr (P::kInline)
@@ -604,9 +628,14 @@ __device__ __forceinline__ bool finish_move(unsigned long long* src, unsigned lo
 // pre_victim / pre_alt: slot 0 of H0 as read before the lock and its
 // alternate bucket (~0u if none); that bucket is read in the same round as the
 // coherent reload of H0, so the first kick usually needs no extra round trip.
+// Under hint_at_unlock<P>() the hint bits the section adds to c are returned
+// in `pend` (with the last coherent read of c's hint in `hint_now`) instead of
+// being set with an atomic of their own; slow_insert publishes them with the
+// unlock.
 template <class P>
 __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val, uint32_t c,
-                                 unsigned long long pre_victim, uint32_t pre_alt) {
+                                 unsigned long long pre_victim, uint32_t pre_alt, uint32_t& pend,
+                                 uint32_t& hint_now) {
   constexpr int S = P::kS;
   constexpr int NV = kMaxKicks < S ? kMaxKicks : S;  // the kicks' victims: slots 0 .. NV-1
   const unsigned long long kv = kv_pack(key, val);
@@ -644,6 +673,7 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       for (int q = 0; q < NV; ++q) vic[q] = cs[q];
     }
     const uint32_t hint = h.hint;
+    hint_now = hint;
     if (m) {
       if (update_slot(cb + __builtin_ctz(m), cur, kv)) return R_UPDATED;
       continue;
@@ -705,7 +735,10 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       // before the victim can leave its H0 (already set when vh0 == c and the
       // coherent hint read above saw it)
       const uint32_t vbit = ovf_bit_of<P>(vkey);
-      if (a != vh0 && !(vh0 == c && (hint & vbit))) hint_set<P>(t, vh0, vbit);
+      if (a != vh0 && !(vh0 == c && ((hint | pend) & vbit))) {
+        if constexpr (hint_at_unlock<P>()) pend |= vbit;  // two hashes: a != vh0 means vh0 == c
+        else hint_set<P>(t, vh0, vbit);
+      }
       int d = -1;
       while (ae) {
         const int j = __builtin_ctz(ae);
@@ -728,7 +761,8 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       }
       __hip_atomic_store(stash_of<P>(t, c, h) + cnt, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(count_ptr<P>(t, c), cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      hint_set<P>(t, c, stash_bit<P>());
+      if constexpr (hint_at_unlock<P>()) pend |= stash_bit<P>();
+      else hint_set<P>(t, c, stash_bit<P>());
       return R_STASHED;
     }
     return R_FAILED;
@@ -755,8 +789,10 @@ __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
         pa = alt_bucket<P>(kv_key(ps0), c, t.nb);
         if (pa == c) pa = ~0u;
       }
-      result = locked_insert<P>(t, key, val, c, ps0, pa);
-      unlock(lock_ptr<P>(t, c));
+      uint32_t pend = 0, hint_now = 0;
+      result = locked_insert<P>(t, key, val, c, ps0, pa, pend, hint_now);
+      if (hint_at_unlock<P>() && pend) unlock_with_hint<P>(t, c, hint_now | pend);
+      else unlock(lock_ptr<P>(t, c));
     } else {
       for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
       backoff = backoff < 64 ? backoff * 2 : 64;

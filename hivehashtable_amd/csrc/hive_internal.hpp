@@ -123,8 +123,15 @@ hive_status_t launch_lookup(hive_table* t, Src src, size_t n, Out out, int phase
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_LOOKUP].block_threads;
   const uint32_t lanes = t->launch[OP_LOOKUP].lanes_per_key > 1 ? t->launch[OP_LOOKUP].lanes_per_key : 1;
+  const uint32_t kpl = t->launch[OP_LOOKUP].keys_per_lane > 1 ? t->launch[OP_LOOKUP].keys_per_lane : 1;
   if (!fits_one_launch(n * lanes, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
-  if (lanes == 8) {
+  if (kpl == 4) {
+    HIVE_POLICY_SWITCH(table_kind(t), (k_lookup_multi<P, Src, Out, 4><<<grid_for(n, blk * 4), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                    phase)));
+  } else if (kpl == 2) {
+    HIVE_POLICY_SWITCH(table_kind(t), (k_lookup_multi<P, Src, Out, 2><<<grid_for(n, blk * 2), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                    phase)));
+  } else if (lanes == 8) {
     HIVE_POLICY_SWITCH(table_kind(t), (k_lookup_coop<P, Src, Out, 8><<<grid_for(n * 8, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                    phase)));
   } else if (lanes == 4) {

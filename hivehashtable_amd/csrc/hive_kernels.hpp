@@ -360,6 +360,115 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
   out.put(i, r, v);
 }
 
+// the value of a bucket's lowest slot in match mask m (m != 0), by selects
+// over the unrolled slots: a dynamic index into a register array would put
+// the bucket in scratch memory
+template <int S>
+__device__ __forceinline__ uint32_t lowest_match_val(const unsigned long long (&s)[S], uint32_t m) {
+  const int j = __builtin_ctz(m);
+  uint32_t v = 0;
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+    if (q == j) v = kv_val(s[q]);
+  return v;
+}
+
+// K keys per lane (K = 2 or 4): each lane runs lookup_one's probe sequence
+// for K keys at once, issuing a round's bucket reads for all of them before
+// consuming any, so a wavefront keeps K times as many random reads in flight
+// and pays the key load, launch and output once per K keys.  Key k of a
+// lane is op blockIdx.x * blockDim.x * K + k * blockDim.x + threadIdx.x,
+// so key loads and result stores stay coalesced.
+template <class P, class Src, class Out, int K>
+__global__ __launch_bounds__(kMaxBlock) void k_lookup_multi(TableView t, Src src, size_t n, Out out, int phase) {
+  const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x * K + threadIdx.x;
+  uint32_t key[K], val[K], b0[K];
+  bool open[K], act[K];
+  uint8_t res[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const size_t i = base + static_cast<size_t>(k) * blockDim.x;
+    act[k] = i < n;
+    key[k] = act[k] ? src.key(i) : 0u;  // requested together with the op type
+    if (act[k] && phase >= 0) {
+      const int ty = src.type(i, OP_LOOKUP);
+      if (ty != OP_LOOKUP) {
+        if (ty != OP_INSERT && ty != OP_DELETE && ty != src.deferred_type()) out.put(i, R_NOOP, 0);
+        act[k] = false;
+      }
+    }
+    val[k] = 0;
+    res[k] = R_NOT_FOUND;
+    open[k] = act[k] && key[k] != 0u;
+    b0[k] = bucket_of<P>(0, key[k], t.nb);
+  }
+  {  // round 1: every key's H0
+    unsigned long long s[K][P::kS];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (open[k]) read_bucket<P>(bucket_ptr<P>(t, b0[k]), s[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!open[k]) continue;
+      const uint32_t m = match_mask(s[k], key[k]);
+      if (m) {
+        val[k] = lowest_match_val(s[k], m);
+        res[k] = R_FOUND;
+        open[k] = false;
+      }
+    }
+  }
+  // round 2: the hint words of the keys H0 missed (read only after the miss,
+  // as in lookup_one)
+  Hdr h[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) h[k] = open[k] ? read_hdr<P>(t, b0[k]) : Hdr{0u, 0u, 0u};
+  // rounds 3..: the other candidates of the keys whose overflow class is set
+#pragma unroll
+  for (int c = 1; c < P::NH; ++c) {
+    unsigned long long s[K][P::kS];
+    bool probe[K];
+    uint32_t bc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      bc[k] = bucket_of<P>(c, key[k], t.nb);
+      bool dup = false;
+#pragma unroll
+      for (int j = 0; j < c; ++j) dup |= bucket_of<P>(j, key[k], t.nb) == bc[k];
+      probe[k] = open[k] && (h[k].hint & ovf_bit_of<P>(key[k])) && !dup;
+      if (probe[k]) read_bucket<P>(bucket_ptr<P>(t, bc[k]), s[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!probe[k]) continue;
+      const uint32_t m = match_mask(s[k], key[k]);
+      if (m) {
+        val[k] = lowest_match_val(s[k], m);
+        res[k] = R_FOUND;
+        open[k] = false;
+      }
+    }
+  }
+  // the canonical stash (rare: only buckets whose stash bit is set)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!open[k] || !(h[k].hint & stash_bit<P>())) continue;
+    const uint32_t cnt = stash_count_of<P>(t, b0[k], h[k]);
+    const unsigned long long* sb = stash_of<P>(t, b0[k], h[k]);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const unsigned long long cur = read_u64<P>(sb + j);
+      if (cur != 0ull && kv_key(cur) == key[k]) {
+        val[k] = kv_val(cur);
+        res[k] = R_FOUND;
+        break;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (act[k]) out.put(base + static_cast<size_t>(k) * blockDim.x, res[k], val[k]);
+}
+
 // ------------------------------------------------------ cooperative lookup
 // L lanes per key (L = 4 or 8, a group of consecutive lanes of one
 // wavefront): lane r of a group loads slots [r*S/L, (r+1)*S/L) of each probed

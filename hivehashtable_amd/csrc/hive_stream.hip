@@ -593,6 +593,7 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   hive_bulk_tune_t best{};
   best.insert_ms = best.lookup_ms = 1e300;
   for (double& v : best.lookup_ms_by_lanes) v = 1e300;
+  for (double& v : best.lookup_ms_by_keys) v = 1e300;
   hive_stats_t st{};
   if ((r = hive_stats(t, &st, stream)) != HIVE_OK) return r;
   best.load_factor = st.load_factor;
@@ -653,10 +654,13 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
     HIP_TRY(ar.alloc(&st8, n_lookup));
     HIP_TRY(ar.alloc(&v32, n_lookup));
     const hive_launch_t saved = t->launch[OP_LOOKUP];
-    const uint32_t lanes[] = {1, 4, 8};
-    for (int li = 0; li < 3; ++li)
+    // (lanes per key, keys per lane): the lane-per-key form, the
+    // bucket-cooperative forms, and lane-per-key with several keys per lane
+    const uint32_t lanes[] = {1, 4, 8, 1, 1};
+    const uint32_t keys[] = {1, 1, 1, 2, 4};
+    for (int li = 0; li < 5; ++li)
       for (uint32_t blk : blocks) {
-        t->launch[OP_LOOKUP] = hive_launch_t{blk, 1, 0, 0, saved.split_by_type, lanes[li]};
+        t->launch[OP_LOOKUP] = hive_launch_t{blk, keys[li], 0, 0, saved.split_by_type, lanes[li]};
         double ms = 1e300;
         for (int rep = 0; rep < 3; ++rep) {  // read-only: repeat and keep the fastest
           double m = 0;
@@ -670,11 +674,16 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
           ms = std::min(ms, m);
         }
         ++cands;
-        best.lookup_ms_by_lanes[li] = std::min(best.lookup_ms_by_lanes[li], ms);
+        if (keys[li] == 1) best.lookup_ms_by_lanes[li] = std::min(best.lookup_ms_by_lanes[li], ms);
+        if (lanes[li] == 1) {
+          const int ki = keys[li] == 1 ? 0 : (keys[li] == 2 ? 1 : 2);
+          best.lookup_ms_by_keys[ki] = std::min(best.lookup_ms_by_keys[ki], ms);
+        }
         if (ms < best.lookup_ms) {
           best.lookup_ms = ms;
           best.lookup_block_threads = blk;
           best.lookup_lanes_per_key = lanes[li];
+          best.lookup_keys_per_lane = keys[li];
         }
       }
     t->launch[OP_LOOKUP] = saved;
@@ -687,6 +696,7 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   if (n_lookup) {
     t->launch[OP_LOOKUP].block_threads = best.lookup_block_threads;
     t->launch[OP_LOOKUP].lanes_per_key = best.lookup_lanes_per_key;
+    t->launch[OP_LOOKUP].keys_per_lane = best.lookup_keys_per_lane;
   }
   *out = best;
   return HIVE_OK;

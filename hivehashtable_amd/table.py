@@ -344,6 +344,8 @@ class HiveTable:
         out = {f: getattr(res, f) for f, _ in BulkTune._fields_}
         out["lookup_ms_by_lanes"] = {1: res.lookup_ms_by_lanes[0], 4: res.lookup_ms_by_lanes[1],
                                      8: res.lookup_ms_by_lanes[2]}
+        out["lookup_ms_by_keys"] = {1: res.lookup_ms_by_keys[0], 2: res.lookup_ms_by_keys[1],
+                                    4: res.lookup_ms_by_keys[2]}
         return out
 
     def autotune_layout(self, insert_keys=None, insert_vals=None, lookup_keys=None, stream=None):

@@ -105,7 +105,10 @@ typedef struct {
 /* Launch shape of the plain bulk kernels.  0 = library default / autotuned. */
 typedef struct {
   uint32_t block_threads;    /* threads per block: 64..1024, multiple of 64 */
-  uint32_t keys_per_lane;    /* keys processed per lane (1) */
+  uint32_t keys_per_lane;    /* lookups: 0 or 1 = one key per lane; 2 or 4 = each
+                                lane probes that many keys at once, issuing a
+                                round's bucket reads for all of them together
+                                (same results); other ops: 1 */
   uint32_t inline_slow_path; /* inserts: 0 = stage slow-path ops in LDS behind one
                                 block barrier (throughput), 1 = each lane runs its
                                 own slow path at once (latency) */
@@ -251,8 +254,9 @@ hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const 
  * current load factor: times the insert of the sample keys (device arrays,
  * vals may be NULL) on a scratch copy of the table at 256/512/1024 threads
  * per block x {staged, inline} slow path, and the lookup of the sample lookup
- * keys on the table itself at 256/512/1024 threads x {1, 4, 8} lanes per key,
- * then stores the fastest insert and lookup shapes in the handle.  The table
+ * keys on the table itself at 256/512/1024 threads x {1, 4, 8} lanes per key
+ * and x {2, 4} keys per lane, then stores the fastest insert and lookup shapes
+ * in the handle.  The table
  * is left unchanged.  Either sample may be empty (n = 0: that op is not
  * tuned).  Synchronizes. */
 typedef struct {
@@ -265,6 +269,8 @@ typedef struct {
   double lookup_ms_by_lanes[3];   /* best lookup time with 1, 4 and 8 lanes per key */
   double load_factor;             /* of the table while tuned */
   uint32_t candidates;
+  uint32_t lookup_keys_per_lane;  /* chosen keys per lane (1 with lanes_per_key > 1) */
+  double lookup_ms_by_keys[3];    /* best lookup time with 1, 2 and 4 keys per lane (one lane per key) */
 } hive_bulk_tune_t;
 hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
                                  size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup, hive_bulk_tune_t* out,

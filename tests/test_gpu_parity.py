@@ -101,11 +101,13 @@ def test_lookup_and_erase_on_oracle_built_table(H, O, policy, layout):
     t.import_raw(raw.slots, raw.stash, raw.stash_count)
     probe = np.concatenate([keys, W.distinct_keys(5000, 13, exclude=keys), np.array([0], np.uint32)])
     ov, os_ = o.lookup(probe)
-    # lane-per-key and cooperative (4 or 8 lanes per key) lookups, two block sizes
-    for lanes, blk in ((1, 256), (4, 256), (8, 256), (4, 1024), (8, 64)):
-        t.set_launch(1, blk, lanes_per_key=lanes)
+    # lane-per-key, cooperative (4 or 8 lanes per key) and several keys per
+    # lane (2 or 4), block sizes whose last block is partly empty
+    for lanes, kpl, blk in ((1, 1, 256), (4, 1, 256), (8, 1, 256), (4, 1, 1024), (8, 1, 64), (1, 2, 256),
+                            (1, 4, 64), (1, 4, 1024), (1, 2, 512)):
+        t.set_launch(1, blk, keys_per_lane=kpl, lanes_per_key=lanes)
         gv, gs = t.lookup(probe)
-        assert np.array_equal(np_(gs), os_) and np.array_equal(np_(gv, np.uint32), ov), (lanes, blk)
+        assert np.array_equal(np_(gs), os_) and np.array_equal(np_(gv, np.uint32), ov), (lanes, kpl, blk)
     t.set_launch(1, 256)
     rng = np.random.default_rng(14)
     dk = np.concatenate([rng.choice(keys, 4000, replace=False), W.distinct_keys(500, 15, exclude=keys),
@@ -256,9 +258,11 @@ def test_big_mixed_batches_split_by_type(H, O):
         ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
     norm = lambda s: np.where(s == 3, 1, s)
     keyset = np.sort(o.dump() & 0xFFFFFFFF)
-    for how, split in (("mixed", 1), ("launches", 1), ("graph", 1), ("mixed", 0)):
+    for how, split, kpl in (("mixed", 1, 1), ("launches", 1, 1), ("graph", 1, 1), ("mixed", 0, 1), ("mixed", 1, 2),
+                            ("launches", 0, 4)):
         t = H.HiveTable(slots)
         t.set_launch(0, 256, split_by_type=split)
+        t.set_launch(1, 256, keys_per_lane=kpl)  # the lookup phase with several keys per lane
         t.insert(pre)
         if how == "mixed":
             outs = [t.mixed(types[int(off[b]):int(off[b + 1])], keys[int(off[b]):int(off[b + 1])],
@@ -678,14 +682,19 @@ def test_bulk_autotune_leaves_table_unchanged(H, O):
     lk, _ = W.lookup_set(keys, keys.size // 2, 0.5, 82)
     ref_v, ref_s = (np_(x) for x in t.lookup(lk))
     res = t.autotune_bulk(keys[keys.size // 2:], None, lk)
-    assert res["candidates"] == 15
+    assert res["candidates"] == 21
     assert res["insert_block_threads"] in (256, 512, 1024) and res["insert_inline_slow_path"] in (0, 1)
     assert res["lookup_block_threads"] in (256, 512, 1024) and res["lookup_lanes_per_key"] in (1, 4, 8)
+    assert res["lookup_keys_per_lane"] in (1, 2, 4)
+    assert res["lookup_lanes_per_key"] == 1 or res["lookup_keys_per_lane"] == 1
     assert res["load_factor"] == pytest.approx(0.45, abs=0.005)
-    assert min(res["lookup_ms_by_lanes"].values()) == pytest.approx(res["lookup_ms"])
+    best = min(min(res["lookup_ms_by_lanes"].values()), min(res["lookup_ms_by_keys"].values()))
+    assert best == pytest.approx(res["lookup_ms"])
+    assert res["lookup_ms_by_keys"][1] == pytest.approx(res["lookup_ms_by_lanes"][1])
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
     lp = t.get_launch(1)
-    assert (lp[0], lp[5]) == (res["lookup_block_threads"], res["lookup_lanes_per_key"])
+    assert (lp[0], lp[1], lp[5]) == (res["lookup_block_threads"], res["lookup_keys_per_lane"],
+                                     res["lookup_lanes_per_key"])
     assert t.get_launch(0)[0] == res["insert_block_threads"]
     v, st = (np_(x) for x in t.lookup(lk))
     assert np.array_equal(v, ref_v) and np.array_equal(st, ref_s)
