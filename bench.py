@@ -62,6 +62,10 @@ def parse():
     p.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the LF x exist-ratio sweep (2 policies)")
     p.add_argument("--no-growth", action="store_true", help="skip the in-place growth leg (2^22 -> 2^27 slots)")
+    p.add_argument("--layout-sweep", action="store_true",
+                   help="also run every bucket layout at LF 0.65 / 0.9 / 1.0 (insert, lookup, slow path, reads)")
+    p.add_argument("--layout-point", default="", metavar="LAYOUT:LF",
+                   help="only run one point of the layout sweep and print its JSON (profiling runs)")
     p.add_argument("--serve-requests", type=int, default=1000)
     p.add_argument("--serve-offered-mops", type=float, default=4000.0,
                    help="offered load of the open-loop serving runs, million ops per second")
@@ -187,18 +191,35 @@ def mixed_leg(a, H, W, torch, dev):
         t.mixed_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1], 3)
         torch.cuda.synchronize()
     else:
-        tune = t.autotune_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1])
+        # the library's default choice is bounded by the per-batch launches'
+        # p99 and leaves the persistent kernel out; it is timed here only to
+        # report it beside the other forms
+        tune = t.autotune_stream(d_ty[: int(off[ns])], d_k[: int(off[ns])], d_v[: int(off[ns])], off[: ns + 1],
+                                 try_persistent=True, max_p99_us=-1.0)
         log(f"autotune: {tune}")
     out = {"batches": len(sizes), "ops": n_ops, "preload_keys": int(pre.size), "key_space": 1 << 27,
            "batch_ops_min_max": [int(sizes.min()), int(sizes.max())], "mix_insert_lookup_delete": [0.5, 0.4, 0.1],
-           "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}, "modes": {}}
+           "autotune": {k: (round(v, 4) if isinstance(v, float) else
+                            ({str(kk): round(vv, 4) for kk, vv in v.items()} if isinstance(v, dict) else v))
+                        for k, v in tune.items()}, "modes": {}}
     ref = None
     agree = True
     names = {0: "launches", 1: "graph", 2: "persistent", 3: "windowed"}
-    wanted = [m for m, nm in names.items() if nm in a.stream_modes.split(",")]
-    for mode in wanted:
+    runs = [(m, names[m]) for m in names if names[m] in a.stream_modes.split(",")]
+    # the windows the autotuner keeps are those within the launches' p99; the
+    # fastest windows regardless of latency run as "windowed_throughput"
+    if (not a.no_autotune and 3 in dict(runs) and tune.get("bounded_window_ops")
+            and (tune["window_ops"], tune["window_batch_ops"]) != (tune["bounded_window_ops"],
+                                                                   tune["bounded_window_batch_ops"])):
+        runs.append((3, "windowed_throughput"))
+    for mode, name in runs:
         t.clear()
         t.insert(d_pre)
+        if name == "windowed_throughput":
+            t.set_stream_window(tune["window_ops"], tune["window_batch_ops"])
+        elif mode == 3 and not a.no_autotune:
+            t.set_stream_window(tune["bounded_window_ops"] or tune["window_ops"],
+                                tune["bounded_window_batch_ops"] or tune["window_batch_ops"])
         torch.cuda.synchronize()
         gv, gs, done, setup = t.mixed_stream(d_ty, d_k, d_v, off, mode)
         # per-batch service time, us: from the previous completion (of another
@@ -213,11 +234,17 @@ def mixed_leg(a, H, W, torch, dev):
         else:
             same = bool(torch.equal(st[d_clean], ref[0][d_clean]) and torch.equal(gv[d_clean], ref[1][d_clean]))
             agree &= same
-        out["modes"][names[mode]] = {
+        out["modes"][name] = {
             "total_ms": round(total_ms, 3), "mops": round(n_ops / total_ms / 1e3, 2),
             "p50_batch_us": round(float(np.percentile(lat, 50)), 2), "p99_batch_us": round(float(np.percentile(lat, 99)), 2),
             "setup_ms": round(setup, 3)}
-        log(f"mixed {names[mode]}: {out['modes'][names[mode]]}")
+        if mode == 3 and not a.no_autotune:
+            bounded = name == "windowed" and tune.get("bounded_window_ops")
+            out["modes"][name]["window_ops"] = tune["bounded_window_ops"] if bounded else tune["window_ops"]
+            out["modes"][name]["window_batch_ops"] = (tune["bounded_window_batch_ops"] if bounded
+                                                      else tune["window_batch_ops"])
+        log(f"mixed {name}: {out['modes'][name]}")
+    out["chosen_form"] = names.get(tune.get("best_mode")) if tune and "best_mode" in tune else None
     out["modes_agree"] = agree
     t.close()
     return out
@@ -278,9 +305,10 @@ def serve_leg(a, H, W, torch, dev):
         # own mixed batch (plain per-batch launches, no coalescing)
         # "saturated_ordered": every request sees the earlier requests' writes
         # inside a batch too (windowed batches, one request per window batch)
-        res = {"ops": int(off[-1]), "open": run(t, ty, k, v, off, arr),
-               "saturated": run(t, ty, k, v, off, np.zeros_like(arr)),
+        res = {"ops": int(off[-1]), "open_ordered": run(t, ty, k, v, off, arr, ordered=1),
                "saturated_ordered": run(t, ty, k, v, off, np.zeros_like(arr), ordered=1),
+               "open": run(t, ty, k, v, off, arr),
+               "saturated": run(t, ty, k, v, off, np.zeros_like(arr)),
                "per_request_launches": run(t, ty, k, v, off, np.zeros_like(arr), max_batch_ops=1)}
         res["coalescing_speedup"] = round(res["saturated"]["mops"] / max(res["per_request_launches"]["mops"], 1e-9), 2)
         out["mixes"][mix] = res
@@ -290,16 +318,38 @@ def serve_leg(a, H, W, torch, dev):
                                          fresh=fresh)
     res = {"ops": int(off[-1])}
     slots_d = int(math.ceil(n_pre / 0.899 / 8)) * 8
-    for name, arrivals, cap, ordered in (("open", arr, 0, 0), ("saturated", np.zeros_like(arr), 0, 0),
-                                         ("saturated_ordered", np.zeros_like(arr), 0, 1),
-                                         ("per_request_launches", np.zeros_like(arr), 1, 0)):
-        t = serving_table(slots_d)
-        res["initial_slots"] = slots_d
+    res["initial_slots"] = slots_d
+    # "open_ordered_no_resize": the same trace on a table with room for every
+    # insert (2^27 slots), so that no resize stalls the queue
+    for name, arrivals, cap, ordered, slots in (("open_ordered", arr, 0, 1, slots_d),
+                                                ("open_ordered_no_resize", arr, 0, 1, 1 << 27),
+                                                ("saturated_ordered", np.zeros_like(arr), 0, 1, slots_d),
+                                                ("open", arr, 0, 0, slots_d),
+                                                ("saturated", np.zeros_like(arr), 0, 0, slots_d),
+                                                ("per_request_launches", np.zeros_like(arr), 1, 0, slots_d)):
+        t = serving_table(slots)
         res[name] = run(t, ty, k, v, off, arrivals, max_batch_ops=cap, ordered=ordered)
         t.close()
     res["coalescing_speedup"] = round(res["saturated"]["mops"] / max(res["per_request_launches"]["mops"], 1e-9), 2)
     out["mixes"]["D"] = res
     log(f"serve D: {res}")
+    # per mix: ordered serving first (every request gets its alone-results, in
+    # request order), the snapshot batches second
+    summ = {}
+    for mix, r in out["mixes"].items():
+        summ[mix] = {"ordered": {"open_p50_us": r["open_ordered"]["p50_us"], "open_p99_us": r["open_ordered"]["p99_us"],
+                                 "saturated_mops": r["saturated_ordered"]["mops"],
+                                 "reads_found": r["saturated_ordered"]["reads_found"],
+                                 "resizes": r["open_ordered"]["resizes"]},
+                     "snapshot": {"open_p50_us": r["open"]["p50_us"], "open_p99_us": r["open"]["p99_us"],
+                                  "saturated_mops": r["saturated"]["mops"],
+                                  "reads_found": r["saturated"]["reads_found"]}}
+        if "open_ordered_no_resize" in r:
+            summ[mix]["ordered"]["open_p50_us_without_resize"] = r["open_ordered_no_resize"]["p50_us"]
+            summ[mix]["ordered"]["open_p99_us_without_resize"] = r["open_ordered_no_resize"]["p99_us"]
+            summ[mix]["ordered"]["resize_ms"] = r["open_ordered"]["resize_ms"]
+    out["summary"] = summ
+    log(f"serve summary: {summ}")
     return out
 
 
@@ -390,6 +440,59 @@ def sweep_leg(a, H, W, torch, dev):
         del sets
     for t in tables.values():
         t.close()
+    return out
+
+
+LAYOUT_SWEEP = (("bucket8", 0.65), ("bucket8", 0.9), ("bucket8", 1.0), ("bucket16", 0.65), ("bucket16", 0.9),
+                ("bucket16", 1.0), ("inline8", 0.65), ("inline8", 0.9), ("inline8", 1.0))
+
+
+def layout_point(a, H, W, torch, dev, layout, lf):
+    """One bucket layout at one load factor of 2^26 slots (Default2Hash): the
+    insert of keys_for_lf distinct keys into the cleared table (median of 3),
+    its slow-path share and stash use; then 2^26 lookups, 50 % present (median
+    of 3), and the bucket reads per lookup counted by hive_probe_stats."""
+    slots = 1 << 26
+    stream = torch.cuda.current_stream()
+    n = W.keys_for_lf(slots, lf)
+    keys = W.distinct_keys(n, 4300)
+    miss = W.distinct_keys(slots // 2, 4300, start=n + (n >> 6) + 64)
+    d_k = torch.from_numpy(keys.view(np.int32)).to(dev)
+    d_v = torch.from_numpy(W.values(n, 4301).view(np.int32)).to(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4302)
+    cat = torch.cat([d_k[torch.randint(0, n, (slots // 2,), device=dev, generator=g)],
+                     torch.from_numpy(miss.view(np.int32)).to(dev)])
+    d_lk = cat[torch.randperm(slots, device=dev, generator=g)].contiguous()
+    d_st = torch.empty(n, dtype=torch.uint8, device=dev)
+    d_lv = torch.empty(slots, dtype=torch.int32, device=dev)
+    d_ls = torch.empty(slots, dtype=torch.uint8, device=dev)
+    t = H.HiveTable(slots, "Default2Hash", layout=layout)
+
+    def ins():
+        t.clear()
+        t.insert(d_k, d_v, status=d_st)
+    ins_ms = _timed(torch, stream, ins)
+    st = t.stats()
+    look_ms = _timed(torch, stream, lambda: t.lookup(d_lk, vals=d_lv, status=d_ls))
+    _, lreads, lstash = t.probe_stats(d_lk)
+    placed = int(torch.isin(d_st, torch.tensor([1, 3], dtype=torch.uint8, device=dev)).sum().item())
+    found = int((d_ls == 4).sum().item())
+    t.close()
+    return {"layout": layout, "load_factor": lf, "inserts": n, "insert_ms": round(ins_ms, 4),
+            "lookup_ms": round(look_ms, 4), "slow_path_frac": round(st["slow_path_ops"] / n, 5),
+            "stash_entries": st["stash_live"], "placed": placed, "failed": n - placed,
+            "bucket_reads_per_lookup": round(lreads / slots, 5), "stash_reads_per_lookup": round(lstash / slots, 6),
+            "lookups_found": found, "device_bytes": st["device_bytes"]}
+
+
+def layout_sweep_leg(a, H, W, torch, dev):
+    """Every bucket layout at LF 0.65 / 0.9 / 1.0 (layout_point each)."""
+    out = []
+    for layout, lf in LAYOUT_SWEEP:
+        row = layout_point(a, H, W, torch, dev, layout, lf)
+        out.append(row)
+        log(f"layout sweep: {row}")
     return out
 
 
@@ -501,20 +604,31 @@ def cpu_baseline(keys, vals, lk, threads, host, gpu_found, gpu_vals, policy, nb,
 
 
 def mixed_summary(mixed):
-    """The metric's second half: the fastest stream form on the uneven mixed
-    batches, its per-batch p50/p99 latency, and its speedup over per-batch
-    launches (same results: modes_agree)."""
+    """The metric's second half on the uneven mixed batches: the stream form
+    the autotuner chose (the fastest whose p99 batch service time stays within
+    the per-batch launches' p99 on its sample), its per-batch p50/p99 latency
+    and speedup over per-batch launches on the whole stream; then the fastest
+    form regardless of latency (same results: modes_agree)."""
     modes = (mixed or {}).get("modes") or {}
     if not modes:
         return None
-    best = min(modes, key=lambda m: modes[m]["total_ms"])
-    out = {"form": best, "total_ms": modes[best]["total_ms"], "mops": modes[best]["mops"],
-           "p50_batch_us": modes[best]["p50_batch_us"], "p99_batch_us": modes[best]["p99_batch_us"],
-           "results_agree": mixed.get("modes_agree")}
+
+    def row(m):
+        return {"form": m, "total_ms": modes[m]["total_ms"], "mops": modes[m]["mops"],
+                "p50_batch_us": modes[m]["p50_batch_us"], "p99_batch_us": modes[m]["p99_batch_us"]}
+    chosen = mixed.get("chosen_form")
+    chosen = chosen if chosen in modes else min(modes, key=lambda m: modes[m]["total_ms"])
+    out = row(chosen)
+    out["results_agree"] = mixed.get("modes_agree")
     if "launches" in modes:
         out["launches_p50_batch_us"] = modes["launches"]["p50_batch_us"]
         out["launches_p99_batch_us"] = modes["launches"]["p99_batch_us"]
-        out["speedup_vs_launches"] = round(modes["launches"]["total_ms"] / modes[best]["total_ms"], 3)
+        out["speedup_vs_launches"] = round(modes["launches"]["total_ms"] / modes[chosen]["total_ms"], 3)
+        out["p99_within_launches"] = modes[chosen]["p99_batch_us"] <= modes["launches"]["p99_batch_us"]
+    fastest = min(modes, key=lambda m: modes[m]["total_ms"])
+    out["fastest"] = row(fastest)
+    if "launches" in modes:
+        out["fastest"]["speedup_vs_launches"] = round(modes["launches"]["total_ms"] / modes[fastest]["total_ms"], 3)
     return out
 
 
@@ -535,6 +649,10 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     H.lib()  # fails loudly if libhive.so is missing
+    if a.layout_point:
+        lay, lf = a.layout_point.split(":")
+        print(json.dumps(layout_point(a, H, W, torch, torch.device("cuda:0"), lay, float(lf))), flush=True)
+        return 0
     slots = 1 << a.log2_slots
     nb = slots // 8
     n_ins = W.keys_for_lf(slots, a.lf)
@@ -569,19 +687,26 @@ def main():
     nb = t.num_buckets
     shape_tune = None
     if not a.block and not a.no_autotune:
-        # lookup shapes (block size, lanes per key, keys per lane) timed on the
-        # filled table, at the step's own load factor; the table is unchanged
+        # insert shapes (block size, slow path staged / inline / second launch)
+        # timed on a scratch copy of the empty table; then lookup shapes (block
+        # size, lanes per key, keys per lane) on the filled table, at the
+        # step's own load factor; the table is unchanged by either
         t.clear()
+        bi = t.autotune_bulk(d_keys, d_vals, None)
         t.insert(d_keys, d_vals, status=d_st)
         bt = t.autotune_bulk(None, None, d_lk)
-        shape_tune = {"lookup_block_threads": bt["lookup_block_threads"],
+        shape_tune = {"insert_block_threads": bi["insert_block_threads"],
+                      "insert_slow_path": {0: "staged", 1: "inline", 2: "second launch"}[bi["insert_inline_slow_path"]],
+                      "insert_ms": round(bi["insert_ms"], 4),
+                      "insert_ms_by_slow_path": {k: round(v, 4) for k, v in bi["insert_ms_by_slow_path"].items()},
+                      "lookup_block_threads": bt["lookup_block_threads"],
                       "lookup_lanes_per_key": bt["lookup_lanes_per_key"],
                       "lookup_keys_per_lane": bt["lookup_keys_per_lane"],
                       "lookup_ms": round(bt["lookup_ms"], 4),
                       "lookup_ms_by_lanes_per_key": {str(k): round(v, 4) for k, v in bt["lookup_ms_by_lanes"].items()},
                       "lookup_ms_by_keys_per_lane": {str(k): round(v, 4) for k, v in bt["lookup_ms_by_keys"].items()},
                       "candidates": bt["candidates"]}
-        log(f"lookup shape tune: {shape_tune}")
+        log(f"shape tune: {shape_tune}")
 
     def step(ev=None):
         if ev is not None:
@@ -701,6 +826,7 @@ def main():
         serving = serve_leg(a, H, W, torch, dev)
     sweep = sweep_leg(a, H, W, torch, dev) if not a.no_sweep else None
     growth = growth_leg(a, H, W, torch, dev) if not a.no_growth else None
+    layouts = layout_sweep_leg(a, H, W, torch, dev) if a.layout_sweep else None
 
     cpu = None
     if not a.no_cpu:
@@ -734,7 +860,7 @@ def main():
         "kernel_ms": {"clear": round(clear_ms, 4), "insert": round(ins_ms, 4), "lookup": round(look_ms, 4)},
         "verified": verified,
         "layout_tune": layout_tune,
-        "lookup_shape_tune": shape_tune,
+        "shape_tune": shape_tune,
         "insert_slow_path_frac": round(slow_frac, 5),
         "stash_entries": tstats["stash_live"],
         "roofline": {
@@ -763,9 +889,11 @@ def main():
         },
         "mixed_latency": mixed_summary(mixed),
         "mixed_stream": mixed,
+        "serving_summary": serving["summary"] if serving else None,
         "serving": serving,
         "lf_exist_sweep": sweep,
         "growth": growth,
+        "layout_sweep": layouts,
         "cpu_baseline": cpu,
         "git": git_sha(),
     }

@@ -61,7 +61,16 @@ class Tune(C.Structure):
                 ("blocks_per_cu", C.c_uint32), ("stream_inline", C.c_uint32), ("best_mode", C.c_int),
                 ("launches_ms", C.c_double), ("graph_ms", C.c_double), ("persistent_ms", C.c_double),
                 ("candidates", C.c_uint32), ("window_ops", C.c_uint64), ("window_batch_ops", C.c_uint64),
-                ("windowed_ms", C.c_double)]
+                ("windowed_ms", C.c_double), ("windowed_p99_us", C.c_double), ("launches_p99_us", C.c_double),
+                ("graph_p99_us", C.c_double), ("persistent_p99_us", C.c_double), ("p99_bound_us", C.c_double),
+                ("bounded_window_ops", C.c_uint64), ("bounded_window_batch_ops", C.c_uint64),
+                ("bounded_windowed_ms", C.c_double), ("bounded_windowed_p99_us", C.c_double),
+                ("bounded_split_by_type", C.c_uint32), ("windowed_ms_by_size", C.c_double * 8),
+                ("windowed_p99_us_by_size", C.c_double * 8)]
+
+
+class TuneOpts(C.Structure):
+    _fields_ = [("try_persistent", C.c_uint32), ("max_p99_us", C.c_double), ("p99_slack", C.c_double)]
 
 
 class BulkTune(C.Structure):
@@ -69,7 +78,7 @@ class BulkTune(C.Structure):
                 ("lookup_block_threads", C.c_uint32), ("lookup_lanes_per_key", C.c_uint32),
                 ("insert_ms", C.c_double), ("lookup_ms", C.c_double), ("lookup_ms_by_lanes", C.c_double * 3),
                 ("load_factor", C.c_double), ("candidates", C.c_uint32), ("lookup_keys_per_lane", C.c_uint32),
-                ("lookup_ms_by_keys", C.c_double * 3)]
+                ("lookup_ms_by_keys", C.c_double * 3), ("insert_ms_by_slow_path", C.c_double * 3)]
 
 
 class LayoutTune(C.Structure):
@@ -80,14 +89,15 @@ class LayoutTune(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("max_batch_ops", C.c_uint64), ("max_inflight", C.c_uint32), ("max_load_factor", C.c_double),
                 ("growth", C.c_double), ("hbm_reserve_bytes", C.c_uint64), ("stash_headroom", C.c_double),
-                ("ordered", C.c_uint32)]
+                ("ordered", C.c_uint32), ("min_load_factor", C.c_double), ("shrink_load_factor", C.c_double)]
 
 
 class ServeReport(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("ops", C.c_uint64),
                 ("rejected_requests", C.c_uint64), ("resizes", C.c_uint64), ("final_slots", C.c_uint64),
                 ("span_ms", C.c_double), ("mops", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double),
-                ("max_us", C.c_double), ("mean_batch_ops", C.c_double), ("resize_ms", C.c_double)]
+                ("max_us", C.c_double), ("mean_batch_ops", C.c_double), ("resize_ms", C.c_double),
+                ("shrinks", C.c_uint64), ("shrink_ms", C.c_double)]
 
 
 class SequenceReport(C.Structure):
@@ -151,6 +161,7 @@ def lib():
         "hive_set_stream_shape": [p, u32, u32, u32],
         "hive_set_stream_window": [p, C.c_uint64, C.c_uint64],
         "hive_autotune_stream": [p, p, p, p, p, sz, C.POINTER(Tune), p],
+        "hive_autotune_stream_ex": [p, p, p, p, p, sz, C.POINTER(TuneOpts), C.POINTER(Tune), p],
         "hive_autotune_bulk": [p, p, p, sz, p, sz, C.POINTER(BulkTune), p],
         "hive_autotune_layout": [p, p, p, sz, p, sz, C.POINTER(LayoutTune), p],
         "hive_serve_trace": [p, p, p, p, p, p, sz, p, p, C.POINTER(ServeConfig), p, C.POINTER(ServeReport), p],

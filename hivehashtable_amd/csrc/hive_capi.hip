@@ -224,6 +224,7 @@ hive_status_t hive_destroy(hive_table_t* t) {
   free_view(t->v);
   if (t->scratch) (void)hipFree(t->scratch);
   if (t->part) (void)hipFree(t->part);
+  if (t->pend) (void)hipFree(t->pend);
   (void)free_window(t);
   delete t;
   return HIVE_OK;
@@ -242,6 +243,7 @@ hive_status_t hive_insert(hive_table_t* t, const uint32_t* keys, const uint32_t*
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK) return r;
   if (n && !keys) return fail(HIVE_EINVAL, "null keys");
+  if ((r = ensure_pending(t, n, static_cast<hipStream_t>(stream))) != HIVE_OK) return r;
   return launch_insert(t, SrcNative{keys, vals, nullptr}, n, OutNative{status, nullptr}, -1,
                        static_cast<hipStream_t>(stream));
 }
@@ -547,7 +549,8 @@ hive_status_t hive_set_launch(hive_table_t* t, int op_type, const hive_launch_t*
   if (c.keys_per_lane != 1 && c.keys_per_lane != 2 && c.keys_per_lane != 4)
     return fail(HIVE_EINVAL, "keys_per_lane must be 1, 2 or 4");
   if (c.keys_per_lane != 1 && op_type != OP_LOOKUP) return fail(HIVE_EINVAL, "keys_per_lane > 1 is for lookups");
-  if (c.inline_slow_path > 1) return fail(HIVE_EINVAL, "inline_slow_path must be 0 or 1");
+  if (c.inline_slow_path > 2) return fail(HIVE_EINVAL, "inline_slow_path must be 0, 1 or 2");
+  if (c.inline_slow_path == 2 && op_type != OP_INSERT) return fail(HIVE_EINVAL, "inline_slow_path 2 is for inserts");
   if (c.combine_duplicates > 1) return fail(HIVE_EINVAL, "combine_duplicates must be 0 or 1");
   if (c.split_by_type > 1) return fail(HIVE_EINVAL, "split_by_type must be 0 or 1");
   if (c.lanes_per_key == 0) c.lanes_per_key = 1;

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -51,6 +52,8 @@ struct hive_table {
   uint32_t* win_buf = nullptr;           // windowed streams: two buffers of per-op arrays, chain marks, control block
   size_t win_cap = 0;                    // key-set entries (a power of two; windows of up to win_cap / 2 ops)
   uint32_t win_epoch = 0;                // epoch of the last window (entries of other epochs are free)
+  unsigned long long* pend = nullptr;    // split inserts: pending kv [cap], op index [cap] (u32), count
+  size_t pend_cap = 0;                   // ops the pending list holds
 };
 
 namespace hive {
@@ -98,16 +101,58 @@ inline int table_kind(const hive_table* t) {
     default: return ::hive::fail(HIVE_EINVAL, "unknown policy or layout");                                        \
   }
 
+// The pending list of split inserts (inline_slow_path = 2) for batches of up
+// to n ops, allocated by the bulk insert entry point and the bulk autotuner,
+// never under stream capture; launches without a list big enough run staged.
+// (force: whatever the table's insert launch option is)
+inline hive_status_t ensure_pending(hive_table* t, size_t n, hipStream_t s, bool force = false) {
+  if ((!force && t->launch[OP_INSERT].inline_slow_path != 2) || n <= t->pend_cap || n > 0xFFFFFFFFull) return HIVE_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return HIVE_OK;
+  }
+  unsigned long long* p = nullptr;
+  if (hipMalloc(&p, n * 12 + 256) != hipSuccess) {
+    (void)hipGetLastError();
+    return HIVE_OK;  // no list: split inserts run staged
+  }
+  if (t->pend) {
+    (void)hipDeviceSynchronize();  // the old list may still be in use by queued work
+    (void)hipFree(t->pend);
+  }
+  t->pend = p;
+  t->pend_cap = n;
+  return HIVE_OK;
+}
+inline PendingList pending_list(const hive_table* t) {
+  uint32_t* idx = reinterpret_cast<uint32_t*>(t->pend + t->pend_cap);
+  return PendingList{t->pend, idx, reinterpret_cast<uint32_t*>(t->pend + t->pend_cap + (t->pend_cap + 1) / 2)};
+}
+
 template <class Src, class Out>
 hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase, hipStream_t s,
                             BatchClock clk = BatchClock{nullptr, nullptr}) {
   if (n == 0) return HIVE_OK;
   const uint32_t blk = t->launch[OP_INSERT].block_threads;
   if (!fits_one_launch(n, blk)) return fail(HIVE_EINVAL, "batch too large for one launch");
+  if (t->launch[OP_INSERT].inline_slow_path == 2 && !t->launch[OP_INSERT].combine_duplicates && t->pend &&
+      n <= t->pend_cap) {
+    const PendingList pl = pending_list(t);
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
+    const unsigned slow_grid = static_cast<unsigned>(std::max(1, ncu) * 16);
+    k_fill_u32<<<1, 64, 0, s>>>(pl.count, 1, 0u);
+    HIVE_POLICY_SWITCH(table_kind(t), (k_insert_fast<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
+                                                                                                phase, pl)));
+    HIVE_POLICY_SWITCH(table_kind(t), (k_insert_slow<P, Out><<<slow_grid, 256, 0, s>>>(t->v, pl, out, clk)));
+    HIP_CHECK_LAUNCH();
+    return HIVE_OK;
+  }
   if (t->launch[OP_INSERT].combine_duplicates) {
     HIVE_POLICY_SWITCH(table_kind(t), (k_insert_combined<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                     phase, clk)));
-  } else if (t->launch[OP_INSERT].inline_slow_path) {
+  } else if (t->launch[OP_INSERT].inline_slow_path == 1) {
     HIVE_POLICY_SWITCH(table_kind(t), (k_insert<P, Src, Out, true><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                  phase, clk)));
   } else {

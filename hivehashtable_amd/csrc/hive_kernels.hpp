@@ -221,6 +221,70 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   clk.finish();
 }
 
+// Split insert (launch option inline_slow_path = 2): two launches.  The first
+// runs every lane's fast path and appends the ops that must take the slow
+// path to a device list (key and value packed, and the op's index; one
+// atomic per wavefront); the second runs the slow path over that list.  The
+// fast-path kernel then carries only the fast path's registers (51 VGPRs,
+// 8 waves per SIMD, against 78 and 6 for k_insert with the in-line 8-slot
+// layout), so more of its single CASes are in flight.  *count is zero on
+// entry to the first launch.
+struct PendingList {
+  unsigned long long* kv;
+  uint32_t* idx;
+  uint32_t* count;
+};
+
+template <class P, class Src, class Out>
+__global__ __launch_bounds__(kMaxBlock) void k_insert_fast(TableView t, Src src, size_t n, Out out, int phase,
+                                                           PendingList pl) {
+  if (block_past_list(src)) return;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int dtab = 0;
+  const uint32_t key = i < n ? src.key(i) : 0u;  // requested together with the op type
+  const uint32_t val = i < n ? src.val(i, key) : 0u;
+  bool pend = false;
+  if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
+    const uint8_t r = fast_insert<P>(t, key, val);
+    if (r == R_PENDING) {
+      pend = true;
+    } else {
+      out.put(i, r, 0);
+      dtab += (r == R_INSERTED);
+    }
+  }
+  const unsigned long long b = __ballot(pend);
+  if (b) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(pl.count, static_cast<uint32_t>(__popcll(b)));
+    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
+    if (pend) {
+      const uint32_t j = base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
+      pl.kv[j] = kv_pack(key, val);
+      pl.idx[j] = static_cast<uint32_t>(i);
+    }
+  }
+  wave_count(t, dtab, 0, 0);
+}
+
+template <class P, class Out>
+__global__ __launch_bounds__(256) void k_insert_slow(TableView t, PendingList pl, Out out,
+                                                     BatchClock clk = BatchClock{nullptr, nullptr}) {
+  const uint32_t cnt = *pl.count;  // written by the previous launch
+  int dtab = 0, dstash = 0, dslow = 0;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const unsigned long long kv = pl.kv[j];
+    const uint8_t r = slow_insert<P>(t, kv_key(kv), kv_val(kv));
+    out.put(pl.idx[j], r, 0);
+    dtab += (r == R_INSERTED);
+    dstash += (r == R_STASHED);
+    ++dslow;
+  }
+  wave_count(t, dtab, dstash, dslow);
+  clk.finish();
+}
+
 // Inserts with the duplicate keys of one block combined before they reach
 // memory (launch option for skewed batches, where thousands of ops update a
 // few hot keys).  Each key's first lane in a block-local LDS hash table is its

@@ -132,9 +132,15 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   if (c.growth == 0.0) c.growth = 2.0;
   if (c.hbm_reserve_bytes == 0) c.hbm_reserve_bytes = 1ull << 30;
   if (c.stash_headroom == 0.0) c.stash_headroom = 0.5;
+  if (c.min_load_factor > 0.0 && c.shrink_load_factor == 0.0) c.shrink_load_factor = 0.75;
   if (c.max_inflight > 8 || !(c.max_load_factor > 0.0 && c.max_load_factor <= 1.0) || !(c.growth > 1.0) ||
       !(c.stash_headroom > 0.0 && c.stash_headroom <= 1.0) || c.ordered > 1)
     return fail(HIVE_EINVAL, "serve config out of range");
+  const bool shrinking = c.min_load_factor > 0.0;
+  if (!(c.min_load_factor >= 0.0) ||
+      (shrinking && !(c.shrink_load_factor > c.min_load_factor && c.shrink_load_factor <= c.max_load_factor)))
+    return fail(HIVE_EINVAL, "shrink config out of range: need 0 < min_load_factor < shrink_load_factor <= "
+                             "max_load_factor");
   hive_serve_report_t rep{};
   rep.requests = nreq;
   if (nreq == 0) {
@@ -210,6 +216,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   uint64_t capacity = st.capacity;
   uint64_t table_ub = st.table_live;  // upper bound of table entries
   uint64_t stash_ub = st.stash_live;  // upper bound of stash entries
+  uint64_t del_since = 0;             // deletes launched since the last exact count
   EventPool pool;
   for (uint32_t i = 0; i < c.max_inflight; ++i) {
     hipEvent_t e = nullptr;
@@ -246,6 +253,32 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     capacity = x.capacity;
     table_ub = x.table_live;
     stash_ub = x.stash_live;
+    del_since = 0;
+    return HIVE_OK;
+  };
+  // Shrink in place when the exact load factor (table entries / capacity, as
+  // count_occupancy_kernel counts it, include/hive_kernels.cuh:303-325) is
+  // below min_load_factor: to ceil(entries / shrink_load_factor / slots)
+  // buckets, entries counting the stash (src/hive_dispatch.cu:189-206 sizes
+  // the table for 75 % occupancy).  `drained`: the queue is empty and the
+  // counts exact.
+  auto maybe_shrink = [&](bool drained) -> hive_status_t {
+    const Clock::time_point tr = Clock::now();
+    hive_status_t rr;
+    if (!drained && ((rr = retire(true)) != HIVE_OK || (rr = refresh()) != HIVE_OK)) return rr;
+    if (static_cast<double>(table_ub) >= c.min_load_factor * static_cast<double>(capacity)) return HIVE_OK;
+    const uint64_t live = table_ub + stash_ub;
+    uint64_t want = static_cast<uint64_t>(std::ceil(static_cast<double>(live) / c.shrink_load_factor / t->v.S));
+    if (want < 1) want = 1;
+    if (want >= t->v.nb) return HIVE_OK;
+    if ((rr = hive_resize(t, want * t->v.S, stream)) != HIVE_OK) {
+      if (rr != HIVE_ECAPACITY) return rr;
+      g_last_error.clear();  // the entries did not fit the smaller table: it stays as it is
+      return HIVE_OK;
+    }
+    if ((rr = refresh()) != HIVE_OK) return rr;
+    rep.shrink_ms += std::chrono::duration<double, std::milli>(Clock::now() - tr).count();
+    ++rep.shrinks;
     return HIVE_OK;
   };
   auto fits = [&](uint64_t ins) {
@@ -257,6 +290,13 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
     while (arrived < nreq && h_arrival_us[arrived] <= now) ++arrived;
     if ((r = retire(false)) != HIVE_OK) return r;
     if (launched < arrived && q.size() < c.max_inflight) {
+      // the deletes launched since the last exact count may have taken the
+      // load factor below the shrink threshold: count and maybe shrink
+      if (shrinking && del_since &&
+          static_cast<double>(table_ub - std::min(del_since, table_ub)) <
+              c.min_load_factor * static_cast<double>(capacity) &&
+          (r = maybe_shrink(false)) != HIVE_OK)
+        return r;
       // coalesce every arrived request, up to max_batch_ops (at least one request)
       size_t end = launched + 1;
       uint64_t ins = h_ins[launched];
@@ -285,7 +325,11 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
         if (!fits(ins)) {
           // no room to grow: keep the longest prefix whose inserts still fit,
           // or that the stash can absorb
-          const uint64_t stash_cap = static_cast<uint64_t>(t->v.nb) * kStashSlots;
+          // stash entries the table can hold: 8 per bucket, or, in the
+          // in-line layout, 8 per bucket of its stash pool
+          const uint64_t stash_cap =
+              static_cast<uint64_t>(t->v.layout == kLayoutInline ? std::min(t->v.nb, t->v.pool_cap) : t->v.nb) *
+              kStashSlots;
           const double stash_room = c.stash_headroom * static_cast<double>(stash_cap);
           size_t keep = launched;
           uint64_t kins = 0;
@@ -338,6 +382,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
       HIP_TRY(hipEventRecord(ev, s));
       q.push_back(Batch{launched, end, ev});
       table_ub += ins;
+      del_since += wr - ins;
       ++rep.batches;
       rep.ops += n;
       launched = end;
@@ -345,6 +390,7 @@ extern "C" hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types,
   }
   HIP_TRY(hipStreamSynchronize(s));
   if ((r = refresh()) != HIVE_OK) return r;
+  if (shrinking && (r = maybe_shrink(true)) != HIVE_OK) return r;
   std::vector<double> served;
   served.reserve(nreq);
   for (size_t i = 0; i < nreq; ++i)

@@ -457,21 +457,51 @@ hive_status_t copy_view(const TableView& dst, const TableView& src, hipStream_t 
 
 }  // namespace
 
-extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
-                                              const uint32_t* vals, const uint64_t* h_offsets, size_t nbatches,
-                                              hive_tune_t* out, void* stream) {
+namespace {
+
+// p99 of the batches' service times: a batch's time from the previous
+// distinct completion to its own (batches of one window complete together,
+// and completions are in stream order), percentile by linear interpolation
+double p99_service_us(const std::vector<double>& done_ns) {
+  if (done_ns.empty()) return 0.0;
+  std::vector<double> lat(done_ns.size());
+  double prev = 0.0, cur = 0.0;
+  for (size_t b = 0; b < done_ns.size(); ++b) {
+    if (b == 0 || done_ns[b] != cur) {
+      prev = b == 0 ? 0.0 : cur;
+      cur = done_ns[b];
+    }
+    lat[b] = (done_ns[b] - prev) * 1e-3;
+  }
+  std::sort(lat.begin(), lat.end());
+  const double pos = 0.99 * static_cast<double>(lat.size() - 1);
+  const size_t lo = static_cast<size_t>(pos), hi = std::min(lo + 1, lat.size() - 1);
+  return lat[lo] + (lat[hi] - lat[lo]) * (pos - static_cast<double>(lo));
+}
+
+}  // namespace
+
+extern "C" hive_status_t hive_autotune_stream_ex(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
+                                                 const uint32_t* vals, const uint64_t* h_offsets, size_t nbatches,
+                                                 const hive_tune_opts_t* opts, hive_tune_t* out, void* stream) {
   hive_status_t r;
   if ((r = check_table(t)) != HIVE_OK || (r = set_device(t)) != HIVE_OK) return r;
   if (!out || !h_offsets || nbatches == 0) return fail(HIVE_EINVAL, "autotune needs a sample stream and an output");
+  const hive_tune_opts_t o = opts ? *opts : hive_tune_opts_t{0u, -1.0, 0.1};
+  if (o.try_persistent > 1 || !(o.max_p99_us == o.max_p99_us) || !(o.p99_slack >= 0.0))
+    return fail(HIVE_EINVAL, "bad autotune options");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t nops = h_offsets[nbatches];
   uint64_t maxn = 0;
   for (size_t b = 0; b < nbatches; ++b) maxn = std::max<uint64_t>(maxn, h_offsets[b + 1] - h_offsets[b]);
-  const uint64_t windows[] = {uint64_t(1) << 20, uint64_t(1) << 21, uint64_t(1) << 22};
-  const uint64_t coalesce[] = {uint64_t(1) << 17, uint64_t(1) << 18, uint64_t(1) << 19};
+  // windows of 2^15 .. 2^22 ops, each coalescing batches below 1/32 .. 1/4
+  // of its size (2^10 .. 2^19): small windows finish within a big batch's
+  // time, big ones amortize the window's fixed kernels over more ops
+  constexpr int kWindowSizes = 8, kWindowLog0 = 15;
+  const uint64_t wmax = uint64_t(1) << (kWindowLog0 + kWindowSizes - 1);
   // scratch shared with the copy below, so the copy never reallocates it
-  if ((r = ensure_partition(t, std::max<uint64_t>(maxn, windows[2]))) != HIVE_OK) return r;
-  if ((r = ensure_window(t, windows[2])) != HIVE_OK) return r;
+  if ((r = ensure_partition(t, std::max<uint64_t>(maxn, wmax))) != HIVE_OK) return r;
+  if ((r = ensure_window(t, wmax)) != HIVE_OK) return r;
   hive_table tmp = *t;  // same policy and device, scratch copy of the table state
   if ((r = alloc_view(tmp.v, t->v.nb, layout_of(t->v))) != HIVE_OK) return r;
   struct ViewGuard {
@@ -484,17 +514,20 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
   HIP_TRY(ar.alloc(&ov, nops));
   HIP_TRY(ar.alloc(&st, nops));
   std::vector<double> done(nbatches);
-  auto run = [&](int mode, double* ms) -> hive_status_t {
+  // one timed run of the sample: total ms and p99 batch service time
+  auto run = [&](int mode, double* ms, double* p99) -> hive_status_t {
     hive_status_t rr = copy_view(tmp.v, t->v, s);
     if (rr != HIVE_OK) return rr;
     double setup = 0.0;
     rr = hive_mixed_stream(&tmp, types, keys, vals, h_offsets, nbatches, ov, st, mode, done.data(), &setup, stream);
     if (rr != HIVE_OK) return rr;
     *ms = done[nbatches - 1] * 1e-6;
+    *p99 = p99_service_us(done);
     return HIVE_OK;
   };
   hive_tune_t best{};
-  best.launches_ms = best.graph_ms = best.persistent_ms = best.windowed_ms = 1e300;
+  best.launches_ms = best.graph_ms = best.windowed_ms = 1e300;
+  best.bounded_windowed_ms = 1e300;
   uint32_t cands = 0;
   const uint32_t blocks[] = {128, 256, 512, 1024};
   const bool any_big = maxn >= kCompactMinOps;  // the split only applies to big batches
@@ -504,83 +537,135 @@ extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* ty
         for (int op = 0; op < 3; ++op)
           tmp.launch[op] = hive_launch_t{blk, 1, inl, t->launch[op].combine_duplicates, split,
                                          t->launch[op].lanes_per_key};
-        double ms = 0;
-        if ((r = run(HIVE_STREAM_LAUNCHES, &ms)) != HIVE_OK) return r;
+        double ms = 0, p99 = 0;
+        if ((r = run(HIVE_STREAM_LAUNCHES, &ms, &p99)) != HIVE_OK) return r;
         ++cands;
         if (ms < best.launches_ms) {
           best.launches_ms = ms;
+          best.launches_p99_us = p99;
           best.block_threads = blk;
           best.inline_slow_path = inl;
           best.split_by_type = split;
         }
       }
+  const double bound = o.max_p99_us < 0.0 ? best.launches_p99_us * (1.0 + o.p99_slack) : o.max_p99_us;  // 0: none
+  auto within = [&](double p99) { return bound <= 0.0 || p99 <= bound; };
   for (int op = 0; op < 3; ++op)
     tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
                                    best.split_by_type, t->launch[op].lanes_per_key};
-  if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms)) != HIVE_OK) return r;
+  if ((r = run(HIVE_STREAM_GRAPH, &best.graph_ms, &best.graph_p99_us)) != HIVE_OK) return r;
   ++cands;
-  const uint32_t tiles[] = {0, 256, 1024, 4096};
-  const uint32_t bpcs[] = {2, 4, 8};
-  for (uint32_t inl = 0; inl < 2; ++inl)
-    for (uint32_t tile : tiles)
-      for (uint32_t bpc : bpcs) {
-        tmp.stream_tile = tile;
-        tmp.stream_blocks_per_cu = bpc;
-        tmp.stream_inline = inl;
-        double ms = 0;
-        if ((r = run(HIVE_STREAM_PERSISTENT, &ms)) != HIVE_OK) return r;
-        ++cands;
-        if (ms < best.persistent_ms) {
-          best.persistent_ms = ms;
-          best.tile_ops = tile;
-          best.blocks_per_cu = bpc;
-          best.stream_inline = inl;
+  if (o.try_persistent) {
+    best.persistent_ms = 1e300;
+    const uint32_t tiles[] = {0, 256, 1024, 4096};
+    const uint32_t bpcs[] = {2, 4, 8};
+    for (uint32_t inl = 0; inl < 2; ++inl)
+      for (uint32_t tile : tiles)
+        for (uint32_t bpc : bpcs) {
+          tmp.stream_tile = tile;
+          tmp.stream_blocks_per_cu = bpc;
+          tmp.stream_inline = inl;
+          double ms = 0, p99 = 0;
+          if ((r = run(HIVE_STREAM_PERSISTENT, &ms, &p99)) != HIVE_OK) return r;
+          ++cands;
+          if (ms < best.persistent_ms) {
+            best.persistent_ms = ms;
+            best.persistent_p99_us = p99;
+            best.tile_ops = tile;
+            best.blocks_per_cu = bpc;
+            best.stream_inline = inl;
+          }
         }
-      }
+  }
   uint32_t win_split = best.split_by_type;
   {  // first touch of the window scratch, untimed
-    double ms = 0;
-    tmp.stream_window_ops = windows[2];
-    tmp.stream_window_batch = coalesce[2];
-    r = run(HIVE_STREAM_WINDOWED, &ms);
+    double ms = 0, p99 = 0;
+    tmp.stream_window_ops = wmax;
+    tmp.stream_window_batch = wmax / 8;
+    r = run(HIVE_STREAM_WINDOWED, &ms, &p99);
     t->win_epoch = tmp.win_epoch;
     if (r != HIVE_OK) return r;
   }
   for (uint32_t split = 0; split < 2; ++split)
-    for (uint64_t w : windows)
-      for (uint64_t cb : coalesce) {
-        if (cb > w) continue;
+    for (int wl = kWindowLog0; wl < kWindowLog0 + kWindowSizes; ++wl)
+      for (int cl = wl - 5; cl <= wl - 2; ++cl) {
+        const uint64_t w = uint64_t(1) << wl, cb = uint64_t(1) << cl;
+        if (cl > 19) continue;
+        // type compaction applies to windows of at least kCompactMinOps ops
+        if (split && w < kCompactMinOps) continue;
         for (int op = 0; op < 3; ++op)
           tmp.launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path,
                                          t->launch[op].combine_duplicates, split, t->launch[op].lanes_per_key};
         tmp.stream_window_ops = w;
         tmp.stream_window_batch = cb;
-        double ms = 0;
-        r = run(HIVE_STREAM_WINDOWED, &ms);
+        double ms = 0, p99 = 0;
+        r = run(HIVE_STREAM_WINDOWED, &ms, &p99);
         t->win_epoch = tmp.win_epoch;  // the key set is shared: later windows continue its epochs
         if (r != HIVE_OK) return r;
         ++cands;
+        double& by_size = best.windowed_ms_by_size[wl - kWindowLog0];
+        if (by_size == 0.0 || ms < by_size) {
+          by_size = ms;
+          best.windowed_p99_us_by_size[wl - kWindowLog0] = p99;
+        }
         if (ms < best.windowed_ms) {
           best.windowed_ms = ms;
+          best.windowed_p99_us = p99;
           best.window_ops = w;
           best.window_batch_ops = cb;
           win_split = split;
         }
+        if (within(p99) && ms < best.bounded_windowed_ms) {
+          best.bounded_windowed_ms = ms;
+          best.bounded_windowed_p99_us = p99;
+          best.bounded_window_ops = w;
+          best.bounded_window_batch_ops = cb;
+          best.bounded_split_by_type = split;
+        }
       }
   best.candidates = cands;
-  const double modes[4] = {best.launches_ms, best.graph_ms, best.persistent_ms, best.windowed_ms};
-  best.best_mode = static_cast<int>(std::min_element(modes, modes + 4) - modes);
-  if (best.best_mode == HIVE_STREAM_WINDOWED) best.split_by_type = win_split;
+  best.p99_bound_us = bound > 0.0 ? bound : 0.0;
+  const bool have_bounded = best.bounded_window_ops != 0;
+  if (!have_bounded) best.bounded_windowed_ms = 0.0;
+  // the fastest form within the bound (per-batch launches are always within
+  // the default bound, which they define)
+  double best_ms = 1e300;
+  best.best_mode = HIVE_STREAM_LAUNCHES;
+  auto consider = [&](int mode, double ms, double p99) {
+    if (within(p99) && ms < best_ms) {
+      best_ms = ms;
+      best.best_mode = mode;
+    }
+  };
+  consider(HIVE_STREAM_LAUNCHES, best.launches_ms, o.max_p99_us < 0.0 ? 0.0 : best.launches_p99_us);
+  consider(HIVE_STREAM_GRAPH, best.graph_ms, best.graph_p99_us);
+  if (o.try_persistent) consider(HIVE_STREAM_PERSISTENT, best.persistent_ms, best.persistent_p99_us);
+  if (have_bounded) consider(HIVE_STREAM_WINDOWED, best.bounded_windowed_ms, best.bounded_windowed_p99_us);
+  // the handle keeps the window within the bound (else the fastest window)
+  const uint64_t keep_w = have_bounded ? best.bounded_window_ops : best.window_ops;
+  const uint64_t keep_cb = have_bounded ? best.bounded_window_batch_ops : best.window_batch_ops;
+  const uint32_t keep_split = have_bounded ? best.bounded_split_by_type : win_split;
+  if (best.best_mode == HIVE_STREAM_WINDOWED) best.split_by_type = keep_split;
   for (int op = 0; op < 3; ++op)
     t->launch[op] = hive_launch_t{best.block_threads, 1, best.inline_slow_path, t->launch[op].combine_duplicates,
                                   best.split_by_type, t->launch[op].lanes_per_key};
-  t->stream_tile = best.tile_ops;
-  t->stream_blocks_per_cu = best.blocks_per_cu;
-  t->stream_inline = best.stream_inline;
-  t->stream_window_ops = best.window_ops;
-  t->stream_window_batch = best.window_batch_ops;
+  if (o.try_persistent) {
+    t->stream_tile = best.tile_ops;
+    t->stream_blocks_per_cu = best.blocks_per_cu;
+    t->stream_inline = best.stream_inline;
+  } else {
+    best.persistent_ms = 0.0;
+  }
+  t->stream_window_ops = keep_w;
+  t->stream_window_batch = keep_cb;
   *out = best;
   return HIVE_OK;
+}
+
+extern "C" hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
+                                              const uint32_t* vals, const uint64_t* h_offsets, size_t nbatches,
+                                              hive_tune_t* out, void* stream) {
+  return hive_autotune_stream_ex(t, types, keys, vals, h_offsets, nbatches, nullptr, out, stream);
 }
 
 extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
@@ -623,6 +708,8 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
   uint32_t cands = 0;
   DevArena ar;
   if (n_insert) {
+    // the split form's pending list, shared with the scratch copy below
+    if ((r = ensure_pending(t, n_insert, s, true)) != HIVE_OK) return r;
     hive_table tmp = *t;  // scratch copy of the table state, same policy and device
     if ((r = alloc_view(tmp.v, t->v.nb, layout_of(t->v))) != HIVE_OK) return r;
     struct ViewGuard {
@@ -631,7 +718,9 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
     } guard{&tmp.v};
     uint8_t* st8 = nullptr;
     HIP_TRY(ar.alloc(&st8, n_insert));
-    for (uint32_t inl = 0; inl < 2; ++inl)
+    // slow path staged in LDS (0), inline (1), or in a second launch (2;
+    // only when the pending list could be allocated)
+    for (uint32_t inl = 0; inl < (t->pend_cap >= n_insert ? 3u : 2u); ++inl)
       for (uint32_t blk : blocks) {
         tmp.launch[OP_INSERT] = hive_launch_t{blk, 1, inl, 0, t->launch[OP_INSERT].split_by_type, 1};
         if ((r = copy_view(tmp.v, t->v, s)) != HIVE_OK) return r;
@@ -641,6 +730,8 @@ extern "C" hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* ins
                   &ms);
         if (r != HIVE_OK) return r;
         ++cands;
+        double& by_mode = best.insert_ms_by_slow_path[inl];
+        if (by_mode == 0.0 || ms < by_mode) by_mode = ms;
         if (ms < best.insert_ms) {
           best.insert_ms = ms;
           best.insert_block_threads = blk;

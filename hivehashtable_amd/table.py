@@ -10,7 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BulkTune, LayoutTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune, LAYOUTS, POLICIES,
+from ._lib import (BulkTune, LayoutTune, Launch, Operation, SequenceReport, ServeConfig, ServeReport, Stats, Tune,
+                   TuneOpts, LAYOUTS, POLICIES,
                    check, lib,
                    OP_INSERT,
                    OP_LOOKUP, OP_DELETE)
@@ -166,27 +167,37 @@ class HiveTable:
                                        C.byref(setup), _stream_ptr(stream)), "hive_mixed_stream")
         return out_vals, status, done[:nb], setup.value
 
-    def autotune_stream(self, types, keys, vals, offsets, stream=None):
-        """Time launch shapes on a scratch copy of this table; keep the fastest."""
+    def autotune_stream(self, types, keys, vals, offsets, stream=None, try_persistent=False, max_p99_us=-1.0,
+                        p99_slack=0.1):
+        """Time the stream forms on a scratch copy of this table; keep the
+        fastest whose p99 batch service time stays within max_p99_us (< 0: the
+        per-batch launches' p99 times 1 + p99_slack; 0: no bound).
+        try_persistent: also time the persistent kernel."""
         torch = _torch()
         ty = _dev(types, torch.uint8)
         k = _dev(keys, torch.int32)
         v = _dev(vals, torch.int32)
         off = np.ascontiguousarray(offsets, np.uint64)
         res = Tune()
-        check(self.L.hive_autotune_stream(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
-                                          off.size - 1, C.byref(res), _stream_ptr(stream)), "hive_autotune_stream")
-        return {f: getattr(res, f) for f, _ in Tune._fields_}
+        opts = TuneOpts(int(bool(try_persistent)), float(max_p99_us), float(p99_slack))
+        check(self.L.hive_autotune_stream_ex(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
+                                             off.size - 1, C.byref(opts), C.byref(res), _stream_ptr(stream)),
+              "hive_autotune_stream_ex")
+        out = {f: getattr(res, f) for f, _ in Tune._fields_}
+        out["windowed_ms_by_size"] = {1 << (15 + i): res.windowed_ms_by_size[i] for i in range(8)}
+        out["windowed_p99_us_by_size"] = {1 << (15 + i): res.windowed_p99_us_by_size[i] for i in range(8)}
+        return out
 
     def serve(self, types, keys, vals, req_offsets, arrival_us, max_batch_ops=0, max_inflight=0, max_load_factor=0.0,
               growth=0.0, hbm_reserve_bytes=0, stash_headroom=0.0, ordered=0, out_vals=None, status=None,
-              stream=None):
+              stream=None, min_load_factor=0.0, shrink_load_factor=0.0):
         """Replay a request trace with continuous batching (hive_serve_trace).
 
         Request r = ops[req_offsets[r]:req_offsets[r+1]], arriving arrival_us[r]
         after the call starts; ordered=1: every request sees the writes of the
-        requests before it, also inside one batch.  Returns (out_vals, status,
-        latency_us, report)."""
+        requests before it, also inside one batch; min_load_factor > 0: shrink
+        in place below it (to shrink_load_factor, default 0.75).  Returns
+        (out_vals, status, latency_us, report)."""
         torch = _torch()
         ty = _dev(types, torch.uint8)
         k = _dev(keys, torch.int32)
@@ -201,7 +212,8 @@ class HiveTable:
         if status is None:
             status = torch.empty(k.numel(), dtype=torch.uint8, device=k.device)
         cfg = ServeConfig(int(max_batch_ops), int(max_inflight), float(max_load_factor), float(growth),
-                          int(hbm_reserve_bytes), float(stash_headroom), int(ordered))
+                          int(hbm_reserve_bytes), float(stash_headroom), int(ordered), float(min_load_factor),
+                          float(shrink_load_factor))
         lat = np.zeros(max(nreq, 1), np.float64)
         rep = ServeReport()
         check(self.L.hive_serve_trace(self.h, _ptr(ty), _ptr(k), _ptr(v), off.ctypes.data_as(C.c_void_p),
@@ -346,6 +358,9 @@ class HiveTable:
                                      8: res.lookup_ms_by_lanes[2]}
         out["lookup_ms_by_keys"] = {1: res.lookup_ms_by_keys[0], 2: res.lookup_ms_by_keys[1],
                                     4: res.lookup_ms_by_keys[2]}
+        out["insert_ms_by_slow_path"] = {"staged": res.insert_ms_by_slow_path[0],
+                                         "inline": res.insert_ms_by_slow_path[1],
+                                         "second launch": res.insert_ms_by_slow_path[2]}
         return out
 
     def autotune_layout(self, insert_keys=None, insert_vals=None, lookup_keys=None, stream=None):

@@ -111,7 +111,11 @@ typedef struct {
                                 (same results); other ops: 1 */
   uint32_t inline_slow_path; /* inserts: 0 = stage slow-path ops in LDS behind one
                                 block barrier (throughput), 1 = each lane runs its
-                                own slow path at once (latency) */
+                                own slow path at once (latency), 2 = two launches:
+                                every fast path, then the slow paths from a device
+                                list (hive_insert allocates the list, 12 bytes per
+                                op, outside stream capture; other entry points and
+                                batches larger than the list run as 0) */
   uint32_t combine_duplicates; /* inserts: 1 = ops of one block on the same key are
                                   combined into one table update before reaching
                                   memory (skewed batches; slow path inline) */
@@ -226,11 +230,18 @@ hive_status_t hive_set_stream_window(hive_table_t* t, uint64_t window_ops, uint6
  * load factor, leaving the table unchanged -- for per-batch launches at
  * 128..1024 threads per block x {staged, inline} insert slow path (x split
  * by type or not, when the sample has batches of >= 2^18 ops), the
- * captured graph, the persistent kernel over tile sizes {auto, 256, 1024,
- * 4096} x {2, 4, 8} blocks per CU x {staged, inline}, and windows of
- * {2^20, 2^21, 2^22} ops coalescing batches below {2^17, 2^18, 2^19} ops x
- * {split by type or not}, then stores the fastest launch shapes and window in
- * the handle.  Synchronizes. */
+ * captured graph, windows of {2^15 .. 2^22} ops coalescing batches below
+ * 1/32 .. 1/4 of the window (at most 2^19 ops) x {split by type or not, for
+ * windows of >= 2^18 ops}, and, on
+ * request (hive_autotune_stream_ex), the persistent kernel over tile sizes
+ * {auto, 256, 1024, 4096} x {2, 4, 8} blocks per CU x {staged, inline}.
+ * Every candidate's p99 batch service time is measured (a batch's time from
+ * the previous completion to its own; the batches of a window complete
+ * together).  The chosen form is the fastest whose p99 stays within the
+ * latency bound -- by default the p99 of the per-batch launches plus 10 %, so
+ * the choice does not trade latency for throughput -- and its launch shapes and
+ * window are stored in the handle.  The fastest window regardless of latency
+ * is reported too.  Synchronizes. */
 typedef struct {
   uint32_t block_threads;  /* chosen threads per block of the plain kernels */
   uint32_t inline_slow_path; /* chosen insert slow-path placement of the plain kernels */
@@ -243,17 +254,43 @@ typedef struct {
   double graph_ms;
   double persistent_ms;
   uint32_t candidates;     /* configurations timed */
-  uint64_t window_ops;     /* chosen window size of the windowed form */
-  uint64_t window_batch_ops; /* chosen: batches below this size are coalesced */
-  double windowed_ms;
+  uint64_t window_ops;     /* fastest window (regardless of latency): its size */
+  uint64_t window_batch_ops; /* ... batches below this size are coalesced */
+  double windowed_ms;      /* ... and its sample time */
+  double windowed_p99_us;  /* ... and its p99 batch service time */
+  double launches_p99_us;  /* p99 batch service time of the chosen per-batch launches */
+  double graph_p99_us;
+  double persistent_p99_us; /* (persistent fields: 0 when not tried) */
+  double p99_bound_us;     /* the latency bound of the choice (0 = none) */
+  uint64_t bounded_window_ops; /* fastest window within the bound (0 = none): its size */
+  uint64_t bounded_window_batch_ops;
+  double bounded_windowed_ms;
+  double bounded_windowed_p99_us;
+  uint32_t bounded_split_by_type;
+  double windowed_ms_by_size[8];     /* per window size 2^15 .. 2^22: the best sample time */
+  double windowed_p99_us_by_size[8]; /* ... and that candidate's p99 batch service time */
 } hive_tune_t;
+typedef struct {
+  uint32_t try_persistent; /* 1 = also time the persistent kernel (0: it never won on MI355X) */
+  double max_p99_us;       /* latency bound of the choice: < 0 = the per-batch launches' p99
+                              times (1 + p99_slack), 0 = none (fastest form), > 0 = this many
+                              microseconds */
+  double p99_slack;        /* with max_p99_us < 0: tolerance over the launches' p99, >= 0 (the
+                              p99 of a 100-batch sample is about its second-longest batch) */
+} hive_tune_opts_t;
 hive_status_t hive_autotune_stream(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                    const uint64_t* h_offsets, size_t nbatches, hive_tune_t* out, void* stream);
+/* hive_autotune_stream with options (NULL = {0, -1, 0.1}: no persistent
+ * kernel, bounded by the launches' p99 plus 10 %). */
+hive_status_t hive_autotune_stream_ex(hive_table_t* t, const uint8_t* types, const uint32_t* keys,
+                                      const uint32_t* vals, const uint64_t* h_offsets, size_t nbatches,
+                                      const hive_tune_opts_t* opts, hive_tune_t* out, void* stream);
 
 /* Autotuner for the bulk kernels (no reference counterpart), at the table's
  * current load factor: times the insert of the sample keys (device arrays,
  * vals may be NULL) on a scratch copy of the table at 256/512/1024 threads
- * per block x {staged, inline} slow path, and the lookup of the sample lookup
+ * per block x {staged, inline, second launch} slow path, and the lookup of the
+ * sample lookup
  * keys on the table itself at 256/512/1024 threads x {1, 4, 8} lanes per key
  * and x {2, 4} keys per lane, then stores the fastest insert and lookup shapes
  * in the handle.  The table
@@ -271,6 +308,8 @@ typedef struct {
   uint32_t candidates;
   uint32_t lookup_keys_per_lane;  /* chosen keys per lane (1 with lanes_per_key > 1) */
   double lookup_ms_by_keys[3];    /* best lookup time with 1, 2 and 4 keys per lane (one lane per key) */
+  double insert_ms_by_slow_path[3]; /* best insert time with the slow path staged, inline, in a second
+                                       launch (0 = not timed) */
 } hive_bulk_tune_t;
 hive_status_t hive_autotune_bulk(hive_table_t* t, const uint32_t* insert_keys, const uint32_t* insert_vals,
                                  size_t n_insert, const uint32_t* lookup_keys, size_t n_lookup, hive_bulk_tune_t* out,
@@ -319,6 +358,14 @@ hive_status_t hive_autotune_layout(hive_table_t* t, const uint32_t* insert_keys,
  * keeps hbm_reserve_bytes after the new table; if not, requests that insert
  * are admitted while stash use stays below stash_headroom of the stash
  * capacity, and rejected otherwise (their ops get HIVE_R_FAILED, unexecuted).
+ * Shrink (min_load_factor > 0; off by default, as the reference's
+ * DYNAMIC_RESIZE, src/hive_dispatch.cu:189-206): once the deletes launched
+ * since the last exact count could have taken the load factor ((table +
+ * stash entries) / capacity) below min_load_factor, and after the last
+ * batch, the queue is drained and the exact count read; below
+ * min_load_factor the table is rehashed in place to ceil(entries /
+ * shrink_load_factor / slots per bucket) buckets (the reference's target of
+ * 75 % occupancy with the default 0.75).
  * h_latency_us[r] (host, may be NULL) receives arrival-to-completion time
  * (completion = the host observing the batch's event).  Synchronizes. */
 typedef struct {
@@ -329,6 +376,8 @@ typedef struct {
   uint64_t hbm_reserve_bytes; /* free HBM a resize must leave (0 = 1 GiB) */
   double stash_headroom;      /* admit inserts without a resize while stash use < this (0 = 0.5) */
   uint32_t ordered;           /* 1 = each request sees every earlier request's writes, also inside a batch */
+  double min_load_factor;     /* shrink below it (0 = never shrink; the reference's trigger is 0.25) */
+  double shrink_load_factor;  /* load factor right after a shrink, in (min_load_factor, max_load_factor] (0 = 0.75) */
 } hive_serve_config_t;
 typedef struct {
   uint64_t requests;
@@ -341,7 +390,9 @@ typedef struct {
   double mops;                /* ops / span */
   double p50_us, p99_us, max_us; /* request latency */
   double mean_batch_ops;
-  double resize_ms;           /* time spent resizing (inside span) */
+  double resize_ms;           /* time spent growing (inside span) */
+  uint64_t shrinks;           /* shrinks (resizes counts growths) */
+  double shrink_ms;           /* time spent shrinking, drain and count included */
 } hive_serve_report_t;
 hive_status_t hive_serve_trace(hive_table_t* t, const uint8_t* types, const uint32_t* keys, const uint32_t* vals,
                                const uint64_t* h_req_offsets, const double* h_arrival_us, size_t nreq,

@@ -480,11 +480,12 @@ def test_launch_shapes_give_identical_results(H, O):
     keys = W.distinct_keys(int(slots * 0.85), 71)
     lk, _ = W.lookup_set(keys, keys.size, 0.5, 72)
     ref = None
-    for blk, inl, comb in ((64, 0, 0), (256, 0, 0), (1024, 0, 0), (64, 1, 0), (512, 1, 0), (256, 0, 1), (1024, 1, 1)):
+    for blk, inl, comb in ((64, 0, 0), (256, 0, 0), (1024, 0, 0), (64, 1, 0), (512, 1, 0), (256, 0, 1), (1024, 1, 1),
+                           (256, 2, 0), (1024, 2, 0), (64, 2, 0)):
         t = H.HiveTable(slots)
-        for op in (0, 1, 2):
-            t.set_launch(op, blk, inline_slow_path=inl, combine_duplicates=comb)
-        assert t.get_launch(1) == (blk, 1, inl, comb, 1, 1)
+        for op in (0, 1, 2):  # inline_slow_path 2 (a second launch) is for inserts only
+            t.set_launch(op, blk, inline_slow_path=inl if op == 0 or inl < 2 else 0, combine_duplicates=comb)
+        assert t.get_launch(0) == (blk, 1, inl, comb, 1, 1)
         t.insert(keys)
         gv, gs = t.lookup(lk)
         cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
@@ -494,26 +495,33 @@ def test_launch_shapes_give_identical_results(H, O):
         ref = cur
 
 
-@pytest.mark.parametrize("policy,lf,layout", [("Default2Hash", 0.9, "bucket8"), ("MurmurCityHash", 0.95, "bucket8"),
-                                             ("Default2Hash", 1.0, "bucket8"), ("Default2Hash", 0.9, "bucket16"),
-                                             ("MurmurCityHash", 1.0, "bucket16"), ("Default2Hash", 0.9, "inline8"),
-                                             ("MurmurCityHash", 1.0, "inline8")])
-def test_full_size_properties(H, policy, lf, layout):
+@pytest.mark.parametrize("policy,lf,layout,slow", [("Default2Hash", 0.9, "bucket8", 0),
+                                                  ("MurmurCityHash", 0.95, "bucket8", 0),
+                                                  ("Default2Hash", 1.0, "bucket8", 0), ("Default2Hash", 0.9, "bucket16", 0),
+                                                  ("MurmurCityHash", 1.0, "bucket16", 0),
+                                                  ("Default2Hash", 0.9, "inline8", 0),
+                                                  ("MurmurCityHash", 1.0, "inline8", 0),
+                                                  ("Default2Hash", 0.9, "inline8", 2),
+                                                  ("Default2Hash", 1.0, "inline8", 2)])
+def test_full_size_properties(H, policy, lf, layout, slow):
     """2^26 slots at the sweep's load factors: size-independent properties
     (every placed key found with its value, absent keys not found, exact
-    counts, no key stored twice, re-insert = update)."""
+    counts, no key stored twice, re-insert = update); slow = 2: the insert
+    slow path in a second launch."""
     import torch
     width = SPB[layout]
-    print(f"[case] 2^26 slots LF {lf} {policy} layout {layout} full size", flush=True)
+    print(f"[case] 2^26 slots LF {lf} {policy} layout {layout} slow path {slow} full size", flush=True)
     slots = 1 << 26
     n = W.keys_for_lf(slots, lf)
     keys = W.distinct_keys(n, 44)
     vals = W.values(n, 45)
     absent = W.distinct_keys(1 << 22, 44, start=n + (n >> 6) + 64)
     t = H.HiveTable(slots, policy, layout=layout)
+    t.set_launch(0, 256, inline_slow_path=slow)
     st = t.insert(keys, vals)
     ok = placed(np_(st))
     assert ok.mean() > 0.9999
+    assert t.stats()["slow_path_ops"] > 0
     assert (np_(st)[~ok] == 0).all()  # the only other outcome: FAILED (stash full)
     kv = np_(t.dump()).view(np.uint64)
     assert kv.size == ok.sum() and np.unique(kv & np.uint64(0xFFFFFFFF)).size == kv.size
@@ -604,17 +612,40 @@ def test_stream_autotune_leaves_table_unchanged(H, O):
     t = H.HiveTable(slots)
     t.insert(pre)
     before = gpu_raw(t, O).live_kv()
+    windows = tuple(1 << e for e in range(15, 23))
+    coalesce = tuple(1 << e for e in range(10, 20))
+    # windows 2^15..2^22 x coalescing below 1/32..1/4 of them (at most 2^19):
+    # 31 pairs, of which the 19 with windows of >= 2^18 ops are also timed
+    # with type compaction
+    n_win = 31 + 19
+    # default: no persistent kernel; the choice is bounded by the launches' p99
     res = t.autotune_stream(types, keys, vals, off)
-    assert res["candidates"] == 51
-    assert res["window_ops"] in (1 << 20, 1 << 21, 1 << 22) and res["windowed_ms"] > 0
-    assert res["window_batch_ops"] in (1 << 17, 1 << 18, 1 << 19)
-    assert res["best_mode"] in (0, 1, 2, 3)
+    assert res["candidates"] == 8 + 1 + n_win  # launches (no big batches: split only), graph, windows
+    assert res["window_ops"] in windows and res["windowed_ms"] > 0 and res["windowed_p99_us"] > 0
+    assert res["window_batch_ops"] in coalesce and res["window_batch_ops"] <= res["window_ops"] // 4
+    assert res["best_mode"] in (0, 1, 3) and res["persistent_ms"] == 0
     assert res["block_threads"] in (128, 256, 512, 1024) and res["inline_slow_path"] in (0, 1)
-    assert res["tile_ops"] in (0, 256, 1024, 4096) and res["blocks_per_cu"] in (2, 4, 8)
-    assert res["stream_inline"] in (0, 1)
-    assert min(res["launches_ms"], res["graph_ms"], res["persistent_ms"]) > 0
+    assert min(res["launches_ms"], res["graph_ms"]) > 0 and res["launches_p99_us"] > 0
+    assert res["p99_bound_us"] == pytest.approx(res["launches_p99_us"] * 1.1)
+    if res["bounded_window_ops"]:
+        assert res["bounded_windowed_p99_us"] <= res["p99_bound_us"]
+        assert res["bounded_windowed_ms"] >= res["windowed_ms"]
+    assert set(res["windowed_ms_by_size"]) == set(windows)
+    assert min(res["windowed_ms_by_size"].values()) == pytest.approx(res["windowed_ms"])
+    assert all(v > 0 for v in res["windowed_p99_us_by_size"].values())
+    chosen_p99 = {0: res["launches_p99_us"], 1: res["graph_p99_us"], 3: res["bounded_windowed_p99_us"]}
+    assert chosen_p99[res["best_mode"]] <= res["p99_bound_us"] * (1 + 1e-9)
     assert np.array_equal(gpu_raw(t, O).live_kv(), before)
     assert t.get_launch(0) == (res["block_threads"], 1, res["inline_slow_path"], 0, res["split_by_type"], 1)
+    # with the persistent kernel and no latency bound: the fastest form
+    res = t.autotune_stream(types, keys, vals, off, try_persistent=True, max_p99_us=0.0)
+    assert res["candidates"] == 8 + 1 + 24 + n_win and res["p99_bound_us"] == 0
+    assert res["tile_ops"] in (0, 256, 1024, 4096) and res["blocks_per_cu"] in (2, 4, 8)
+    assert res["stream_inline"] in (0, 1) and res["persistent_ms"] > 0
+    assert res["bounded_window_ops"] == res["window_ops"] and res["bounded_windowed_ms"] == res["windowed_ms"]
+    best = min(res["launches_ms"], res["graph_ms"], res["persistent_ms"], res["windowed_ms"])
+    assert [res["launches_ms"], res["graph_ms"], res["persistent_ms"], res["windowed_ms"]][res["best_mode"]] == best
+    assert np.array_equal(gpu_raw(t, O).live_kv(), before)
 
 
 @pytest.mark.parametrize("policy,layout", [("Default2Hash", "bucket8"), ("TripleHash", "bucket8"),
@@ -682,8 +713,8 @@ def test_bulk_autotune_leaves_table_unchanged(H, O):
     lk, _ = W.lookup_set(keys, keys.size // 2, 0.5, 82)
     ref_v, ref_s = (np_(x) for x in t.lookup(lk))
     res = t.autotune_bulk(keys[keys.size // 2:], None, lk)
-    assert res["candidates"] == 21
-    assert res["insert_block_threads"] in (256, 512, 1024) and res["insert_inline_slow_path"] in (0, 1)
+    assert res["candidates"] == 9 + 15
+    assert res["insert_block_threads"] in (256, 512, 1024) and res["insert_inline_slow_path"] in (0, 1, 2)
     assert res["lookup_block_threads"] in (256, 512, 1024) and res["lookup_lanes_per_key"] in (1, 4, 8)
     assert res["lookup_keys_per_lane"] in (1, 2, 4)
     assert res["lookup_lanes_per_key"] == 1 or res["lookup_keys_per_lane"] == 1

@@ -327,3 +327,76 @@ def test_serve_cfg5_mixes_against_the_oracle(H, O, mix):
             assert np.array_equal(np.sort(np_(t.dump()).view(np.uint64) & np.uint64(0xFFFFFFFF)),
                                   np.sort(ref_content & np.uint64(0xFFFFFFFF))), name
         t.close()
+
+
+@pytest.mark.parametrize("layout", ["bucket8", "inline8"])
+def test_serve_shrinks_below_the_load_factor_threshold(H, O, layout):
+    """Opt-in shrink (min_load_factor 0.25, the reference's DYNAMIC_RESIZE
+    trigger, src/hive_dispatch.cu:189-206): a table filled to LF 1.0 (stash
+    populated) serves requests that delete 90 % of its keys, then
+    requests that insert, look up and delete.  Ordered serving must shrink
+    the table in place and still give every request the oracle's results run
+    request by request, ending with the oracle's content (keys and values).
+    The shrink sizes the table for the entries at LF 0.75."""
+    import math
+    print(f"[case] serve shrink, layout {layout}", flush=True)
+    slots = 1 << 14
+    pre = W.distinct_keys(slots, 701)
+    pre_v = W.values(pre.size, 702)
+    fresh = W.distinct_keys(2000, 701, start=pre.size + 2)
+    rng = np.random.default_rng(703)
+    order = rng.permutation(pre.size)
+    n_del = int(pre.size * 0.9)
+    dels = pre[order[:n_del]]
+    kept = pre[order[n_del:]]
+    reqs = []  # (types, keys)
+    for part in np.array_split(dels, 40):  # deletes, 40 requests
+        reqs.append((np.full(part.size, 2, np.uint8), part))
+    for i in range(15):  # then inserts of fresh keys, lookups of kept and deleted keys, a few deletes
+        ins = fresh[i * 100:(i + 1) * 100]
+        look = np.concatenate([rng.choice(kept, 60), rng.choice(dels, 20), ins[:10]])
+        de = rng.choice(kept, 5, replace=False)
+        ty = np.concatenate([np.zeros(ins.size, np.uint8), np.ones(look.size, np.uint8), np.full(de.size, 2, np.uint8)])
+        ks = np.concatenate([ins, look, de]).astype(np.uint32)
+        p = rng.permutation(ks.size)
+        reqs.append((ty[p], ks[p]))
+    types = np.concatenate([r[0] for r in reqs])
+    keys = np.concatenate([r[1] for r in reqs]).astype(np.uint32)
+    vals = W.values(keys.size, 704)
+    off = np.concatenate([[0], np.cumsum([r[1].size for r in reqs])]).astype(np.uint64)
+    nreq = off.size - 1
+    for name, arrivals in (("spaced", np.arange(nreq) * 40.0), ("saturated", np.zeros(nreq))):
+        t = H.HiveTable(slots, layout=layout)
+        pst = np_(t.insert(pre, pre_v))
+        assert t.stats()["stash_live"] > 0  # the stash is populated
+        start = np_(t.dump()).view(np.uint64)
+        o = O.Oracle(4 * slots // 8)
+        o.insert((start & np.uint64(0xFFFFFFFF)).astype(np.uint32), (start >> np.uint64(32)).astype(np.uint32))
+        ov = np.empty(keys.size, np.uint32)
+        os_ = np.empty(keys.size, np.uint8)
+        for r in range(nreq):
+            lo, hi = int(off[r]), int(off[r + 1])
+            ov[lo:hi], os_[lo:hi] = o.mixed(types[lo:hi], keys[lo:hi], vals[lo:hi])
+        ref_content = np.sort(o.dump())
+        gv, gs, lat, rep = t.serve(types, keys, vals, off, arrivals, ordered=1, min_load_factor=0.25)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        st = t.stats()
+        print(f"[case] serve shrink {layout} {name}: shrinks {rep['shrinks']}, final slots {rep['final_slots']}, "
+              f"live {st['table_live'] + st['stash_live']}, LF {st['load_factor']:.3f}", flush=True)
+        assert rep["shrinks"] >= 1 and rep["final_slots"] < slots and rep["rejected_requests"] == 0
+        assert st["capacity"] == rep["final_slots"]
+        assert np.array_equal(norm(gs), norm(os_)), name
+        assert np.array_equal(gv, ov), name
+        content = np.sort(np_(t.dump()).view(np.uint64))
+        assert np.array_equal(content, ref_content), name
+        assert 0.25 <= st["load_factor"] <= 0.9, name
+        if name == "saturated":  # one batch: the shrink after it sizes the table for LF 0.75
+            assert rep["final_slots"] == math.ceil(content.size / 0.75 / 8) * 8
+        t.close()
+    # without min_load_factor the table never shrinks (admission may still
+    # grow it: it counts a batch's inserts, not its deletes)
+    t = H.HiveTable(slots, layout=layout)
+    t.insert(pre, pre_v)
+    _, _, _, rep = t.serve(types, keys, vals, off, np.zeros(nreq), ordered=1)
+    assert rep["shrinks"] == 0 and rep["final_slots"] >= slots
+    t.close()
