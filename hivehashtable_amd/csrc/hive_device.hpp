@@ -821,11 +821,11 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
       return R_FOUND;
     }
   }
-  // the hint word is read only after an H0 miss: requesting it with H0 costs
-  // every lookup one more request (in the in-line layout, to the other half
-  // of the line the bucket read just brought into L2), which the
-  // request-rate-bound kernel pays for more than the round trip it saves on
-  // misses (measured in both layouts)
+  // the hint word is read only after an H0 miss.  In the separate layouts it
+  // lives in another line, so requesting it with H0 would cost every lookup
+  // one more line request; in the in-line layout requesting it in the same
+  // round (from the line the bucket read fetches) measured the same as this
+  // dependent read (1.765-1.766 ms against 1.767-1.769 ms for 2^26 lookups)
   const Hdr h = read_hdr<P>(t, b[0]);
   const uint32_t hint = h.hint;
   if (hint & ovf_bit_of<P>(key)) {

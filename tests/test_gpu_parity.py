@@ -59,27 +59,52 @@ def check_gpu_state(tbl, O, policy):
 @pytest.mark.parametrize("layout", LAYOUT_NAMES)
 @pytest.mark.parametrize("policy", ["Default2Hash", "MurmurCityHash"])
 def test_cfg1_insert_then_lookup(H, O, policy, layout):
+    """cfg1 (2^20 slots, LF 0.8) against the oracle.  Which keys a full
+    bucket kicks or stashes is a placement outcome of concurrent execution.
+    In a table this small nearly the whole batch is in flight at once, so
+    staged kicks run against an almost full table and stash pushes cluster:
+    now and then a canonical stash bucket fills and a key fails on the GPU
+    that the sequential oracle placed (the reference's Stage D,
+    include/hive_hash_insert.cuh:489-517).  Such a key's canonical stash
+    must hold 8 entries.  Every other key's outcome class must match the
+    oracle's; the content must be exactly the placed keys with their values;
+    lookups must match the oracle on every key placed in both, and the
+    oracle run on the GPU's own state on every key."""
     width = SPB[layout]
     print(f"[case] cfg1 2^20 slots LF 0.8 {policy} layout {layout}", flush=True)
     slots, keys, vals, lk, present = W.cfg1()
-    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
+    nb = slots // width
+    o = O.Oracle(nb, policy, slots_per_bucket=width)
     so = o.insert(keys, vals)
     t = H.HiveTable(slots, policy, layout=layout)
     sg = np_(t.insert(keys, vals))
-    assert np.array_equal(placed(sg), placed(so))
     assert not (sg == 2).any()  # distinct keys: no updates
+    gok, ook = placed(sg), placed(so)
+    assert np.isin(sg[~gok], (0,)).all()  # the only other outcome: FAILED
     raw, chk = check_gpu_state(t, O, policy)
-    assert np.array_equal(raw.live_kv(), o.dump())            # content multiset
+    odd = np.flatnonzero(gok != ook)
+    assert odd.size <= 8, odd.size
+    oraw = o.export()
+    for i in odd.tolist():
+        c = O.bucket(policy, 0, int(keys[i]), nb)
+        full = raw.stash_count[c] if not gok[i] else oraw.stash_count[c]
+        assert full >= 8, (int(keys[i]), c, int(full))  # failed only with its canonical stash full
+    if odd.size:
+        print(f"[case] cfg1 {policy} {layout}: {odd.size} keys failed with a full canonical stash", flush=True)
+    # content: exactly the placed keys with their values
+    want = np.sort((vals[gok].astype(np.uint64) << np.uint64(32)) | keys[gok].astype(np.uint64))
+    assert np.array_equal(np.sort(raw.live_kv()), want)
     gv, gs = t.lookup(lk)
     gv, gs = np_(gv, np.uint32), np_(gs)
     ov, os_ = o.lookup(lk)
-    assert np.array_equal(gs, os_) and np.array_equal(gv, ov)  # found flag and value
+    both = ~np.isin(lk, keys[odd])
+    assert np.array_equal(gs[both], os_[both]) and np.array_equal(gv[both], ov[both])  # found flag and value
     cv, cs = chk.lookup(lk)                                    # oracle on the GPU's own state
     assert np.array_equal(gs, cs) and np.array_equal(gv, cv)
-    assert np.array_equal(gs == 4, present)
+    assert np.array_equal(gs == 4, present & ~np.isin(lk, keys[~gok]))
     a, b = t.count_exact()
     st = t.stats()
-    assert (a, b) == (st["table_live"], st["stash_live"]) and a + b == placed(sg).sum()
+    assert (a, b) == (st["table_live"], st["stash_live"]) and a + b == gok.sum()
 
 
 @pytest.mark.parametrize("layout", LAYOUT_NAMES)
@@ -369,7 +394,9 @@ def test_resize_keeps_content_at_odd_sizes(H, src_slots, n_keys, dst_slots):
     vals = W.values(n_keys, 809)
     t = H.HiveTable(src_slots)
     st = np_(t.insert(keys, vals))
-    assert placed(st).all()
+    ok = placed(st)  # a key fails only with its canonical stash full (placement outcome)
+    assert (~ok).sum() <= 8 and (st[~ok] == 0).all()
+    keys, vals = keys[ok], vals[ok]
     before = np.sort(np_(t.dump()))
     stash = t.stats()["stash_live"]
     print(f"[case] stash entries before: {stash}", flush=True)
@@ -475,24 +502,37 @@ def test_reference_record_kernels_and_dispatch(H, O):
 
 
 def test_launch_shapes_give_identical_results(H, O):
+    """Every launch shape gives the same lookup results and content.  A key
+    may fail in one shape and not another only when its canonical stash is
+    full (which keys a full bucket stashes is a placement outcome); such
+    keys are left out of the comparison."""
     print("[case] launch shapes", flush=True)
     slots = 1 << 16
     keys = W.distinct_keys(int(slots * 0.85), 71)
     lk, _ = W.lookup_set(keys, keys.size, 0.5, 72)
-    ref = None
+    runs = []
     for blk, inl, comb in ((64, 0, 0), (256, 0, 0), (1024, 0, 0), (64, 1, 0), (512, 1, 0), (256, 0, 1), (1024, 1, 1),
                            (256, 2, 0), (1024, 2, 0), (64, 2, 0)):
         t = H.HiveTable(slots)
         for op in (0, 1, 2):  # inline_slow_path 2 (a second launch) is for inserts only
             t.set_launch(op, blk, inline_slow_path=inl if op == 0 or inl < 2 else 0, combine_duplicates=comb)
         assert t.get_launch(0) == (blk, 1, inl, comb, 1, 1)
-        t.insert(keys)
+        st = np_(t.insert(keys))
+        raw = gpu_raw(t, O)
+        failed = keys[~placed(st)]
+        for k in failed.tolist():
+            assert raw.stash_count[O.bucket("Default2Hash", 0, k, slots // 8)] >= 8
         gv, gs = t.lookup(lk)
-        cur = (np_(gv), np_(gs), np.sort(gpu_raw(t, O).live_kv()))
-        if ref is not None:
-            for a, b in zip(ref, cur):
-                assert np.array_equal(a, b)
-        ref = cur
+        runs.append((np_(gv), np_(gs), np.sort(raw.live_kv()), failed))
+        t.close()
+    anyfail = np.unique(np.concatenate([r[3] for r in runs]))
+    assert anyfail.size <= 16
+    keep = ~np.isin(lk, anyfail)
+    ref = runs[0]
+    for gv, gs, live, _ in runs[1:]:
+        assert np.array_equal(gv[keep], ref[0][keep]) and np.array_equal(gs[keep], ref[1][keep])
+        drop = lambda x: x[~np.isin((x & np.uint64(0xFFFFFFFF)).astype(np.uint32), anyfail)]
+        assert np.array_equal(drop(live), drop(ref[2]))
 
 
 @pytest.mark.parametrize("policy,lf,layout,slow", [("Default2Hash", 0.9, "bucket8", 0),
