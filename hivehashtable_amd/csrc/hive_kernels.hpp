@@ -141,6 +141,39 @@ __device__ __forceinline__ void wave_count(const TableView& t, int dtab, int dst
   }
 }
 
+// Per-block counter update, for kernels with a barrier after which every
+// lane calls it once (the block counters zeroed before that barrier): each
+// wavefront adds its sums to the block's LDS counters, and the last
+// wavefront to arrive adds the block's totals to the device counters -- one
+// device-scope atomic per counter per block instead of per wavefront.
+struct BlockCounts {
+  int tab, stash, slow;
+  uint32_t arrived;
+};
+__device__ __forceinline__ void block_count(const TableView& t, BlockCounts& bc, int dtab, int dstash, int dslow) {
+  const int wt = wave_sum(dtab);
+  const int ws = wave_sum(dstash);
+  const int wl = wave_sum(dslow);
+  if ((threadIdx.x & 63) == 0) {
+    if (wt) __hip_atomic_fetch_add(&bc.tab, wt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (ws) __hip_atomic_fetch_add(&bc.stash, ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (wl) __hip_atomic_fetch_add(&bc.slow, wl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t waves = blockDim.x >> 6;
+    if (__hip_atomic_fetch_add(&bc.arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == waves - 1u) {
+      const int a = __hip_atomic_load(&bc.tab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int b = __hip_atomic_load(&bc.stash, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int c = __hip_atomic_load(&bc.slow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const int lane = static_cast<int>(blockIdx.x & (kCounterLanes - 1));
+      if (a) atomicAdd(t.counters + (0 * kCounterLanes + lane) * kCounterStride,
+                       static_cast<unsigned long long>(static_cast<long long>(a)));
+      if (b) atomicAdd(t.counters + (1 * kCounterLanes + lane) * kCounterStride,
+                       static_cast<unsigned long long>(static_cast<long long>(b)));
+      if (c) atomicAdd(t.counters + (2 * kCounterLanes + lane) * kCounterStride,
+                       static_cast<unsigned long long>(static_cast<long long>(c)));
+    }
+  }
+}
+
 constexpr int kMaxBlock = 1024;
 
 // Optional completion clock of a launch: the last block to finish stamps the
@@ -195,7 +228,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   } else {
     __shared__ uint32_t queue[kMaxBlock];
     __shared__ uint32_t qn;
-    if (threadIdx.x == 0) qn = 0;
+    __shared__ BlockCounts bc;
+    if (threadIdx.x == 0) {
+      qn = 0;
+      bc = BlockCounts{0, 0, 0, 0u};
+    }
     __syncthreads();
     if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
       const uint8_t r = fast_insert<P>(t, key, val);
@@ -216,6 +253,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
       dstash += (r == R_STASHED);
       dslow = 1;
     }
+    block_count(t, bc, dtab, dstash, dslow);
+    clk.finish();
+    return;
   }
   wave_count(t, dtab, dstash, dslow);
   clk.finish();
