@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--block", type=int, default=0, help="threads per block for the bulk kernels")
+    p.add_argument("--insert-slow-path", type=int, default=-1, choices=[-1, 0, 1, 2],
+                   help="insert slow path: 0 staged, 1 inline, 2 second launch (default: tuned; profiling runs)")
     p.add_argument("--layout", default="auto", choices=["auto", "bucket8", "bucket16", "inline8"],
                    help="bucket layout of the headline table (auto: timed on the workload, fastest kept)")
     p.add_argument("--no-mixed", action="store_true", help="skip the uneven mixed-batch stream leg")
@@ -680,6 +682,8 @@ def main():
     if a.block:
         for op in (0, 1, 2):
             t.set_launch(op, a.block)
+    if a.insert_slow_path >= 0:
+        t.set_launch(0, a.block or 256, inline_slow_path=a.insert_slow_path)
     stream = torch.cuda.current_stream()
     layout_tune = None
     if a.layout == "auto":
@@ -692,7 +696,7 @@ def main():
     spb = t.slots_per_bucket
     nb = t.num_buckets
     shape_tune = None
-    if not a.block and not a.no_autotune:
+    if not a.block and not a.no_autotune and a.insert_slow_path < 0:
         # insert shapes (block size, slow path staged / inline / second launch)
         # timed on a scratch copy of the empty table; then lookup shapes (block
         # size, lanes per key, keys per lane) on the filled table, at the

@@ -102,8 +102,13 @@ inline int table_kind(const hive_table* t) {
   }
 
 // The pending list of split inserts (inline_slow_path = 2) for batches of up
-// to n ops, allocated by the bulk insert entry point and the bulk autotuner,
-// never under stream capture; launches without a list big enough run staged.
+// to n ops (at any block size), allocated by the bulk insert entry point and
+// the bulk autotuner, never under stream capture; launches without a list
+// big enough run staged.  Sub-list capacity: the ops of the blocks of one
+// class (blockIdx mod kPendingLists), at most ceil(blocks / kPendingLists)
+// blocks of at most kMaxBlock threads each, which n / kPendingLists +
+// kMaxBlock bounds for any block size.
+inline size_t pending_sub_cap(size_t n) { return (n + kPendingLists - 1) / kPendingLists + kMaxBlock; }
 // (force: whatever the table's insert launch option is)
 inline hive_status_t ensure_pending(hive_table* t, size_t n, hipStream_t s, bool force = false) {
   if ((!force && t->launch[OP_INSERT].inline_slow_path != 2) || n <= t->pend_cap || n > 0xFFFFFFFFull) return HIVE_OK;
@@ -112,8 +117,9 @@ inline hive_status_t ensure_pending(hive_table* t, size_t n, hipStream_t s, bool
     (void)hipGetLastError();
     return HIVE_OK;
   }
+  const size_t ent = pending_sub_cap(n) * kPendingLists;
   unsigned long long* p = nullptr;
-  if (hipMalloc(&p, n * 12 + 256) != hipSuccess) {
+  if (hipMalloc(&p, ent * 12 + static_cast<size_t>(kPendingLists) * kPendingCountStride * 4 + 256) != hipSuccess) {
     (void)hipGetLastError();
     return HIVE_OK;  // no list: split inserts run staged
   }
@@ -126,8 +132,10 @@ inline hive_status_t ensure_pending(hive_table* t, size_t n, hipStream_t s, bool
   return HIVE_OK;
 }
 inline PendingList pending_list(const hive_table* t) {
-  uint32_t* idx = reinterpret_cast<uint32_t*>(t->pend + t->pend_cap);
-  return PendingList{t->pend, idx, reinterpret_cast<uint32_t*>(t->pend + t->pend_cap + (t->pend_cap + 1) / 2)};
+  const size_t sub = pending_sub_cap(t->pend_cap), ent = sub * kPendingLists;
+  uint32_t* idx = reinterpret_cast<uint32_t*>(t->pend + ent);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(t->pend + ent + (ent + 1) / 2);
+  return PendingList{t->pend, idx, cnt, sub};
 }
 
 template <class Src, class Out>
@@ -142,7 +150,8 @@ hive_status_t launch_insert(hive_table* t, Src src, size_t n, Out out, int phase
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
     const unsigned slow_grid = static_cast<unsigned>(std::max(1, ncu) * 16);
-    k_fill_u32<<<1, 64, 0, s>>>(pl.count, 1, 0u);
+    k_fill_u32<<<kPendingLists * kPendingCountStride / 64, 64, 0, s>>>(pl.count, kPendingLists * kPendingCountStride,
+                                                                       0u);
     HIVE_POLICY_SWITCH(table_kind(t), (k_insert_fast<P, Src, Out><<<grid_for(n, blk), blk, 0, s>>>(t->v, src, n, out,
                                                                                                 phase, pl)));
     HIVE_POLICY_SWITCH(table_kind(t), (k_insert_slow<P, Out><<<slow_grid, 256, 0, s>>>(t->v, pl, out, clk)));

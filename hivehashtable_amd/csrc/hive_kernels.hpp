@@ -263,16 +263,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
 
 // Split insert (launch option inline_slow_path = 2): two launches.  The first
 // runs every lane's fast path and appends the ops that must take the slow
-// path to a device list (key and value packed, and the op's index; one
-// atomic per wavefront); the second runs the slow path over that list.  The
-// fast-path kernel then carries only the fast path's registers (51 VGPRs,
-// 8 waves per SIMD, against 78 and 6 for k_insert with the in-line 8-slot
-// layout), so more of its single CASes are in flight.  *count is zero on
-// entry to the first launch.
+// path to a device list (key and value packed, and the op's index); the
+// second runs the slow path over that list.  The fast-path kernel then
+// carries only the fast path's registers (51 VGPRs, 8 waves per SIMD,
+// against 78 and 6 for k_insert with the in-line 8-slot layout), so more of
+// its single CASes are in flight.  The list is kPendingLists sub-lists, one
+// per class of blocks (blockIdx mod kPendingLists), each with its own
+// counter on its own 128-byte line: appends from every wavefront to one
+// counter serialized at one address (the fast kernel took 7.7 ms instead of
+// the 3 its atomics need).  Sub-list l holds the ops of the blocks of its
+// class, at most sub_cap; its counter is zero on entry to the first launch.
+constexpr uint32_t kPendingLists = 64;
+constexpr uint32_t kPendingCountStride = 32;  // u32 between sub-list counters (128 B)
 struct PendingList {
   unsigned long long* kv;
   uint32_t* idx;
-  uint32_t* count;
+  uint32_t* count;   // [kPendingLists * kPendingCountStride]
+  size_t sub_cap;    // ops per sub-list
 };
 
 template <class P, class Src, class Out>
@@ -296,11 +303,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_fast(TableView t, Src src,
   const unsigned long long b = __ballot(pend);
   if (b) {
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t l = blockIdx.x % kPendingLists;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(pl.count, static_cast<uint32_t>(__popcll(b)));
+    if (lane == 0) base = atomicAdd(pl.count + l * kPendingCountStride, static_cast<uint32_t>(__popcll(b)));
     base = static_cast<uint32_t>(__shfl(static_cast<int>(base), 0));
     if (pend) {
-      const uint32_t j = base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
+      const size_t j = l * pl.sub_cap + base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
       pl.kv[j] = kv_pack(key, val);
       pl.idx[j] = static_cast<uint32_t>(i);
     }
@@ -308,15 +316,35 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_fast(TableView t, Src src,
   wave_count(t, dtab, 0, 0);
 }
 
+// The pending ops of all sub-lists form one index space (their counts'
+// prefix sums, computed by each block in LDS), so the whole grid works on
+// all of them at once.
 template <class P, class Out>
 __global__ __launch_bounds__(256) void k_insert_slow(TableView t, PendingList pl, Out out,
                                                      BatchClock clk = BatchClock{nullptr, nullptr}) {
-  const uint32_t cnt = *pl.count;  // written by the previous launch
+  __shared__ uint32_t start[kPendingLists + 1];  // start[l]: first global index of sub-list l
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t l = 0; l < kPendingLists; ++l) {
+      start[l] = acc;
+      acc += pl.count[l * kPendingCountStride];  // written by the previous launch
+    }
+    start[kPendingLists] = acc;
+  }
+  __syncthreads();
+  const uint32_t total = start[kPendingLists];
   int dtab = 0, dstash = 0, dslow = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
-    const unsigned long long kv = pl.kv[j];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = kPendingLists;  // the sub-list l with start[l] <= g < start[l + 1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (start[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    const size_t e = lo * pl.sub_cap + (g - start[lo]);
+    const unsigned long long kv = pl.kv[e];
     const uint8_t r = slow_insert<P>(t, kv_key(kv), kv_val(kv));
-    out.put(pl.idx[j], r, 0);
+    out.put(pl.idx[e], r, 0);
     dtab += (r == R_INSERTED);
     dstash += (r == R_STASHED);
     ++dslow;
