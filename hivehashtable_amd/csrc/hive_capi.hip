@@ -514,7 +514,8 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
     HIP_TRY(hipMemcpyAsync(dst, h_stash, nb * kStashSlots * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(dc, h_stash_count, nb * 4, hipMemcpyHostToDevice, s));
     k_fill_u64<<<1, 64, 0, s>>>(t->scratch, 2, 0ull);
-    k_fill_u32<<<1, 64, 0, s>>>(t->v.pool_next, 1, 0u);
+    k_fill_u32<<<kPoolCursors * kPoolCursorStride / 64, 64, 0, s>>>(t->v.pool_next, kPoolCursors * kPoolCursorStride,
+                                                                     0u);
     HIVE_POLICY_SWITCH(table_kind(t), (k_raw_to_inline<P><<<grid_for(nb, 256), 256, 0, s>>>(t->v, ds, dst, dc, t->scratch),
                                        k_rebuild_meta<P><<<grid_for(nb * t->v.S, 256), 256, 0, s>>>(t->v)));
     HIP_CHECK_LAUNCH();

@@ -348,8 +348,8 @@ inline int layout_of(const TableView& v) {
 }
 // stash buckets of an in-line table's pool: a quarter of the buckets
 inline uint32_t pool_buckets(uint32_t nb) { return nb / 4 + 64; }
-// counters, then (in-line layout) the pool cursor on a line of its own
-inline size_t counter_bytes() { return counter_words() * 8 + 128; }
+// counters, then (in-line layout) the pool cursors, each on a line of its own
+inline size_t counter_bytes() { return counter_words() * 8 + static_cast<size_t>(kPoolCursors) * kPoolCursorStride * 4; }
 
 inline size_t view_bytes(uint32_t nb, int layout) {
   if (layout == HIVE_LAYOUT_INLINE8)

@@ -870,12 +870,12 @@ template <class P>
 __global__ void k_reset_stash(TableView t) {
   const size_t w = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if constexpr (P::kInline) {
+    if (w < kPoolCursors) t.pool_next[w * kPoolCursorStride] = 0u;  // launched with at least 256 threads
     if (w >= t.nb) return;
     uint32_t* h = header_ptr<P>(t, static_cast<uint32_t>(w));
     h[0] &= ~kInlineStashBit;
     h[2] = 0u;
     h[3] = 0u;
-    if (w == 0) *t.pool_next = 0u;
   } else {
     uint32_t keep = 0;
     for (uint32_t j = 0; j < kHintsPerWord; ++j) keep |= kOverflowMask << (j * kHintBits);
@@ -945,14 +945,14 @@ __global__ void k_raw_to_inline(TableView t, const unsigned long long* slots, co
   h[3] = 0u;
   const uint32_t n = count[b];
   if (n == 0u) return;
-  const uint32_t idx = atomicAdd(t.pool_next, 1u);
-  if (idx >= t.pool_cap) {
+  const uint32_t ref = pool_take(t, static_cast<uint32_t>(b));
+  if (ref == 0u) {
     atomicAdd(failed, 1ull);
     return;
   }
   h[2] = n;
-  h[3] = idx + 1u;
-  unsigned long long* sb = t.stash + static_cast<size_t>(idx) * kStashSlots;
+  h[3] = ref;
+  unsigned long long* sb = t.stash + static_cast<size_t>(ref - 1u) * kStashSlots;
   for (int j = 0; j < kStashSlots; ++j) sb[j] = stash[b * kStashSlots + j];
 }
 
