@@ -255,9 +255,13 @@ inline bool use_partition(const hive_table* t, size_t n, const uint8_t* types) {
   return t->launch[OP_INSERT].split_by_type && n >= kCompactMinOps && n <= 0xFFFFFFFFull && types;
 }
 inline hive_status_t partition_batch(SrcNative src, size_t n, uint32_t* part, hipStream_t s) {
+  // about 512 blocks, each owning a whole number of 1024-op tiles
+  constexpr size_t kBlocks = 512;
+  const size_t tiles = (n + kPartitionBlock - 1) / kPartitionBlock;
+  const size_t span = ((tiles + kBlocks - 1) / kBlocks) * kPartitionBlock;
   k_fill_u32<<<1, 64, 0, s>>>(part + 9 * n, 3, 0u);
-  k_partition<<<grid_for(n, 1024), 1024, 0, s>>>(src.types, src.keys, src.vals, n, part, part + 3 * n, part + 6 * n,
-                                                 part + 9 * n);
+  k_partition<<<grid_for(n, static_cast<uint32_t>(span)), kPartitionBlock, 0, s>>>(
+      src.types, src.keys, src.vals, n, span, part, part + 3 * n, part + 6 * n, part + 9 * n);
   HIP_CHECK_LAUNCH();
   return HIVE_OK;
 }

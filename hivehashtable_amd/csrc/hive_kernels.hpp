@@ -430,47 +430,77 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src 
 // Lookups stay in the batch (their phase kernel filters by type; they are the
 // bulk of read-heavy batches, where a copy would only add traffic).  Order
 // within a list is arbitrary: a phase's ops are concurrent.
-static __global__ __launch_bounds__(1024) void k_partition(const uint8_t* types, const uint32_t* keys,
-                                                          const uint32_t* vals, size_t n, uint32_t* lkeys,
-                                                          uint32_t* lvals, uint32_t* lidx, uint32_t* cnt) {
-  __shared__ uint32_t c[3], base[3];
-  if (threadIdx.x < 3) c[threadIdx.x] = 0;
-  __syncthreads();
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int ty = -1;
-  if (i < n) {
-    ty = static_cast<int>(types[i]);
-    if (ty != OP_INSERT && ty != OP_DELETE) ty = -1;
+// A block owns `span` consecutive ops (a multiple of its 1024 threads): it
+// counts them by type first (wavefront ballots, LDS totals), takes its
+// positions in each list with one device-scope atomic per type, then ranks
+// and writes them.  Device-scope atomics on one address serialize (about
+// 0.2 G/s), so one atomic per 1024 ops -- 8 k on two counters for a 4 M-op
+// batch -- held the kernel near that rate; a span of many tiles keeps it to
+// a few hundred.
+constexpr uint32_t kPartitionBlock = 1024;
+static __global__ __launch_bounds__(kPartitionBlock) void k_partition(const uint8_t* types, const uint32_t* keys,
+                                                                      const uint32_t* vals, size_t n, size_t span,
+                                                                      uint32_t* lkeys, uint32_t* lvals, uint32_t* lidx,
+                                                                      uint32_t* cnt) {
+  __shared__ uint32_t c[3], run[3], base[3];
+  if (threadIdx.x < 3) {
+    c[threadIdx.x] = 0;
+    run[threadIdx.x] = 0;
   }
-  // ranks within the wavefront by ballot; one LDS atomic per wavefront and list
+  __syncthreads();
+  const size_t lo = static_cast<size_t>(blockIdx.x) * span;
+  const size_t hi = lo + span < n ? lo + span : n;
   const uint32_t lane = threadIdx.x & 63u;
   const unsigned long long below = (1ull << lane) - 1ull;
-  const unsigned long long mi = __ballot(ty == OP_INSERT), md = __ballot(ty == OP_DELETE);
-  uint32_t wi = 0, wd = 0;
-  if (mi) {
-    const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(mi)) - 1);
-    uint32_t x = 0;
-    if (lane == lead) x = atomicAdd(&c[OP_INSERT], static_cast<uint32_t>(__popcll(mi)));
-    wi = __shfl(x, static_cast<int>(lead));
-  }
-  if (md) {
-    const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(md)) - 1);
-    uint32_t x = 0;
-    if (lane == lead) x = atomicAdd(&c[OP_DELETE], static_cast<uint32_t>(__popcll(md)));
-    wd = __shfl(x, static_cast<int>(lead));
+  auto type_at = [&](size_t i) {
+    int ty = -1;
+    if (i < hi) {
+      ty = static_cast<int>(types[i]);
+      if (ty != OP_INSERT && ty != OP_DELETE) ty = -1;
+    }
+    return ty;
+  };
+  // pass 1: the block's counts per type
+  for (size_t t0 = lo; t0 < hi; t0 += blockDim.x) {
+    const int ty = type_at(t0 + threadIdx.x);
+    const unsigned long long mi = __ballot(ty == OP_INSERT), md = __ballot(ty == OP_DELETE);
+    if (lane == 0) {
+      if (mi) atomicAdd(&c[OP_INSERT], static_cast<uint32_t>(__popcll(mi)));
+      if (md) atomicAdd(&c[OP_DELETE], static_cast<uint32_t>(__popcll(md)));
+    }
   }
   __syncthreads();
   if ((threadIdx.x == OP_INSERT || threadIdx.x == OP_DELETE) && c[threadIdx.x])
     base[threadIdx.x] = atomicAdd(cnt + threadIdx.x, c[threadIdx.x]);
   __syncthreads();
-  if (ty >= 0) {
-    const bool ins = ty == OP_INSERT;
-    const size_t pos = static_cast<size_t>(ty) * n + base[ty] + (ins ? wi : wd) +
-                       static_cast<uint32_t>(__popcll((ins ? mi : md) & below));
-    const uint32_t key = keys[i];
-    lkeys[pos] = key;
-    lvals[pos] = vals ? vals[i] : key + 1u;
-    lidx[pos] = static_cast<uint32_t>(i);
+  // pass 2: ranks within the block (wavefront ballot, one LDS atomic per
+  // wavefront and list) and the writes
+  for (size_t t0 = lo; t0 < hi; t0 += blockDim.x) {
+    const size_t i = t0 + threadIdx.x;
+    const int ty = type_at(i);
+    const unsigned long long mi = __ballot(ty == OP_INSERT), md = __ballot(ty == OP_DELETE);
+    uint32_t wi = 0, wd = 0;
+    if (mi) {
+      const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(mi)) - 1);
+      uint32_t x = 0;
+      if (lane == lead) x = atomicAdd(&run[OP_INSERT], static_cast<uint32_t>(__popcll(mi)));
+      wi = __shfl(x, static_cast<int>(lead));
+    }
+    if (md) {
+      const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(md)) - 1);
+      uint32_t x = 0;
+      if (lane == lead) x = atomicAdd(&run[OP_DELETE], static_cast<uint32_t>(__popcll(md)));
+      wd = __shfl(x, static_cast<int>(lead));
+    }
+    if (ty >= 0) {
+      const bool ins = ty == OP_INSERT;
+      const size_t pos = static_cast<size_t>(ty) * n + base[ty] + (ins ? wi : wd) +
+                         static_cast<uint32_t>(__popcll((ins ? mi : md) & below));
+      const uint32_t key = keys[i];
+      lkeys[pos] = key;
+      lvals[pos] = vals ? vals[i] : key + 1u;
+      lidx[pos] = static_cast<uint32_t>(i);
+    }
   }
 }
 
