@@ -641,6 +641,9 @@ __device__ __forceinline__ bool finish_move(unsigned long long* src, unsigned lo
 // The whole protocol for one key while holding lock(H0(key)):
 //   Stage A / A' re-check   include/hive_hash_insert.cuh:522-541, :166-218
 //   Stage B re-try          include/hive_hash_insert.cuh:586-614
+//   Stage B at H1           not in the reference, whose new keys go to H0:
+//                           an empty slot of the key's own H1, before any
+//                           kick (8-slot buckets)
 //   Stage C cuckoo kicks    include/hive_hash_insert.cuh:574-692
 //   Stage D stash push      include/hive_hash_insert.cuh:489-517
 // The victim is copied into its alternate bucket (claimed by CAS 0 -> victim)
@@ -665,6 +668,14 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
   // than the two buckets of one round are live in registers.
   uint32_t pre_empty = 0;  // empty slots of pre_alt, read with the first coherent round
   bool have_pre = false;
+  // 8-slot buckets: the key's own H1 (and its empty slots), read in the same
+  // round.  A claim there places the key with one CAS instead of a kick's
+  // two (copy, then replace); its hint bit rides on the unlock where
+  // hint_at_unlock<P>().  (16-slot buckets keep their registers for the
+  // round's two buckets and take the slow path half as often.)
+  constexpr bool kOwnH1 = S == 8;
+  const uint32_t own_alt = kOwnH1 ? bucket_of<P>(1, key, t.nb) : c;
+  uint32_t own_empty = 0;
   for (int attempt = 0; attempt < 256; ++attempt) {
     bool retry = false;
     // one coherent round: the canonical bucket and its hint bits.  Only
@@ -683,6 +694,11 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
         read_bucket<P>(bucket_ptr<P>(t, pre_alt), pre);
         pre_empty = empty_mask(pre);
         have_pre = true;
+      }
+      if (kOwnH1 && attempt == 0 && own_alt != c) {
+        unsigned long long oa[S];
+        read_bucket<P>(bucket_ptr<P>(t, own_alt), oa);
+        own_empty = empty_mask(oa);
       }
       m = match_mask(cs, key);
       e = empty_mask(cs);
@@ -731,6 +747,25 @@ __device__ uint8_t locked_insert(const TableView& t, uint32_t key, uint32_t val,
       e &= e - 1;
     }
     if (retry) continue;
+    if constexpr (kOwnH1) {
+      // Stage B at H1.  Slots are claimed only by CAS 0 -> kv and c is full,
+      // so c's lock holder is the only one placing this key; the stale plain
+      // view of H1 can only overstate its empty slots, and the CAS decides.
+      // The key's class bit in c is published before the unlock (with it, or
+      // by an atomic the unlock waits for): an insert of the same key finds
+      // c full and reads the bit under the lock.
+      unsigned long long* ob = bucket_ptr<P>(t, own_alt);
+      while (own_empty) {
+        const int j = __builtin_ctz(own_empty);
+        own_empty &= own_empty - 1;
+        if (cas64(ob + j, 0ull, kv) == 0ull) {
+          const uint32_t kbit = ovf_bit_of<P>(key);
+          if constexpr (hint_at_unlock<P>()) pend |= kbit;
+          else if (!(hint & kbit)) hint_set<P>(t, c, kbit);
+          return R_INSERTED;
+        }
+      }
+    }
 #pragma unroll
     for (int kick = 0; kick < kMaxKicks; ++kick) {
       const int vs = kick % S;
