@@ -21,13 +21,18 @@ namespace hive {
 // kernels resolve), -1 for every caller-supplied source, whose ops of any
 // type other than insert / lookup / delete get R_NOOP
 // (include/hive_kernels.cuh:181-184).
+// Batch inputs are read once and results written once: nontemporal loads and
+// stores keep them from displacing table lines in L2 (2^26 lookups at LF 0.9:
+// 1.763 -> 1.741 ms).
 struct SrcNative {
   const uint32_t* keys;
   const uint32_t* vals;   // NULL: value = key + 1
   const uint8_t* types;   // NULL: every op has the kernel's own type
   int deferred = -1;      // OP_DEFERRED only for a window's internal types
-  __device__ __forceinline__ uint32_t key(size_t i) const { return keys[i]; }
-  __device__ __forceinline__ uint32_t val(size_t i, uint32_t k) const { return vals ? vals[i] : k + 1u; }
+  __device__ __forceinline__ uint32_t key(size_t i) const { return __builtin_nontemporal_load(keys + i); }
+  __device__ __forceinline__ uint32_t val(size_t i, uint32_t k) const {
+    return vals ? __builtin_nontemporal_load(vals + i) : k + 1u;
+  }
   __device__ __forceinline__ int type(size_t i, int dflt) const { return types ? static_cast<int>(types[i]) : dflt; }
   __device__ __forceinline__ int deferred_type() const { return deferred; }
 };
@@ -85,8 +90,8 @@ struct OutNative {
   uint8_t* status;
   uint32_t* vals;
   __device__ __forceinline__ void put(size_t i, uint8_t st, uint32_t v) const {
-    if (status) status[i] = st;
-    if (vals) vals[i] = v;
+    if (status) __builtin_nontemporal_store(st, status + i);
+    if (vals) __builtin_nontemporal_store(v, vals + i);
   }
 };
 
