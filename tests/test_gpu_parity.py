@@ -148,16 +148,22 @@ def test_lookup_and_erase_on_oracle_built_table(H, O, policy, layout):
 
 
 @pytest.mark.parametrize("layout", LAYOUT_NAMES)
-def test_insert_at_full_load_keeps_invariants(H, O, layout):
+@pytest.mark.parametrize("policy", ["Default2Hash", "TripleHash", "MurmurCityBitHash", "Lookup3Hash"])
+def test_insert_at_full_load_keeps_invariants(H, O, layout, policy):
+    """LF 1.0: most of the last keys take the locked slow path (own H1,
+    kicks, stash pushes), in two- and three-hash policies.  The oracle's
+    structural check on the GPU's state (every key once, in a candidate
+    bucket or its canonical stash), and every placed key found by the GPU's
+    lookup, which follows the hint bits."""
     width = SPB[layout]
-    print(f"[case] LF 1.0 insert, 2^15 slots, layout {layout}", flush=True)
+    print(f"[case] LF 1.0 insert, 2^15 slots, {policy} layout {layout}", flush=True)
     slots = 1 << 15
     keys = W.distinct_keys(slots, 21)
-    o = O.Oracle(slots // width, slots_per_bucket=width)
+    o = O.Oracle(slots // width, policy, slots_per_bucket=width)
     so = o.insert(keys)
-    t = H.HiveTable(slots, layout=layout)
+    t = H.HiveTable(slots, policy, layout=layout)
     sg = np_(t.insert(keys))
-    raw, chk = check_gpu_state(t, O, "Default2Hash")
+    raw, chk = check_gpu_state(t, O, policy)
     ok = placed(sg)
     assert raw.live_kv().size == ok.sum()
     gv, gs = t.lookup(keys)
