@@ -523,6 +523,8 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
     HIP_TRY(hipMemcpyAsync(&failed, t->scratch, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (failed) return fail(HIVE_ECAPACITY, "import: more stash buckets than the stash pool holds");
+    k_fill_u32<<<1, 64, 0, s>>>(t->v.holes, 1, 1u);  // an imported state may hold hint bits on non-full buckets
+    HIP_CHECK_LAUNCH();
     uint64_t a = 0, b = 0;
     if ((r = count_exact_view(t, t->v, s, &a, &b)) != HIVE_OK) return r;
     return reset_counters(t, t->v, s, a, b);
@@ -533,6 +535,7 @@ hive_status_t hive_import_raw(hive_table_t* t, const uint64_t* h_slots, const ui
   k_fill_u32<<<fill_grid(meta_words(t->v.nb)), 256, 0, s>>>(t->v.meta, meta_words(t->v.nb), 0u);
   HIVE_POLICY_SWITCH(table_kind(t), (k_rebuild_meta<P><<<grid_for(nb * t->v.S, 256), 256, 0, s>>>(t->v)));
   k_fill_u32<<<fill_grid(t->v.nb), 256, 0, s>>>(t->v.locks, t->v.nb, 0u);
+  k_fill_u32<<<1, 64, 0, s>>>(t->v.holes, 1, 1u);  // an imported state may hold hint bits on non-full buckets
   HIP_CHECK_LAUNCH();
   uint64_t a = 0, b = 0;
   if ((r = count_exact_view(t, t->v, s, &a, &b)) != HIVE_OK) return r;

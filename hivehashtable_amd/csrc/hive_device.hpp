@@ -82,6 +82,7 @@ struct TableView {
   uint32_t* locks;                  // nb (separate layout)
   unsigned long long* counters;     // [3][kCounterLanes][kCounterStride]: table, stash, slow-path ops
   uint32_t* pool_next;              // pool cursors, kPoolCursorStride apart (in-line header layout)
+  uint32_t* holes;                  // nonzero once an erase has emptied a bucket slot (see note_hole)
   uint32_t nb;
   uint32_t S;                       // slots per bucket (8 or 16), for code not instantiated per layout
   uint32_t layout;                  // 0 separate arrays, 1 in-line header
@@ -859,8 +860,12 @@ __device__ uint8_t slow_insert(const TableView& t, uint32_t key, uint32_t val) {
 // include/hive_hash_lookup.cuh:126-155: H0, then the other candidates (only
 // if H0's overflow bit is set), then stash[H0] (if its stash bit is set).
 // The hint word is read only after an H0 miss.  Returns R_FOUND / R_NOT_FOUND.
+// full_hints: the table has the full-hint property (no slot emptied by an
+// erase since it was cleared or rebuilt; see note_hole), so a miss in an H0
+// with an empty slot is final.
 template <class P>
-__device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, uint32_t& val) {
+__device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, uint32_t& val,
+                                              bool full_hints = false) {
   val = 0;
   if (key == 0) return R_NOT_FOUND;
   uint32_t b[P::NH];
@@ -874,6 +879,7 @@ __device__ __forceinline__ uint8_t lookup_one(const TableView& t, uint32_t key, 
       val = kv_val(s[__builtin_ctz(m)]);
       return R_FOUND;
     }
+    if (full_hints && empty_mask(s) != 0u) return R_NOT_FOUND;
   }
   // the hint word is read only after an H0 miss.  In the separate layouts it
   // lives in another line, so requesting it with H0 would cost every lookup
@@ -937,6 +943,23 @@ __device__ __forceinline__ int erase_bucket(unsigned long long* bk, uint32_t key
 }
 
 // lowest matching entry among the first cnt stash entries
+// Full-hint property.  A bucket's hint bits (an overflow class or the stash
+// bit) are set only while the bucket is full: a kick moves a key out of its
+// full H0 (or, with three hashes, out of a bucket it reached after its full
+// H0), a new key goes to its own H1 or the stash only when its H0 is full.
+// Slots are emptied only by erases (a kick's transient copy is withdrawn only
+// when its source no longer holds the victim, which takes an erase), so as
+// long as no erase has emptied a slot since the table was cleared or rebuilt
+// by a rehash, every bucket with a hint bit set is still full.  A lookup
+// whose key is not in an H0 with an empty slot then has its answer without
+// reading the hint word.  t.holes records the first such erase: zeroed with
+// the counters (clear, a new view for a rehash), set by erase_one when it
+// empties a bucket slot and by a raw import.  A plain load first, so that
+// each L2 stores it about once.
+__device__ __forceinline__ void note_hole(const TableView& t) {
+  if (*t.holes == 0u) *t.holes = 1u;
+}
+
 template <class P>
 __device__ __forceinline__ int erase_stash(unsigned long long* sb, uint32_t cnt, uint32_t key) {
   for (uint32_t j = 0; j < cnt; ++j) {
@@ -965,6 +988,7 @@ __device__ __forceinline__ uint8_t erase_one(const TableView& t, uint32_t key, i
     const uint32_t cnt = stash_count_of<P>(t, b[0], h);
     removed_stash = erase_stash<P>(stash_of<P>(t, b[0], h), cnt, key);
   }
+  if (removed_tab) note_hole(t);
   return (removed_tab + removed_stash) ? R_ERASED : R_NOT_FOUND;
 }
 

@@ -352,8 +352,10 @@ inline int layout_of(const TableView& v) {
 }
 // stash buckets of an in-line table's pool: a quarter of the buckets
 inline uint32_t pool_buckets(uint32_t nb) { return nb / 4 + 64; }
-// counters, then (in-line layout) the pool cursors, each on a line of its own
-inline size_t counter_bytes() { return counter_words() * 8 + static_cast<size_t>(kPoolCursors) * kPoolCursorStride * 4; }
+// counters, then (in-line layout) the pool cursors, each on a line of its own,
+// then the holes flag on a line of its own
+inline size_t pool_cursor_bytes() { return static_cast<size_t>(kPoolCursors) * kPoolCursorStride * 4; }
+inline size_t counter_bytes() { return counter_words() * 8 + pool_cursor_bytes() + 128; }
 
 inline size_t view_bytes(uint32_t nb, int layout) {
   if (layout == HIVE_LAYOUT_INLINE8)
@@ -400,6 +402,7 @@ inline hive_status_t alloc_view(TableView& v, uint32_t nb, int layout) {
   }
   v.counters = reinterpret_cast<unsigned long long*>(take(counter_bytes()));
   if (inl) v.pool_next = reinterpret_cast<uint32_t*>(v.counters + counter_words());
+  v.holes = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(v.counters + counter_words()) + pool_cursor_bytes());
   return HIVE_OK;
 }
 

@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup(TableView t, Src src, size
     }
   }
   uint32_t v;
-  const uint8_t r = lookup_one<P>(t, key, v);
+  const uint8_t r = lookup_one<P>(t, key, v, *t.holes == 0u);
   out.put(i, r, v);
 }
 
@@ -570,6 +570,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_multi(TableView t, Src src
     b0[k] = bucket_of<P>(0, key[k], t.nb);
   }
   {  // round 1: every key's H0
+    const bool full_hints = *t.holes == 0u;  // a miss in an H0 with an empty slot is final (note_hole)
     unsigned long long s[K][P::kS];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -581,6 +582,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_lookup_multi(TableView t, Src src
       if (m) {
         val[k] = lowest_match_val(s[k], m);
         res[k] = R_FOUND;
+        open[k] = false;
+      } else if (full_hints && empty_mask(s[k]) != 0u) {
         open[k] = false;
       }
     }

@@ -174,6 +174,58 @@ def test_insert_at_full_load_keeps_invariants(H, O, layout, policy):
     assert abs(int((~ok).sum()) - int((~placed(so)).sum())) <= max(64, 0.02 * slots)
 
 
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_lookups_after_erases_read_the_hints(H, O, layout):
+    """Lookups stop at an H0 with an empty slot only while no erase has
+    emptied a slot since the table was cleared or rehashed (every bucket with
+    a hint bit is then full).  At LF 0.97 many full buckets hold the hint
+    bits of displaced and stashed keys; erasing 40 % of the keys opens holes
+    in them, through a bulk erase and through a mixed batch.  Every remaining
+    key must still be found with its value, and after a clear and after a
+    resize the table works as before."""
+    width = SPB[layout]
+    print(f"[case] lookups after erases, layout {layout}", flush=True)
+    slots = 1 << 16
+    keys = W.distinct_keys(int(slots * 0.97), 61)
+    vals = W.values(keys.size, 62)
+    rng = np.random.default_rng(63)
+    for how in ("erase", "mixed"):
+        t = H.HiveTable(slots, layout=layout)
+        st = np_(t.insert(keys, vals))
+        ok = placed(st)
+        assert t.count_exact()[1] > 0  # a populated stash
+        gv, gs = t.lookup(keys)
+        assert np.array_equal(np_(gs) == 4, ok)
+        gone = rng.random(keys.size) < 0.4
+        if how == "erase":
+            es = np_(t.erase(keys[gone]))
+        else:
+            ty = np.full(int(gone.sum()), 2, np.uint8)
+            es = np_(t.mixed(ty, keys[gone])[1])
+        assert np.array_equal(es == 6, ok[gone])
+        gv, gs = t.lookup(keys)
+        gv, gs = np_(gv, np.uint32), np_(gs)
+        keep = ok & ~gone
+        assert np.array_equal(gs == 4, keep)
+        assert np.array_equal(gv[keep], vals[keep])
+        raw, chk = check_gpu_state(t, O, "Default2Hash")
+        cv, cs = chk.lookup(keys)
+        assert np.array_equal(gs, cs) and np.array_equal(gv, cv)
+        # a resize rebuilds the table: lookups as before
+        t.resize(slots * 2)
+        gv, gs = t.lookup(keys)
+        assert np.array_equal(np_(gs) == 4, keep)
+        assert np.array_equal(np_(gv, np.uint32)[keep], vals[keep])
+        # a clear empties it; refilled, the same keys are found again
+        t.clear()
+        st = np_(t.insert(keys, vals))
+        ok2 = placed(st)
+        gv, gs = t.lookup(keys)
+        assert np.array_equal(np_(gs) == 4, ok2)
+        assert np.array_equal(np_(gv, np.uint32)[ok2], vals[ok2])
+        t.close()
+
+
 @pytest.mark.parametrize("combine", [0, 1])
 def test_duplicates_in_one_batch_and_invalid_keys(H, O, combine):
     print(f"[case] duplicate keys / key 0, combine={combine}", flush=True)
