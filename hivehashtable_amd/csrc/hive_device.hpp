@@ -551,19 +551,25 @@ __device__ __forceinline__ bool is_dup(const uint32_t (&b)[P::NH], int i) {
 //   Stage B  (include/hive_hash_insert.cuh:271-310)  claim lowest empty H0 slot
 // Returns R_PENDING when the stash must be consulted, the canonical bucket is
 // full, or a CAS revealed a concurrent change; the op then takes slow_insert.
+// full_hints: the table has the full-hint property (see note_hole).  An H0
+// read with an empty slot then had no hint bits, and a claim that succeeds
+// finds H0 still not full, so no copy of the key can live elsewhere: the
+// hint word is read only for a full H0, in a second round.  Otherwise it is
+// read in the same round as the slots.
 template <class P>
-__device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key, uint32_t val) {
+__device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key, uint32_t val,
+                                               bool full_hints = false) {
   if (key == 0) return R_INVALID_KEY;
   const unsigned long long kv = kv_pack(key, val);
   uint32_t b[P::NH];
 #pragma unroll
   for (int i = 0; i < P::NH; ++i) b[i] = bucket_of<P>(i, key, t.nb);
-  uint32_t m0, e0, hint;
+  uint32_t m0, e0, hint = 0;
   unsigned long long cur0 = 0;  // the lowest matching slot's value
   {
     unsigned long long s0[P::kS];
     read_bucket<P>(bucket_ptr<P>(t, b[0]), s0);
-    hint = read_hdr<P>(t, b[0]).hint;  // the same round (in the in-line layout, the same line)
+    if (!full_hints) hint = read_hdr<P>(t, b[0]).hint;  // the same round (in the in-line layout, the same line)
     m0 = match_mask(s0, key);
     e0 = empty_mask(s0);
 #pragma unroll
@@ -571,6 +577,7 @@ __device__ __forceinline__ uint8_t fast_insert(const TableView& t, uint32_t key,
       if (m0 && q == __builtin_ctz(m0)) cur0 = s0[q];
   }
   if (m0) return update_slot(bucket_ptr<P>(t, b[0]) + __builtin_ctz(m0), cur0, kv) ? R_UPDATED : R_PENDING;
+  if (full_hints && e0 == 0u) hint = read_hdr<P>(t, b[0]).hint;
   if (hint & ovf_bit_of<P>(key)) {
 #pragma unroll
     for (int i = 1; i < P::NH; ++i) {

@@ -215,13 +215,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
   }
   const size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x;
   const size_t i = base + threadIdx.x;
+  const bool full_hints = *t.holes == 0u;
   int dtab = 0, dstash = 0, dslow = 0;
   // key and value are requested together with the op type (one round trip)
   const uint32_t key = i < n ? src.key(i) : 0u;
   const uint32_t val = i < n ? src.val(i, key) : 0u;
   if constexpr (Inline) {
     if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-      uint8_t r = fast_insert<P>(t, key, val);
+      uint8_t r = fast_insert<P>(t, key, val, full_hints);
       if (r == R_PENDING) {
         r = slow_insert<P>(t, key, val);
         dslow = 1;
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert(TableView t, Src src, size
     }
     __syncthreads();
     if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-      const uint8_t r = fast_insert<P>(t, key, val);
+      const uint8_t r = fast_insert<P>(t, key, val, full_hints);
       if (r == R_PENDING) {
         queue[atomicAdd(&qn, 1u)] = threadIdx.x;
       } else {
@@ -293,11 +294,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_fast(TableView t, Src src,
   if (block_past_list(src)) return;
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int dtab = 0;
+  const bool full_hints = *t.holes == 0u;
   const uint32_t key = i < n ? src.key(i) : 0u;  // requested together with the op type
   const uint32_t val = i < n ? src.val(i, key) : 0u;
   bool pend = false;
   if (i < n && (phase < 0 || src.type(i, OP_INSERT) == OP_INSERT)) {
-    const uint8_t r = fast_insert<P>(t, key, val);
+    const uint8_t r = fast_insert<P>(t, key, val, full_hints);
     if (r == R_PENDING) {
       pend = true;
     } else {
