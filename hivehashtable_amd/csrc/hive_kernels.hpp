@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_insert_combined(TableView t, Src 
   int dtab = 0, dstash = 0, dslow = 0;
   uint8_t r = R_FAILED;
   if (mine && leader == threadIdx.x) {
-    r = fast_insert<P>(t, key, val);
+    r = fast_insert<P>(t, key, val, *t.holes == 0u);
     if (r == R_PENDING) {
       r = slow_insert<P>(t, key, val);
       dslow = 1;
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
         for (uint32_t j = 0; j < cnt; ++j) {
           const unsigned long long e = sb[j];
           if (e == 0ull) continue;
-          uint8_t r = fast_insert<P>(dst, kv_key(e), kv_val(e));
+          uint8_t r = fast_insert<P>(dst, kv_key(e), kv_val(e), *dst.holes == 0u);
           if (r == R_PENDING) r = slow_insert<P>(dst, kv_key(e), kv_val(e));
           dtab += (r == R_INSERTED);
           dstash += (r == R_STASHED);
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_rehash(TableView dst, TableView s
     }
   }
   if (kv != 0ull) {
-    const uint8_t r = fast_insert<P>(dst, kv_key(kv), kv_val(kv));
+    const uint8_t r = fast_insert<P>(dst, kv_key(kv), kv_val(kv), *dst.holes == 0u);
     if (r == R_PENDING) queue[atomicAdd(&qn, 1u)] = kv;
     else dtab += (r == R_INSERTED);
   }

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kStreamBlock) void k_stream(TableView t, StreamArgs
           dstash -= rs;
         }
       } else if (ty == OP_INSERT) {
-        uint8_t r = fast_insert<P>(t, key, val);
+        uint8_t r = fast_insert<P>(t, key, val);  // no full-hint rule: this kernel's own delete phases may open holes
         if (r == R_PENDING) {
           if constexpr (Inline) {
             r = slow_insert<P>(t, key, val);

@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kWinBlock) void k_chain_apply(TableView t, ChainArg
     }
     if (wt != kApplyInsert) continue;
     const uint32_t i = a.wlo + j, key = a.keys[i], val = op_value(a, i);
-    uint8_t r = fast_insert<P>(t, key, val);
+    uint8_t r = fast_insert<P>(t, key, val, *t.holes == 0u);  // the erase pass ran in an earlier launch
     if (r == R_PENDING) {
       r = slow_insert<P>(t, key, val);
       ++dslow;
