@@ -574,22 +574,31 @@ def cpu_allotment():
                  "job_cpu_share": JOB_CPU_SHARE, "cpu_model": model}
 
 
-def cpu_baseline(keys, vals, lk, threads, host, gpu_found, gpu_vals, policy, nb, runs=5):
+def cpu_baseline(keys, vals, lk, threads, host, gpu_found, gpu_vals, gpu_placed, policy, nb, runs=5):
     """The multithreaded CPU path on one full step (clear, bulk insert, bulk
     lookup) of the same workload, median of `runs` timed runs after one
-    untimed warm-up run."""
+    untimed warm-up run.  Its lookups are compared with the GPU's on every
+    key placed on both sides: which keys a concurrent insert leaves failed
+    (a full canonical stash) is a placement outcome of each path."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
     c = O.CpuTable(nb, policy, threads=threads)
     t_ins, t_look = [], []
     agree = True
+    cmp = {}
     for r in range(runs + 1):
         c.clear()
         st, ti = c.insert(keys, vals)
         v, s, tl = c.lookup(lk)
         if r == 0:
-            agree = bool(np.array_equal(s == O.R_FOUND, gpu_found) and np.array_equal(v, gpu_vals))
+            cpu_placed = np.isin(st, (O.R_INSERTED, O.R_STASHED))
+            failed = keys[~(cpu_placed & gpu_placed)]
+            both = ~np.isin(lk, failed) if failed.size else np.ones(lk.size, bool)
+            agree = bool(np.array_equal((s == O.R_FOUND)[both], gpu_found[both]) and
+                         np.array_equal(v[both], gpu_vals[both]))
+            cmp = {"cpu_failed_inserts": int((~cpu_placed).sum()), "gpu_failed_inserts": int((~gpu_placed).sum()),
+                   "lookups_compared": int(both.sum())}
             continue
         t_ins.append(ti)
         t_look.append(tl)
@@ -608,6 +617,7 @@ def cpu_baseline(keys, vals, lk, threads, host, gpu_found, gpu_vals, policy, nb,
         "seconds_per_run": round(ti + tl, 3),
         "host": host,
         "lookups_agree_with_gpu": agree,
+        **cmp,
     }
 
 
@@ -843,7 +853,7 @@ def main():
         threads, host = cpu_allotment()
         threads = a.cpu_threads or threads
         log(f"cpu baseline: {threads} threads ({host}), one full step, median of 5")
-        cpu = cpu_baseline(keys, vals, lk, threads, host, found, lv, a.policy, slots // 8)
+        cpu = cpu_baseline(keys, vals, lk, threads, host, found, lv, placed, a.policy, slots // 8)
         log(f"cpu: {cpu['value']} Mops/s, {cpu['seconds_per_run']} s per run, agree={cpu['lookups_agree_with_gpu']}")
 
     out = {
