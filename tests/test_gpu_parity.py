@@ -175,6 +175,32 @@ def test_insert_at_full_load_keeps_invariants(H, O, layout, policy):
 
 
 @pytest.mark.parametrize("layout", LAYOUT_NAMES)
+def test_imported_table_with_holes_is_probed_in_full(H, O, layout):
+    """An imported state may hold hint bits on buckets with empty slots (here
+    the oracle's table at LF 0.97 after 40 % of its keys were erased), so an
+    import turns the lookups' stop at an H0 with room off: every lookup must
+    equal the oracle's."""
+    width = SPB[layout]
+    print(f"[case] imported table with holes, layout {layout}", flush=True)
+    nb = 1 << 12
+    keys = W.distinct_keys(int(nb * width * 0.97), 16)
+    vals = W.values(keys.size, 17)
+    o = O.Oracle(nb, "Default2Hash", slots_per_bucket=width)
+    o.insert(keys, vals)
+    rng = np.random.default_rng(18)
+    o.erase(rng.choice(keys, int(keys.size * 0.4), replace=False))
+    raw = o.export()
+    t = H.HiveTable(nb * width, "Default2Hash", layout=layout)
+    t.import_raw(raw.slots, raw.stash, raw.stash_count)
+    probe = np.concatenate([keys, W.distinct_keys(5000, 19, exclude=keys)])
+    ov, os_ = o.lookup(probe)
+    for kpl in (1, 2):
+        t.set_launch(1, 256, keys_per_lane=kpl)
+        gv, gs = t.lookup(probe)
+        assert np.array_equal(np_(gs), os_) and np.array_equal(np_(gv, np.uint32), ov), kpl
+
+
+@pytest.mark.parametrize("layout", LAYOUT_NAMES)
 def test_lookups_after_erases_read_the_hints(H, O, layout):
     """Lookups stop at an H0 with an empty slot only while no erase has
     emptied a slot since the table was cleared or rehashed (every bucket with
